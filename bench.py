@@ -1,0 +1,177 @@
+#!/usr/bin/env python3
+"""Fock-build benchmark: (H2O)_64, 6-31G (832 functions), one GPU.
+
+A step is one Fock build (J and K for a fixed density) through
+``rchem_jk_device`` with D, J, K resident in HBM.  Prints one JSON line:
+
+  value        unique contracted shell quartets per second (after Schwarz
+               screening at 1e-12 and 8-fold symmetry) of the whole build;
+  ms_per_step  Fock-build milliseconds;
+  roofline     the dominant (la lb|lc ld) class kernel: algorithmic FP64 flops
+               per launch (generator operation counts, FMA = 2) over its
+               average launch time from HIP events on the library's stream;
+  cpu_baseline the reference algorithm (rchem's JK_direct loop over the
+               reference's own cints coulomb_repulsion, oracle/_ref) on one
+               host core over a bounded random sample of function quartets,
+               expressed as the same Fock build's quartets per second.
+
+Options: --steps K --warmup W --gpus N (N must be 1) --precision fp64|mixed
+         --no-cpu-baseline --cpu-seconds S
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0
+
+
+def core_guess_density(S, H, nocc):
+    lam, U = np.linalg.eigh(S)
+    X = U @ np.diag(lam ** -0.5) @ U.T
+    _, Cp = np.linalg.eigh(X.T @ H @ X)
+    C = X @ Cp
+    return C[:, :nocc] @ C[:, :nocc].T
+
+
+def cpu_baseline(Z, xyz, D, gpu_quartets, seconds):
+    """Reference JK_direct inner loop on one core over random function quartets."""
+    import oracle as O
+    bse = O.load_bse_json(os.path.join(ROOT, "rchem_amd", "data", "6-31g.json"))
+    B = O.cgtos_from_bse(bse, Z, xyz)
+    addr = O.cints_coulomb_repulsion_address()
+    kind = "reference" if addr else "port"
+    rng = np.random.default_rng(1234)
+    done, t_total, batch = 0, 0.0, 256
+    while t_total < seconds:
+        q = rng.integers(0, B.n, (batch, 4))
+        t0 = time.perf_counter()
+        O.jk_direct_sample(B, D, q, boys=O.BOYS_COMPAT, cints_fn=addr)
+        t_total += time.perf_counter() - t0
+        done += batch
+        batch = min(batch * 2, 8192)
+    rate = done / t_total                       # function quartets / s (each: J and K ERI)
+    fock_s = B.n ** 4 / rate                    # JK_direct visits n^4 function quartets
+    return {
+        "value": gpu_quartets / fock_s,
+        "unit": "shell quartets/s (same Fock build)",
+        "cores": 1,
+        "kind": kind,
+        "sample": (f"{done} random (mu,nu,la,si) function quartets of the (H2O)_64 6-31G "
+                   f"JK_direct loop in {t_total:.1f} s ({rate:.0f}/s, 2 contracted ERIs each via "
+                   f"{'the reference cints.c coulomb_repulsion' if addr else 'the C restatement'}); "
+                   f"extrapolated Fock build {fock_s:.3g} s"),
+        "fock_build_s_extrapolated": fock_s,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the Fock build is not sharded across GPUs in this "
+              "build (see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from rchem_amd import basis as rb
+    from rchem_amd._lib import (PREC_FP64, PREC_MIXED, ClassTiming, default_opts, lib)
+    from rchem_amd.inputs import nocc_of, water_cluster
+
+    Z, xyz = water_cluster(dims=(4, 4, 4), seed=64)
+    ctx = rb.Context(0)
+    B = rb.Basis.new(Z, xyz, "6-31G", ctx=ctx)
+    n = B.nbf
+    S, T, V = rb.one_electron(B, xyz, Z)
+    D = core_guess_density(S, T + V, nocc_of(Z))
+    prec = PREC_FP64 if args.precision == "fp64" else PREC_MIXED
+    opts = default_opts(precision=prec, schwarz_thresh=1e-12)
+
+    dev = torch.device("cuda:0")
+    dD = torch.from_numpy(D).to(dev)
+    dJ = torch.zeros((n, n), dtype=torch.float64, device=dev)
+    dK = torch.zeros((n, n), dtype=torch.float64, device=dev)
+    L = lib()
+
+    def step():
+        ctx.check(L.rchem_jk_device(ctx.h, ctypes.c_void_p(dD.data_ptr()),
+                                    ctypes.c_void_p(dJ.data_ptr()),
+                                    ctypes.c_void_p(dK.data_ptr()), ctypes.byref(opts)))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.check(L.rchem_reset_class_timings(ctx.h))
+    ctx.check(L.rchem_set_profiling(ctx.h, 1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.check(L.rchem_set_profiling(ctx.h, 0))
+    st = rb.jk_last_stats(B)
+    timings = (ClassTiming * 64)()
+    cnt = ctypes.c_int()
+    ctx.check(L.rchem_jk_class_timings(ctx.h, timings, 64, ctypes.byref(cnt)))
+    classes = []
+    for i in range(cnt.value):
+        t = timings[i]
+        avg = t.ms / max(t.launches, 1)
+        classes.append({"class": f"({t.la}{t.lb}|{t.lc}{t.ld})", "avg_ms": avg,
+                        "quartets": t.quartets, "prim_quartets": t.primitive_quartets,
+                        "flops": t.flops,
+                        "tflops": t.flops / (avg * 1e-3) / 1e12 if avg > 0 else 0.0})
+    dom = max(classes, key=lambda c: c["avg_ms"])
+    ms = elapsed / args.steps * 1e3
+    value = st.quartets / (ms * 1e-3)
+    total_tflops = st.flops / (ms * 1e-3) / 1e12
+    out = {
+        "metric": "Fock-build ms & ERI quartets/s, mixed vs FP64 on GPU at |dE|<1e-6 Eh",
+        "value": value,
+        "unit": "ERI shell quartets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == PREC_FP64 else "f64+f32",
+        "data": "synthetic",
+        "config": {"workload": "(H2O)_64 6-31G Fock build (J and K), seed 64, 4x4x4 lattice "
+                               "3.1 A, Schwarz 1e-12", "nbf": n, "shells": len(B.shells()[1]),
+                   "precision": args.precision},
+        "quartets": st.quartets,
+        "primitive_quartets": st.primitive_quartets,
+        "integrals": st.integrals,
+        "fock_tflops": total_tflops,
+        "roofline": {"bound": "mfma", "achieved": dom["tflops"], "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": dom["tflops"] / FP64_PEAK_TFLOPS,
+                     "traffic": None, "kernel": dom["class"],
+                     "note": "FP64 VALU kernel; MI355X FP64 vector peak = FP64 matrix peak"},
+        "classes": classes,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(Z, xyz, D, st.quartets, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,190 @@
+/*
+ * rchem_gpu.h -- C ABI of the MI355X (gfx950) restricted-Hartree-Fock hot path.
+ *
+ * Drop-in boundary for berquist/rchem.  Every entry point below names the
+ * reference interface it replaces (file:line in the rchem tree).  The ABI is
+ * plain C: opaque context, plain pointers and sizes, int status codes.
+ *
+ * Conventions (fixed by rchem, reproduced exactly):
+ *   - chemists' notation (mu nu|la si);
+ *   - basis functions ordered atom -> basis-set shell -> l in the shell's
+ *     angular_momentum list -> Cartesian component in get_ijk_list order
+ *     (basis.rs:186-205, shell.rs:1-12), e.g. d = xx, xy, xz, yy, yz, zz;
+ *   - primitive norm per Cartesian component (basis.rs:140-149), contraction
+ *     coefficients used raw, no renormalisation (basis.rs:201);
+ *   - matrices are n x n, row-major, double precision;
+ *   - D = C_occ C_occ^T (no factor 2), F = H + 2J - K, E_elec = sum D o (H+F)
+ *     (rchem.rs:101-127);
+ *   - coordinates are used as given (bohr), no unit conversion (rchem.rs:33-35).
+ *
+ * Threading: a context owns one HIP stream on one device; calls on one
+ * context must not overlap.  Distinct contexts are independent.
+ * Memory: the library owns its device memory; callers own host buffers.
+ * Errors: every function returns RCHEM_OK or an error code; the message of the
+ * last failure on a context is rchem_last_error(ctx).  Nothing aborts.
+ */
+#ifndef RCHEM_GPU_H
+#define RCHEM_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rchem_ctx rchem_ctx;
+
+enum rchem_status {
+  RCHEM_OK = 0,
+  RCHEM_ERR_INVALID = 1,       /* bad argument (null pointer, size, unknown name) */
+  RCHEM_ERR_HIP = 2,           /* HIP runtime or library failure */
+  RCHEM_ERR_NO_DEVICE = 3,     /* no usable gfx950 device */
+  RCHEM_ERR_UNSUPPORTED = 4,   /* angular momentum or option not compiled in */
+  RCHEM_ERR_NO_BASIS = 5,      /* operation needs a basis; none uploaded */
+  RCHEM_ERR_NOT_CONVERGED = 6, /* SCF hit max_iter (results still written) */
+  RCHEM_ERR_NONFINITE = 7,     /* NaN/Inf detected in an input or result */
+  RCHEM_ERR_ALLOC = 8          /* device or host allocation failed */
+};
+
+/* Boys-function evaluation used for electron-repulsion integrals. */
+enum rchem_boys_mode {
+  RCHEM_BOYS_EXACT = 0,  /* accurate F_m(T) (production) */
+  RCHEM_BOYS_COMPAT = 1  /* the reference's NR incomplete gamma, cints.c:302-373 */
+};
+
+enum rchem_precision {
+  RCHEM_PREC_FP64 = 0,   /* every quartet in FP64 */
+  RCHEM_PREC_MIXED = 1   /* quartets with Q_ab Q_cd max|D| < mixed_thresh in FP32 */
+};
+
+typedef struct rchem_opts {
+  int boys_mode;         /* rchem_boys_mode, default RCHEM_BOYS_EXACT */
+  int precision;         /* rchem_precision, default RCHEM_PREC_FP64 */
+  double schwarz_thresh; /* skip quartets with Q_ab Q_cd below this; default 1e-12, 0 keeps all */
+  double mixed_thresh;   /* FP32 cut for RCHEM_PREC_MIXED; default 1e-7 */
+} rchem_opts;
+
+/* Fills *o with the defaults above. */
+int rchem_opts_default(rchem_opts *o);
+
+/* ---- context -------------------------------------------------------------- */
+
+/* Creates a context on HIP device `device` (index among visible devices).
+ * No counterpart in rchem (single-threaded host code); the context holds what
+ * rchem keeps in Basis (basis.rs:176-179) plus device state. */
+int rchem_ctx_create(int device, rchem_ctx **out);
+int rchem_ctx_destroy(rchem_ctx *ctx);
+/* Message of the last failed call on ctx ("" if none).  Valid until the next
+ * call on ctx.  With ctx == NULL, the last rchem_ctx_create failure. */
+const char *rchem_last_error(const rchem_ctx *ctx);
+
+/* ---- basis ---------------------------------------------------------------- */
+
+/* Basis::new(atomnos, atomcoords, basis_set_name) (basis.rs:182-212), with the
+ * basis-set data embedded (STO-3G, 6-31G for H, C, N, O) instead of fetched
+ * from basis_set_exchange (basis.rs:87-106).  xyz is natom x 3 (bohr). */
+int rchem_basis_from_atoms(rchem_ctx *ctx, int natom, const int *Z, const double *xyz,
+                           const char *basis_name);
+
+/* The same from explicit shells: shell s has centre xyz[3s..3s+2], angular
+ * momentum l[s] (0..2), nprim[s] primitives whose exponents and raw contraction
+ * coefficients are exps/coefs[off_s .. off_s+nprim[s]) with off_s the running
+ * sum of nprim.  Functions are numbered shell by shell, components in
+ * get_ijk_list order. */
+int rchem_basis_upload(rchem_ctx *ctx, int nshell, const double *xyz, const int *l,
+                       const int *nprim, const double *exps, const double *coefs);
+
+/* Number of Cartesian basis functions (rchem: basis_set.cgtos.len()). */
+int rchem_basis_nbf(const rchem_ctx *ctx, int *nbf);
+/* Number of shells and total primitives of the current basis. */
+int rchem_basis_dims(const rchem_ctx *ctx, int *nshell, int *nprim_total);
+/* Copies the current shells out (arrays sized by rchem_basis_dims). */
+int rchem_basis_shells(const rchem_ctx *ctx, double *xyz, int *l, int *nprim, double *exps,
+                       double *coefs);
+
+/* ---- one-electron integrals --------------------------------------------- */
+
+/* S, T and V = sum_C Z_C <mu|-1/r_C|nu> (basis.rs:234, :273, :316 and
+ * os86.rs:629-773), n x n each, host pointers; any of S/T/V may be NULL. */
+int rchem_one_electron(rchem_ctx *ctx, int natom, const double *Z, const double *xyz,
+                       double *S, double *T, double *V);
+
+/* ---- two-electron integrals --------------------------------------------- */
+
+/* JK_direct(J, K, basis, D) (basis.rs:383-428): J_mn = sum (mn|ls) D_ls,
+ * K_mn = sum (ml|ns) D_ls; J and K are overwritten.  D must be symmetric.
+ * Host pointers, n x n.  opts may be NULL (defaults).  The reference loops over
+ * every primitive quartet; this uses Schwarz screening and 8-fold symmetry. */
+int rchem_jk(rchem_ctx *ctx, const double *D, double *J, double *K, const rchem_opts *opts);
+
+/* Same with device pointers (HBM-resident D, J, K; n x n doubles each).
+ * Runs on the context's stream and returns without synchronising. */
+int rchem_jk_device(rchem_ctx *ctx, const double *dD, double *dJ, double *dK,
+                    const rchem_opts *opts);
+
+/* build_I(basis) (basis.rs:430-460): the full n^4 tensor [mu,nu,la,si]
+ * row-major into host memory (n^4 doubles).  No screening. */
+int rchem_eri_tensor(rchem_ctx *ctx, double *I, const rchem_opts *opts);
+
+/* Schwarz bounds sqrt(max_comp |(ab|ab)|) for every shell pair a >= b, as an
+ * nshell x nshell lower-triangular-filled matrix (host).  No rchem
+ * counterpart (rchem does not screen). */
+int rchem_schwarz(rchem_ctx *ctx, double *Q, const rchem_opts *opts);
+
+/* Statistics of the last rchem_jk / rchem_jk_device call. */
+typedef struct rchem_jk_stats {
+  double quartets;           /* unique contracted shell quartets computed */
+  double primitive_quartets; /* primitive quartets computed */
+  double integrals;          /* contracted Cartesian integrals computed */
+  double flops;              /* algorithmic FP operations (FMA = 2) */
+  double quartets_fp32;      /* of `quartets`, computed in FP32 */
+} rchem_jk_stats;
+int rchem_jk_last_stats(const rchem_ctx *ctx, rchem_jk_stats *st);
+
+/* Per-class kernel timing.  With profiling on, every rchem_jk / rchem_jk_device
+ * call records a HIP event pair around each (la lb|lc ld) class's launch on the
+ * context stream; rchem_jk_class_timings sums them since the last reset.
+ * No rchem counterpart (rchem has no timing). */
+typedef struct rchem_class_timing {
+  int la, lb, lc, ld;
+  int launches;              /* launches summed into ms */
+  double ms;                 /* summed kernel time */
+  double quartets;           /* per launch */
+  double primitive_quartets; /* per launch */
+  double flops;              /* per launch, algorithmic (FMA = 2) */
+} rchem_class_timing;
+int rchem_set_profiling(rchem_ctx *ctx, int enable);
+int rchem_reset_class_timings(rchem_ctx *ctx);
+/* Writes up to max entries; *count receives the number of classes timed. */
+int rchem_jk_class_timings(rchem_ctx *ctx, rchem_class_timing *out, int max, int *count);
+
+/* ---- SCF ------------------------------------------------------------------ */
+
+typedef struct rchem_scf_opts {
+  int nocc;            /* doubly occupied orbitals (rchem.rs:59,78) */
+  int max_iter;        /* default 1024 (rchem.rs:65) */
+  double thresh_e;     /* |dE_elec| convergence, default 1e-11 (rchem.rs:63) */
+  rchem_opts jk;       /* integral options */
+  int verbose;         /* 1: print rchem's iteration line (rchem.rs:83) */
+} rchem_scf_opts;
+
+typedef struct rchem_scf_result {
+  double e_elec;   /* sum D o (H + F) (rchem.rs:106-108, :80) */
+  double e_nuc;    /* sum Z_A Z_B / R_AB (not printed by rchem) */
+  double e_total;
+  int iterations;
+  int converged;
+} rchem_scf_result;
+
+int rchem_scf_opts_default(rchem_scf_opts *o);
+
+/* rchem's main SCF (rchem.rs:40-90) on the current basis for the given
+ * nuclei: Loewdin X, core-Hamiltonian guess, plain iterations until
+ * |dE_elec| < thresh_e.  D_out (n x n, may be NULL) receives the final
+ * density.  Returns RCHEM_ERR_NOT_CONVERGED if max_iter is reached. */
+int rchem_scf(rchem_ctx *ctx, int natom, const double *Z, const double *xyz,
+              const rchem_scf_opts *opts, rchem_scf_result *res, double *D_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RCHEM_GPU_H */

@@ -1,0 +1,290 @@
+#!/usr/bin/env python3
+"""Generates the per-class Head-Gordon--Pople recurrences as straight-line HIP code.
+
+For every shell-quartet class (la lb|lc ld) with la >= lb, lc >= ld and
+pair-class(la,lb) >= pair-class(lc,ld), l <= LMAX, it emits
+
+  * ``vrr``: the Obara--Saika/HGP vertical recurrence for one primitive
+    quartet, building [e0|f0]^(0) for l_e in [la, la+lb], l_f in [lc, lc+ld]
+    from G_m = K_ab K_cd / sqrt(zeta+eta) * F_m(T) and adding them into the
+    contracted accumulators ``acc`` (the same recurrences as the reference's
+    chgp.c:412-586, with every intermediate computed once);
+  * ``hrr``: the horizontal recurrence on the contracted accumulators,
+    (a, b+1_i| = (a+1_i, b| + AB_i (a b|, then the same on the ket with CD
+    (chgp.c:44-111), producing (ab|cd) in rchem's Cartesian order
+    (shell.rs:1-12) times the per-component normalisation of basis.rs:140-149.
+
+Intermediates are named locals, so after inlining they live in registers.
+The generator also counts floating-point operations per class (add/mul = 1,
+FMA = 2) for the roofline model, written to ``flops_per_class.json``.
+
+Usage: gen_eri.py OUT_INC OUT_JSON [LMAX]
+"""
+import json
+import math
+import sys
+
+AX = "xyz"
+
+
+def cart(l):
+    """Cartesian components in rchem's order (shell.rs:1-12)."""
+    out = []
+    for a in range(1, l + 2):
+        for b in range(1, a + 1):
+            out.append((l + 1 - a, a - b, b - 1))
+    return out
+
+
+def ncart(l):
+    return (l + 1) * (l + 2) // 2
+
+
+def dec(t, i, k=1):
+    t = list(t)
+    t[i] -= k
+    return tuple(t)
+
+
+def inc(t, i):
+    t = list(t)
+    t[i] += 1
+    return tuple(t)
+
+
+def tname(t):
+    return "".join(str(x) for x in t)
+
+
+def comp_norm(t):
+    """1/sqrt((2l-1)!!(2m-1)!!(2n-1)!!): the component-dependent factor of the
+    primitive norm of basis.rs:140-149 (the exponent-dependent factor is folded
+    into the contraction coefficients on the host)."""
+    def f2(n):
+        r = 1
+        while n > 1:
+            r *= n
+            n -= 2
+        return r
+    return 1.0 / math.sqrt(f2(2 * t[0] - 1) * f2(2 * t[1] - 1) * f2(2 * t[2] - 1))
+
+
+class Emitter:
+    def __init__(self):
+        self.lines = []
+        self.flops = 0
+        self.consts = {}
+
+    def const(self, base, k):
+        if k == 1:
+            return base
+        name = f"{base}_{k}"
+        if name not in self.consts:
+            self.consts[name] = f"const R {name} = R({k}) * {base};"
+            self.flops += 1
+        return name
+
+
+class VRR:
+    """Memoised VRR for one class; nodes are (e, f, m)."""
+
+    def __init__(self, em):
+        self.em = em
+        self.memo = {}
+
+    def node(self, e, f, m):
+        key = (e, f, m)
+        if key in self.memo:
+            return self.memo[key]
+        em = self.em
+        if sum(f) == 0:
+            if sum(e) == 0:
+                name = f"G[{m}]"
+                self.memo[key] = name
+                return name
+            # bra step on A: pick the axis that avoids the (e_i - 1) term if possible
+            i = min((k for k in range(3) if e[k] > 0), key=lambda k: (e[k] >= 2, k))
+            e1 = dec(e, i)
+            t1, t2 = self.node(e1, f, m), self.node(e1, f, m + 1)
+            expr = f"PA{AX[i]} * {t1} + WP{AX[i]} * {t2}"
+            fl = 3
+            if e1[i] > 0:
+                e2 = dec(e1, i)
+                t3, t4 = self.node(e2, f, m), self.node(e2, f, m + 1)
+                c = em.const("oo2z", e1[i])
+                expr += f" + {c} * ({t3} - roz * {t4})"
+                fl += 4
+        else:
+            # ket step on C: avoid the (f_i - 1) and electron-transfer terms when possible
+            i = min((k for k in range(3) if f[k] > 0),
+                    key=lambda k: (2 * (f[k] >= 2) + (e[k] > 0), k))
+            f1 = dec(f, i)
+            t1, t2 = self.node(e, f1, m), self.node(e, f1, m + 1)
+            expr = f"QC{AX[i]} * {t1} + WQ{AX[i]} * {t2}"
+            fl = 3
+            if f1[i] > 0:
+                f2 = dec(f1, i)
+                t3, t4 = self.node(e, f2, m), self.node(e, f2, m + 1)
+                c = em.const("oo2e", f1[i])
+                expr += f" + {c} * ({t3} - roe * {t4})"
+                fl += 4
+            if e[i] > 0:
+                e1 = dec(e, i)
+                t5 = self.node(e1, f1, m + 1)
+                c = em.const("oo2ze", e[i])
+                expr += f" + {c} * {t5}"
+                fl += 2
+        name = f"v{tname(e)}_{tname(f)}_{m}"
+        em.lines.append(f"const R {name} = {expr};")
+        em.flops += fl
+        self.memo[key] = name
+        return name
+
+
+def gen_class(la, lb, lc, ld):
+    E = [t for l in range(la, la + lb + 1) for t in cart(l)]
+    F = [t for l in range(lc, lc + ld + 1) for t in cart(l)]
+    acc_index = {}
+    for ie, e in enumerate(E):
+        for jf, f in enumerate(F):
+            acc_index[(e, f)] = ie * len(F) + jf
+    nacc = len(E) * len(F)
+    L = la + lb + lc + ld
+
+    # ---- VRR
+    em = Emitter()
+    v = VRR(em)
+    acc_lines = []
+    for e in E:
+        for f in F:
+            name = v.node(e, f, 0)
+            acc_lines.append(f"acc[{acc_index[(e, f)]}] += {name};")
+            em.flops += 1
+    vrr_flops = em.flops
+    vrr_body = list(em.consts.values()) + em.lines + acc_lines
+
+    # ---- HRR (bra, then ket), memoised
+    hl = []
+    hmemo = {}
+    hflops = [0]
+
+    def bra(a, b, f):
+        key = ("b", a, b, f)
+        if key in hmemo:
+            return hmemo[key]
+        if sum(b) == 0:
+            name = f"acc[{acc_index[(a, f)]}]"
+        else:
+            i = next(k for k in range(3) if b[k] > 0)
+            b1 = dec(b, i)
+            t1 = bra(inc(a, i), b1, f)
+            t2 = bra(a, b1, f)
+            name = f"hb{tname(a)}_{tname(b)}_{tname(f)}"
+            hl.append(f"const R {name} = {t1} + AB{AX[i]} * {t2};")
+            hflops[0] += 2
+        hmemo[key] = name
+        return name
+
+    def ket(a, b, c, d):
+        key = ("k", a, b, c, d)
+        if key in hmemo:
+            return hmemo[key]
+        if sum(d) == 0:
+            name = bra(a, b, c)
+        else:
+            i = next(k for k in range(3) if d[k] > 0)
+            d1 = dec(d, i)
+            t1 = ket(a, b, inc(c, i), d1)
+            t2 = ket(a, b, c, d1)
+            name = f"hk{tname(a)}_{tname(b)}_{tname(c)}_{tname(d)}"
+            hl.append(f"const R {name} = {t1} + CD{AX[i]} * {t2};")
+            hflops[0] += 2
+        hmemo[key] = name
+        return name
+
+    out_lines = []
+    idx = 0
+    for a in cart(la):
+        for b in cart(lb):
+            for c in cart(lc):
+                for d in cart(ld):
+                    name = ket(a, b, c, d)
+                    cn = comp_norm(a) * comp_norm(b) * comp_norm(c) * comp_norm(d)
+                    if abs(cn - 1.0) > 0:
+                        out_lines.append(f"out[{idx}] = R({cn!r}) * {name};")
+                        hflops[0] += 1
+                    else:
+                        out_lines.append(f"out[{idx}] = {name};")
+                    idx += 1
+    nout = idx
+    return dict(L=L, nacc=nacc, nout=nout, vrr=vrr_body, hrr=hl + out_lines,
+                vrr_flops=vrr_flops, hrr_flops=hflops[0])
+
+
+def pair_class(l1, l2):
+    return l1 * (l1 + 1) // 2 + l2
+
+
+def classes(lmax):
+    pairs = [(l1, l2) for l1 in range(lmax + 1) for l2 in range(l1 + 1)]
+    pairs.sort(key=lambda p: pair_class(*p))
+    out = []
+    for bi, (la, lb) in enumerate(pairs):
+        for (lc, ld) in pairs[:bi + 1]:
+            out.append((la, lb, lc, ld))
+    return out
+
+
+def main():
+    out_inc, out_json = sys.argv[1], sys.argv[2]
+    lmax = int(sys.argv[3]) if len(sys.argv) > 3 else 2
+    parts = ["// Generated by gen_eri.py -- do not edit.", ""]
+    flops = {}
+    for (la, lb, lc, ld) in classes(lmax):
+        g = gen_class(la, lb, lc, ld)
+        key = f"{la}{lb}{lc}{ld}"
+        flops[key] = {"vrr_per_primitive_quartet": g["vrr_flops"],
+                      "hrr_per_contracted_quartet": g["hrr_flops"],
+                      "nacc": g["nacc"], "nout": g["nout"], "L": g["L"]}
+        parts.append(f"template <> struct EriClass<{la}, {lb}, {lc}, {ld}> {{")
+        parts.append(f"  static constexpr int L = {g['L']};")
+        parts.append(f"  static constexpr int NACC = {g['nacc']};")
+        parts.append(f"  static constexpr int NOUT = {g['nout']};")
+        parts.append("  template <typename R>")
+        parts.append("  __device__ static __forceinline__ void vrr(const R* __restrict__ G,"
+                     " R PAx, R PAy, R PAz, R WPx, R WPy, R WPz,")
+        parts.append("      R QCx, R QCy, R QCz, R WQx, R WQy, R WQz,"
+                     " R oo2z, R oo2e, R oo2ze, R roz, R roe, R* __restrict__ acc) {")
+        parts.append("    (void)PAx; (void)PAy; (void)PAz; (void)WPx; (void)WPy; (void)WPz;")
+        parts.append("    (void)QCx; (void)QCy; (void)QCz; (void)WQx; (void)WQy; (void)WQz;")
+        parts.append("    (void)oo2z; (void)oo2e; (void)oo2ze; (void)roz; (void)roe;")
+        parts.extend("    " + s for s in g["vrr"])
+        parts.append("  }")
+        parts.append("  template <typename R>")
+        parts.append("  __device__ static __forceinline__ void hrr(const R* __restrict__ acc,"
+                     " R ABx, R ABy, R ABz, R CDx, R CDy, R CDz, R* __restrict__ out) {")
+        parts.append("    (void)ABx; (void)ABy; (void)ABz; (void)CDx; (void)CDy; (void)CDz;")
+        parts.extend("    " + s for s in g["hrr"])
+        parts.append("  }")
+        parts.append("};")
+        parts.append("")
+    sp = [c for c in classes(lmax) if max(c) <= 1]
+    dd = [c for c in classes(lmax) if max(c) > 1]
+    parts.append("#define RCHEM_ERI_CLASSES_SP(X) " + " ".join(f"X({a}, {b}, {c}, {d})" for a, b, c, d in sp))
+    parts.append("#define RCHEM_ERI_CLASSES_D(X) " + " ".join(f"X({a}, {b}, {c}, {d})" for a, b, c, d in dd))
+    parts.append("")
+    with open(out_inc, "w") as fh:
+        fh.write("\n".join(parts))
+    with open(out_json, "w") as fh:
+        json.dump(flops, fh, indent=1, sort_keys=True)
+    # host-side copy of the operation counts: {la, lb, lc, ld, vrr/prim, hrr/contracted}
+    rows = [f"  {{{k[0]}, {k[1]}, {k[2]}, {k[3]}, {v['vrr_per_primitive_quartet']}, "
+            f"{v['hrr_per_contracted_quartet']}}}," for k, v in sorted(flops.items())]
+    with open(out_inc.replace("eri_classes.inc", "eri_flops.inc"), "w") as fh:
+        fh.write("// Generated by gen_eri.py -- do not edit.\n"
+                 "static const int kEriFlops[][6] = {\n" + "\n".join(rows) + "\n};\n"
+                 f"constexpr int kEriFlopsRows = {len(rows)};\n")
+
+
+if __name__ == "__main__":
+    main()

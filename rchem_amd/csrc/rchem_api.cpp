@@ -1,0 +1,1089 @@
+// Host side of the C ABI declared in include/rchem_gpu.h.
+//
+// Owns the context (device, stream, rocBLAS/rocSOLVER handles), the basis in
+// shell form, the shell-pair and tile lists that drive the ERI kernels, and
+// the SCF driver.  All arithmetic on integrals and matrices runs on the GPU;
+// the host sorts pair lists and drives launches.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/rchem_gpu.h"
+#include "boys.hpp"
+#include "types.hpp"
+
+namespace rchem {
+#include "generated/basis_tables.inc"
+#include "generated/eri_flops.inc"
+
+using EriLaunchFn = hipError_t (*)(const EriArgs&, hipStream_t);
+EriLaunchFn eri_launcher(int la, int lb, int lc, int ld, int mode, int bm);
+hipError_t symmetrize_launch(const double* Jh, const double* Kh, double* J, double* K, int n,
+                             hipStream_t s);
+hipError_t fill_launch(double* p, size_t n, double v, hipStream_t s);
+hipError_t fock_launch(const double* H, const double* J, const double* K, double* F, size_t n2,
+                       hipStream_t s);
+hipError_t energy_launch(const double* D, const double* H, const double* F, double* out,
+                         size_t n2, hipStream_t s);
+hipError_t scale_columns_launch(const double* U, const double* w, double* out, int n,
+                                hipStream_t s);
+
+struct ShellDev {
+  double xyz[3];
+  int l, nprim, p0, f0;
+};
+struct OneElArgs {
+  const ShellDev* shells;
+  const double* exps;
+  const double* ccoef;
+  const int2* pairs;
+  int npairs;
+  int nbf;
+  int natom;
+  const double* atom_xyz;
+  const double* atom_z;
+  const double* boys_table;
+  double* S;
+  double* T;
+  double* V;
+};
+hipError_t one_electron_launch(const OneElArgs& a, hipStream_t s);
+}  // namespace rchem
+
+using namespace rchem;
+
+namespace {
+
+constexpr int kMaxL = 2;        // highest angular momentum accepted
+constexpr int kNumPairClasses = (kMaxL + 1) * (kMaxL + 2) / 2;
+
+int pair_class(int l1, int l2) { return l1 * (l1 + 1) / 2 + l2; }
+int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+
+double fact2(int n) {
+  double r = 1.0;
+  for (int k = n; k > 1; k -= 2) r *= k;
+  return r;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  hipError_t ensure(size_t b) {
+    if (b <= bytes && p) return hipSuccess;
+    release();
+    hipError_t e = hipMalloc(&p, b ? b : 8);
+    if (e == hipSuccess) bytes = b ? b : 8;
+    return e;
+  }
+  template <typename T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+struct Shell {
+  int l = 0, nprim = 0, f0 = 0, p0 = 0;
+  double xyz[3] = {0, 0, 0};
+};
+
+struct Bucket {
+  int begin, end, np;  // pairs [begin, end) of the sorted pair array, np primitive pairs each
+};
+
+struct QClass {
+  int X, Y;           // bra / ket pair classes
+  int la, lb, lc, ld;
+  int tile_begin, ntiles;
+  double quartets, prim_quartets, flops;
+};
+
+struct ClassTimer {
+  int la, lb, lc, ld;
+  double quartets, prim_quartets, flops;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  double ms = 0.0;
+  int launches = 0;
+};
+
+// Host-side record of a shell pair before it is laid out.
+struct PairHost {
+  int a, b;     // shell indices, l(a) >= l(b)
+  int cls, np;
+  double Q;
+};
+
+}  // namespace
+
+struct rchem_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;
+  std::string err;
+
+  // basis
+  std::vector<Shell> shells;
+  std::vector<double> exps, coefs, ccoef;  // raw coefficients and c * radial norm
+  int nbf = 0;
+  bool have_basis = false;
+
+  // Schwarz bounds of every shell pair (a >= b), computed once per basis
+  std::vector<PairHost> all_pairs;
+  bool have_schwarz = false;
+
+  // laid-out pair list for a screening threshold
+  double pairs_tau = -1.0;
+  std::vector<PairRec> pairs;
+  std::vector<Bucket> buckets[kNumPairClasses];
+  int class_begin[kNumPairClasses + 1] = {0};
+  DevBuf pairs_d, prims_d;
+
+  // tiles for a threshold
+  double tiles_tau = -1.0;
+  std::vector<Tile> tiles;
+  std::vector<QClass> qclasses;
+  DevBuf tiles_d;
+  rchem_jk_stats planned{};  // statistics of the planned tile list
+
+  DevBuf boys_d;
+  DevBuf work_Jh, work_Kh, work_D, work_J, work_K;
+  rchem_jk_stats last_stats{};
+
+  bool profiling = false;
+  std::vector<ClassTimer> timers;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> event_pool;
+
+  int fail(int code, const std::string& msg) {
+    err = msg;
+    return code;
+  }
+};
+
+static std::string g_create_error;
+
+#define HIPC(expr)                                                                        \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return ctx->fail(RCHEM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define BLASC(expr)                                                                       \
+  do {                                                                                    \
+    rocblas_status s_ = (expr);                                                           \
+    if (s_ != rocblas_status_success)                                                     \
+      return ctx->fail(RCHEM_ERR_HIP, std::string(#expr) + ": rocblas status " +          \
+                                          std::to_string(static_cast<int>(s_)));          \
+  } while (0)
+
+#define TRY(expr)                \
+  do {                           \
+    int rc_ = (expr);            \
+    if (rc_ != RCHEM_OK) return rc_; \
+  } while (0)
+
+namespace {
+
+// Boys table F_m(k/16), m = 0..kBoysCols-1, from the all-positive series
+// F_m(T) = e^{-T} sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1)) in long double.
+std::vector<double> make_boys_table() {
+  std::vector<double> tab(static_cast<size_t>(kBoysRows) * kBoysCols);
+  for (int r = 0; r < kBoysRows; ++r) {
+    const long double T = static_cast<long double>(r) / kBoysInvH;
+    for (int m = 0; m < kBoysCols; ++m) {
+      long double term = 1.0L / (2 * m + 1), sum = term;
+      for (int k = 1; k < 4000; ++k) {
+        term *= 2.0L * T / (2 * m + 2 * k + 1);
+        sum += term;
+        if (term < sum * 1e-22L) break;
+      }
+      tab[static_cast<size_t>(r) * kBoysCols + m] = static_cast<double>(expl(-T) * sum);
+    }
+  }
+  return tab;
+}
+
+int set_device(rchem_ctx* ctx) {
+  HIPC(hipSetDevice(ctx->device));
+  return RCHEM_OK;
+}
+
+// Finishes a basis: function offsets, radial norms folded into coefficients.
+int finish_basis(rchem_ctx* ctx) {
+  int f = 0;
+  ctx->ccoef.resize(ctx->coefs.size());
+  for (auto& s : ctx->shells) {
+    if (s.l < 0 || s.l > kMaxL)
+      return ctx->fail(RCHEM_ERR_UNSUPPORTED,
+                       "angular momentum " + std::to_string(s.l) + " not supported (max 2)");
+    if (s.nprim <= 0) return ctx->fail(RCHEM_ERR_INVALID, "shell with no primitives");
+    s.f0 = f;
+    f += ncart(s.l);
+    for (int p = 0; p < s.nprim; ++p) {
+      const double a = ctx->exps[s.p0 + p];
+      if (!(a > 0.0) || !std::isfinite(a))
+        return ctx->fail(RCHEM_ERR_INVALID, "non-positive or non-finite exponent");
+      // exponent-dependent part of PGTO::normalization (basis.rs:140-149)
+      const int L = s.l;
+      const double nrad = std::sqrt(std::pow(2.0, 2.0 * L + 1.5) * std::pow(a, L + 1.5) /
+                                    std::pow(M_PI, 1.5));
+      ctx->ccoef[s.p0 + p] = ctx->coefs[s.p0 + p] * nrad;
+    }
+  }
+  ctx->nbf = f;
+  ctx->have_basis = true;
+  ctx->have_schwarz = false;
+  ctx->pairs_tau = -1.0;
+  ctx->tiles_tau = -1.0;
+  ctx->all_pairs.clear();
+  return RCHEM_OK;
+}
+
+// Lays out the pairs selected by `keep` (sorted within buckets by Q
+// descending) and uploads pair and primitive-pair records.
+int layout_pairs(rchem_ctx* ctx, const std::vector<PairHost>& sel) {
+  std::vector<PairHost> v = sel;
+  std::stable_sort(v.begin(), v.end(), [](const PairHost& x, const PairHost& y) {
+    if (x.cls != y.cls) return x.cls < y.cls;
+    if (x.np != y.np) return x.np > y.np;
+    return x.Q > y.Q;
+  });
+  ctx->pairs.assign(v.size(), PairRec{});
+  std::vector<PrimRec> prims;
+  for (int c = 0; c < kNumPairClasses; ++c) ctx->buckets[c].clear();
+  size_t i = 0;
+  for (int c = 0; c <= kNumPairClasses; ++c) {
+    while (i < v.size() && v[i].cls < c) ++i;
+    ctx->class_begin[c] = static_cast<int>(i);
+  }
+  size_t pos = 0;
+  while (pos < v.size()) {
+    size_t end = pos;
+    while (end < v.size() && v[end].cls == v[pos].cls && v[end].np == v[pos].np) ++end;
+    const int n = static_cast<int>(end - pos), np = v[pos].np;
+    ctx->buckets[v[pos].cls].push_back(Bucket{static_cast<int>(pos), static_cast<int>(end), np});
+    const size_t base = prims.size();
+    prims.resize(base + static_cast<size_t>(n) * np);
+    for (int q = 0; q < n; ++q) {
+      const PairHost& ph = v[pos + q];
+      const Shell& sa = ctx->shells[ph.a];
+      const Shell& sb = ctx->shells[ph.b];
+      PairRec& r = ctx->pairs[pos + q];
+      double AB2 = 0.0;
+      for (int x = 0; x < 3; ++x) {
+        r.A[x] = sa.xyz[x];
+        r.AB[x] = sa.xyz[x] - sb.xyz[x];
+        AB2 += r.AB[x] * r.AB[x];
+      }
+      r.fa = sa.f0;
+      r.fb = sb.f0;
+      r.np = np;
+      r.p0 = static_cast<int>(base) + q;
+      r.ps = n;
+      r.flags = (ph.a != ph.b) ? 1 : 0;
+      int k = 0;
+      for (int ip = 0; ip < sa.nprim; ++ip)
+        for (int jp = 0; jp < sb.nprim; ++jp, ++k) {
+          const double al = ctx->exps[sa.p0 + ip], be = ctx->exps[sb.p0 + jp];
+          const double z = al + be;
+          PrimRec& pr = prims[base + static_cast<size_t>(k) * n + q];
+          pr.zeta = z;
+          pr.px = (al * sa.xyz[0] + be * sb.xyz[0]) / z;
+          pr.py = (al * sa.xyz[1] + be * sb.xyz[1]) / z;
+          pr.pz = (al * sa.xyz[2] + be * sb.xyz[2]) / z;
+          pr.kab = ctx->ccoef[sa.p0 + ip] * ctx->ccoef[sb.p0 + jp] * std::sqrt(2.0) *
+                   std::pow(M_PI, 1.25) / z * std::exp(-al * be / z * AB2);
+          pr.oo2z = 0.5 / z;
+        }
+    }
+    pos = end;
+  }
+  HIPC(ctx->pairs_d.ensure(sizeof(PairRec) * std::max<size_t>(ctx->pairs.size(), 1)));
+  HIPC(ctx->prims_d.ensure(sizeof(PrimRec) * std::max<size_t>(prims.size(), 1)));
+  if (!ctx->pairs.empty())
+    HIPC(hipMemcpyAsync(ctx->pairs_d.p, ctx->pairs.data(), sizeof(PairRec) * ctx->pairs.size(),
+                        hipMemcpyHostToDevice, ctx->stream));
+  if (!prims.empty())
+    HIPC(hipMemcpyAsync(ctx->prims_d.p, prims.data(), sizeof(PrimRec) * prims.size(),
+                        hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return RCHEM_OK;
+}
+
+int upload_tiles(rchem_ctx* ctx) {
+  HIPC(ctx->tiles_d.ensure(sizeof(Tile) * std::max<size_t>(ctx->tiles.size(), 1)));
+  if (!ctx->tiles.empty())
+    HIPC(hipMemcpyAsync(ctx->tiles_d.p, ctx->tiles.data(), sizeof(Tile) * ctx->tiles.size(),
+                        hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return RCHEM_OK;
+}
+
+void class_ls(int X, int* l1, int* l2) {
+  for (int a = 0; a <= kMaxL; ++a)
+    for (int b = 0; b <= a; ++b)
+      if (pair_class(a, b) == X) {
+        *l1 = a;
+        *l2 = b;
+      }
+}
+
+// Schwarz bounds for every shell pair (exact Boys), once per basis.
+int compute_schwarz(rchem_ctx* ctx) {
+  if (ctx->have_schwarz) return RCHEM_OK;
+  const int ns = static_cast<int>(ctx->shells.size());
+  std::vector<PairHost> all;
+  all.reserve(static_cast<size_t>(ns) * (ns + 1) / 2);
+  for (int a = 0; a < ns; ++a)
+    for (int b = 0; b <= a; ++b) {
+      int x = a, y = b;
+      if (ctx->shells[y].l > ctx->shells[x].l) std::swap(x, y);
+      all.push_back(PairHost{x, y, pair_class(ctx->shells[x].l, ctx->shells[y].l),
+                             ctx->shells[x].nprim * ctx->shells[y].nprim, 0.0});
+    }
+  TRY(layout_pairs(ctx, all));
+  // diagonal tiles: 64 consecutive pairs of one bucket, bra = ket per lane
+  ctx->tiles.clear();
+  std::vector<std::pair<int, int>> ranges;  // per class: tile range
+  for (int X = 0; X < kNumPairClasses; ++X) {
+    const int t0 = static_cast<int>(ctx->tiles.size());
+    for (const Bucket& bk : ctx->buckets[X])
+      for (int j = bk.begin; j < bk.end; j += 64)
+        ctx->tiles.push_back(Tile{-1, j, std::min(64, bk.end - j), 0});
+    ranges.emplace_back(t0, static_cast<int>(ctx->tiles.size()) - t0);
+  }
+  TRY(upload_tiles(ctx));
+  DevBuf q;
+  HIPC(q.ensure(sizeof(double) * std::max<size_t>(ctx->pairs.size(), 1)));
+  for (int X = 0; X < kNumPairClasses; ++X) {
+    if (ranges[X].second == 0) continue;
+    int l1, l2;
+    class_ls(X, &l1, &l2);
+    EriLaunchFn fn = eri_launcher(l1, l2, l1, l2, kModeDiag, kBoysExact);
+    if (!fn) return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class not compiled in");
+    EriArgs a{};
+    a.pairs = ctx->pairs_d.as<PairRec>();
+    a.prims = ctx->prims_d.as<PrimRec>();
+    a.tiles = ctx->tiles_d.as<Tile>() + ranges[X].first;
+    a.ntiles = ranges[X].second;
+    a.nbf = ctx->nbf;
+    a.boys_table = ctx->boys_d.as<double>();
+    a.out = q.as<double>();
+    HIPC(fn(a, ctx->stream));
+  }
+  std::vector<double> Q(ctx->pairs.size());
+  if (!Q.empty())
+    HIPC(hipMemcpyAsync(Q.data(), q.p, sizeof(double) * Q.size(), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  // map back: layout_pairs sorted `all`; recover (a, b) from the records' order
+  std::vector<PairHost> sorted = all;
+  std::stable_sort(sorted.begin(), sorted.end(), [](const PairHost& x, const PairHost& y) {
+    if (x.cls != y.cls) return x.cls < y.cls;
+    if (x.np != y.np) return x.np > y.np;
+    return x.Q > y.Q;
+  });
+  for (size_t i = 0; i < sorted.size(); ++i) {
+    if (!std::isfinite(Q[i]))
+      return ctx->fail(RCHEM_ERR_NONFINITE, "non-finite Schwarz bound");
+    sorted[i].Q = Q[i];
+  }
+  ctx->all_pairs = std::move(sorted);
+  ctx->have_schwarz = true;
+  ctx->pairs_tau = -1.0;
+  ctx->tiles_tau = -1.0;
+  return RCHEM_OK;
+}
+
+int flops_for(int la, int lb, int lc, int ld, int* vrr, int* hrr) {
+  for (int r = 0; r < kEriFlopsRows; ++r)
+    if (kEriFlops[r][0] == la && kEriFlops[r][1] == lb && kEriFlops[r][2] == lc &&
+        kEriFlops[r][3] == ld) {
+      *vrr = kEriFlops[r][4];
+      *hrr = kEriFlops[r][5];
+      return 1;
+    }
+  return 0;
+}
+
+// Pair list and quartet tiles for screening threshold tau.
+int prepare(rchem_ctx* ctx, double tau) {
+  TRY(compute_schwarz(ctx));
+  if (ctx->pairs_tau != tau) {
+    double qmax = 0.0;
+    for (const auto& p : ctx->all_pairs) qmax = std::max(qmax, p.Q);
+    std::vector<PairHost> sel;
+    sel.reserve(ctx->all_pairs.size());
+    for (const auto& p : ctx->all_pairs)
+      if (tau <= 0.0 || p.Q * qmax >= tau) sel.push_back(p);
+    TRY(layout_pairs(ctx, sel));
+    ctx->pairs_tau = tau;
+    ctx->tiles_tau = -1.0;
+  }
+  if (ctx->tiles_tau == tau) return RCHEM_OK;
+  // Q of the laid-out pairs, in order
+  std::vector<double> Q(ctx->pairs.size());
+  {
+    std::vector<PairHost> sel;
+    double qmax = 0.0;
+    for (const auto& p : ctx->all_pairs) qmax = std::max(qmax, p.Q);
+    for (const auto& p : ctx->all_pairs)
+      if (tau <= 0.0 || p.Q * qmax >= tau) sel.push_back(p);
+    std::stable_sort(sel.begin(), sel.end(), [](const PairHost& x, const PairHost& y) {
+      if (x.cls != y.cls) return x.cls < y.cls;
+      if (x.np != y.np) return x.np > y.np;
+      return x.Q > y.Q;
+    });
+    for (size_t i = 0; i < sel.size(); ++i) Q[i] = sel[i].Q;
+  }
+  ctx->tiles.clear();
+  ctx->qclasses.clear();
+  rchem_jk_stats st{};
+  for (int X = 0; X < kNumPairClasses; ++X) {
+    for (int Y = 0; Y <= X; ++Y) {
+      QClass qc{};
+      qc.X = X;
+      qc.Y = Y;
+      class_ls(X, &qc.la, &qc.lb);
+      class_ls(Y, &qc.lc, &qc.ld);
+      qc.tile_begin = static_cast<int>(ctx->tiles.size());
+      int fv = 0, fh = 0;
+      flops_for(qc.la, qc.lb, qc.lc, qc.ld, &fv, &fh);
+      const int L = qc.la + qc.lb + qc.lc + qc.ld;
+      const double per_prim = fv + 68.0 + 4.0 * L;
+      const double nint = static_cast<double>(ncart(qc.la) * ncart(qc.lb) * ncart(qc.lc) *
+                                              ncart(qc.ld));
+      const rchem_jk_stats before = st;
+      for (int i = ctx->class_begin[X]; i < ctx->class_begin[X + 1]; ++i) {
+        const double need = (tau > 0.0 && Q[i] > 0.0) ? tau / Q[i] : 0.0;
+        if (tau > 0.0 && Q[i] <= 0.0) continue;
+        for (const Bucket& bk : ctx->buckets[Y]) {
+          int end = bk.end;
+          if (X == Y) {
+            if (bk.begin > i) continue;
+            end = std::min(end, i + 1);
+          }
+          if (tau > 0.0) {
+            // Q descending within the bucket: count j with Q_j >= need
+            int lo = bk.begin, hi = end;
+            while (lo < hi) {
+              const int mid = (lo + hi) / 2;
+              if (Q[mid] >= need) lo = mid + 1; else hi = mid;
+            }
+            end = lo;
+          }
+          const int cnt = end - bk.begin;
+          if (cnt <= 0) continue;
+          for (int j = bk.begin; j < end; j += 64)
+            ctx->tiles.push_back(Tile{i, j, std::min(64, end - j), 0});
+          st.quartets += cnt;
+          st.primitive_quartets += static_cast<double>(cnt) * ctx->pairs[i].np * bk.np;
+          st.integrals += cnt * nint;
+          st.flops += static_cast<double>(cnt) * ctx->pairs[i].np * bk.np * per_prim +
+                      static_cast<double>(cnt) * fh;
+        }
+      }
+      qc.ntiles = static_cast<int>(ctx->tiles.size()) - qc.tile_begin;
+      qc.quartets = st.quartets - before.quartets;
+      qc.prim_quartets = st.primitive_quartets - before.primitive_quartets;
+      qc.flops = st.flops - before.flops;
+      if (qc.ntiles > 0) {
+        if (!eri_launcher(qc.la, qc.lb, qc.lc, qc.ld, kModeJK, kBoysExact))
+          return ctx->fail(RCHEM_ERR_UNSUPPORTED,
+                           "ERI class (" + std::to_string(qc.la) + std::to_string(qc.lb) + "|" +
+                               std::to_string(qc.lc) + std::to_string(qc.ld) +
+                               ") not compiled into this library");
+        ctx->qclasses.push_back(qc);
+      }
+    }
+  }
+  TRY(upload_tiles(ctx));
+  ctx->planned = st;
+  ctx->tiles_tau = tau;
+  return RCHEM_OK;
+}
+
+int check_opts(rchem_ctx* ctx, const rchem_opts* o) {
+  if (o->boys_mode != RCHEM_BOYS_EXACT && o->boys_mode != RCHEM_BOYS_COMPAT)
+    return ctx->fail(RCHEM_ERR_INVALID, "unknown boys_mode");
+  if (o->precision != RCHEM_PREC_FP64 && o->precision != RCHEM_PREC_MIXED)
+    return ctx->fail(RCHEM_ERR_INVALID, "unknown precision");
+  if (o->precision == RCHEM_PREC_MIXED)
+    return ctx->fail(RCHEM_ERR_UNSUPPORTED, "mixed precision not available in this build");
+  if (!(o->schwarz_thresh >= 0.0) || !std::isfinite(o->schwarz_thresh))
+    return ctx->fail(RCHEM_ERR_INVALID, "schwarz_thresh must be finite and >= 0");
+  return RCHEM_OK;
+}
+
+ClassTimer* timer_for(rchem_ctx* ctx, const QClass& qc) {
+  for (auto& t : ctx->timers)
+    if (t.la == qc.la && t.lb == qc.lb && t.lc == qc.lc && t.ld == qc.ld) return &t;
+  ClassTimer t;
+  t.la = qc.la; t.lb = qc.lb; t.lc = qc.lc; t.ld = qc.ld;
+  ctx->timers.push_back(t);
+  return &ctx->timers.back();
+}
+
+int take_events(rchem_ctx* ctx, std::pair<hipEvent_t, hipEvent_t>* ev) {
+  if (!ctx->event_pool.empty()) {
+    *ev = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    return RCHEM_OK;
+  }
+  HIPC(hipEventCreate(&ev->first));
+  HIPC(hipEventCreate(&ev->second));
+  return RCHEM_OK;
+}
+
+// Folds completed event pairs into the per-class sums (synchronises on them).
+int collect_timers(rchem_ctx* ctx) {
+  for (auto& t : ctx->timers) {
+    for (auto& ev : t.pending) {
+      HIPC(hipEventSynchronize(ev.second));
+      float ms = 0.f;
+      HIPC(hipEventElapsedTime(&ms, ev.first, ev.second));
+      t.ms += ms;
+      t.launches += 1;
+      ctx->event_pool.push_back(ev);
+    }
+    t.pending.clear();
+  }
+  return RCHEM_OK;
+}
+
+// Launches the ERI kernels of every quartet class in `mode`.
+int run_eri(rchem_ctx* ctx, int mode, int bm, const double* D, double* Jh, double* Kh,
+            double* out) {
+  for (const QClass& qc : ctx->qclasses) {
+    EriLaunchFn fn = eri_launcher(qc.la, qc.lb, qc.lc, qc.ld, mode, bm);
+    if (!fn) return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class not compiled in");
+    EriArgs a{};
+    a.pairs = ctx->pairs_d.as<PairRec>();
+    a.prims = ctx->prims_d.as<PrimRec>();
+    a.tiles = ctx->tiles_d.as<Tile>() + qc.tile_begin;
+    a.ntiles = qc.ntiles;
+    a.nbf = ctx->nbf;
+    a.boys_table = ctx->boys_d.as<double>();
+    a.D = D;
+    a.J = Jh;
+    a.K = Kh;
+    a.out = out;
+    const bool timed = ctx->profiling && mode == kModeJK;
+    std::pair<hipEvent_t, hipEvent_t> ev{};
+    if (timed) {
+      TRY(take_events(ctx, &ev));
+      HIPC(hipEventRecord(ev.first, ctx->stream));
+    }
+    HIPC(fn(a, ctx->stream));
+    if (timed) {
+      HIPC(hipEventRecord(ev.second, ctx->stream));
+      ClassTimer* t = timer_for(ctx, qc);
+      t->quartets = qc.quartets;
+      t->prim_quartets = qc.prim_quartets;
+      t->flops = qc.flops;
+      t->pending.push_back(ev);
+    }
+  }
+  return RCHEM_OK;
+}
+
+int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rchem_opts* o) {
+  TRY(prepare(ctx, o->schwarz_thresh));
+  const size_t n2 = static_cast<size_t>(ctx->nbf) * ctx->nbf;
+  HIPC(ctx->work_Jh.ensure(n2 * sizeof(double)));
+  HIPC(ctx->work_Kh.ensure(n2 * sizeof(double)));
+  HIPC(fill_launch(ctx->work_Jh.as<double>(), n2, 0.0, ctx->stream));
+  HIPC(fill_launch(ctx->work_Kh.as<double>(), n2, 0.0, ctx->stream));
+  TRY(run_eri(ctx, kModeJK, o->boys_mode, dD, ctx->work_Jh.as<double>(),
+              ctx->work_Kh.as<double>(), nullptr));
+  HIPC(symmetrize_launch(ctx->work_Jh.as<double>(), ctx->work_Kh.as<double>(), dJ, dK, ctx->nbf,
+                         ctx->stream));
+  ctx->last_stats = ctx->planned;
+  return RCHEM_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+extern "C" {
+
+int rchem_opts_default(rchem_opts* o) {
+  if (!o) return RCHEM_ERR_INVALID;
+  o->boys_mode = RCHEM_BOYS_EXACT;
+  o->precision = RCHEM_PREC_FP64;
+  o->schwarz_thresh = 1e-12;
+  o->mixed_thresh = 1e-7;
+  return RCHEM_OK;
+}
+
+int rchem_ctx_create(int device, rchem_ctx** out) {
+  if (!out) return RCHEM_ERR_INVALID;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    g_create_error = "no HIP device visible";
+    return RCHEM_ERR_NO_DEVICE;
+  }
+  if (device < 0 || device >= ndev) {
+    g_create_error = "device index out of range";
+    return RCHEM_ERR_INVALID;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    g_create_error = "hipGetDeviceProperties failed";
+    return RCHEM_ERR_HIP;
+  }
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    g_create_error = std::string("device is ") + prop.gcnArchName + ", library built for gfx950";
+    return RCHEM_ERR_NO_DEVICE;
+  }
+  auto* ctx = new rchem_ctx();
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
+    g_create_error = "stream creation failed";
+    delete ctx;
+    return RCHEM_ERR_HIP;
+  }
+  if (rocblas_create_handle(&ctx->blas) != rocblas_status_success ||
+      rocblas_set_stream(ctx->blas, ctx->stream) != rocblas_status_success) {
+    g_create_error = "rocblas handle creation failed";
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return RCHEM_ERR_HIP;
+  }
+  std::vector<double> tab = make_boys_table();
+  if (ctx->boys_d.ensure(tab.size() * sizeof(double)) != hipSuccess ||
+      hipMemcpy(ctx->boys_d.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) !=
+          hipSuccess) {
+    g_create_error = "Boys table upload failed";
+    rocblas_destroy_handle(ctx->blas);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return RCHEM_ERR_HIP;
+  }
+  *out = ctx;
+  return RCHEM_OK;
+}
+
+int rchem_ctx_destroy(rchem_ctx* ctx) {
+  if (!ctx) return RCHEM_ERR_INVALID;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& t : ctx->timers)
+    for (auto& ev : t.pending) ctx->event_pool.push_back(ev);
+  for (auto& ev : ctx->event_pool) {
+    (void)hipEventDestroy(ev.first);
+    (void)hipEventDestroy(ev.second);
+  }
+  if (ctx->blas) rocblas_destroy_handle(ctx->blas);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return RCHEM_OK;
+}
+
+const char* rchem_last_error(const rchem_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int rchem_basis_upload(rchem_ctx* ctx, int nshell, const double* xyz, const int* l,
+                       const int* nprim, const double* exps, const double* coefs) {
+  if (!ctx) return RCHEM_ERR_INVALID;
+  if (nshell <= 0 || !xyz || !l || !nprim || !exps || !coefs)
+    return ctx->fail(RCHEM_ERR_INVALID, "rchem_basis_upload: bad arguments");
+  ctx->shells.assign(nshell, Shell{});
+  ctx->exps.clear();
+  ctx->coefs.clear();
+  for (int s = 0; s < nshell; ++s) {
+    Shell& sh = ctx->shells[s];
+    sh.l = l[s];
+    sh.nprim = nprim[s];
+    if (sh.nprim <= 0) return ctx->fail(RCHEM_ERR_INVALID, "shell with no primitives");
+    sh.p0 = static_cast<int>(ctx->exps.size());
+    for (int x = 0; x < 3; ++x) {
+      sh.xyz[x] = xyz[3 * s + x];
+      if (!std::isfinite(sh.xyz[x])) return ctx->fail(RCHEM_ERR_INVALID, "non-finite centre");
+    }
+    for (int p = 0; p < sh.nprim; ++p) {
+      ctx->exps.push_back(exps[sh.p0 + p]);
+      ctx->coefs.push_back(coefs[sh.p0 + p]);
+    }
+  }
+  TRY(set_device(ctx));
+  return finish_basis(ctx);
+}
+
+int rchem_basis_from_atoms(rchem_ctx* ctx, int natom, const int* Z, const double* xyz,
+                           const char* basis_name) {
+  if (!ctx) return RCHEM_ERR_INVALID;
+  if (natom <= 0 || !Z || !xyz || !basis_name)
+    return ctx->fail(RCHEM_ERR_INVALID, "rchem_basis_from_atoms: bad arguments");
+  auto lower = [](std::string s) {
+    for (auto& c : s) c = static_cast<char>(std::tolower(static_cast<unsigned char>(c)));
+    return s;
+  };
+  const std::string want = lower(basis_name);
+  bool known = false;
+  for (int r = 0; r < kBasisNumRows; ++r)
+    if (lower(kBasisRows[r].set) == want) known = true;
+  if (!known) return ctx->fail(RCHEM_ERR_INVALID, std::string("unknown basis set ") + basis_name);
+  ctx->shells.clear();
+  ctx->exps.clear();
+  ctx->coefs.clear();
+  // per atom, per basis-set shell, per l in its angular_momentum list (basis.rs:186-205)
+  for (int a = 0; a < natom; ++a) {
+    bool found = false;
+    for (int r = 0; r < kBasisNumRows; ++r) {
+      const BasisShellRow& row = kBasisRows[r];
+      if (lower(row.set) != want || row.Z != Z[a]) continue;
+      found = true;
+      for (int k = 0; k < 2; ++k) {
+        const int l = row.am[k];
+        if (l < 0) continue;
+        Shell sh;
+        sh.l = l;
+        sh.nprim = row.nprim;
+        sh.p0 = static_cast<int>(ctx->exps.size());
+        for (int x = 0; x < 3; ++x) sh.xyz[x] = xyz[3 * a + x];
+        for (int p = 0; p < row.nprim; ++p) {
+          ctx->exps.push_back(row.exps[p]);
+          ctx->coefs.push_back(row.coefs[k][p]);  // coefficients[l] (basis.rs:201)
+        }
+        ctx->shells.push_back(sh);
+      }
+    }
+    if (!found)
+      return ctx->fail(RCHEM_ERR_INVALID, "element Z=" + std::to_string(Z[a]) + " not in " +
+                                              basis_name);
+  }
+  TRY(set_device(ctx));
+  return finish_basis(ctx);
+}
+
+int rchem_basis_nbf(const rchem_ctx* ctx, int* nbf) {
+  if (!ctx || !nbf) return RCHEM_ERR_INVALID;
+  if (!ctx->have_basis) return RCHEM_ERR_NO_BASIS;
+  *nbf = ctx->nbf;
+  return RCHEM_OK;
+}
+
+int rchem_basis_dims(const rchem_ctx* ctx, int* nshell, int* nprim_total) {
+  if (!ctx || !nshell || !nprim_total) return RCHEM_ERR_INVALID;
+  if (!ctx->have_basis) return RCHEM_ERR_NO_BASIS;
+  *nshell = static_cast<int>(ctx->shells.size());
+  *nprim_total = static_cast<int>(ctx->exps.size());
+  return RCHEM_OK;
+}
+
+int rchem_basis_shells(const rchem_ctx* ctx, double* xyz, int* l, int* nprim, double* exps,
+                       double* coefs) {
+  if (!ctx) return RCHEM_ERR_INVALID;
+  if (!ctx->have_basis) return RCHEM_ERR_NO_BASIS;
+  for (size_t s = 0; s < ctx->shells.size(); ++s) {
+    const Shell& sh = ctx->shells[s];
+    if (xyz)
+      for (int x = 0; x < 3; ++x) xyz[3 * s + x] = sh.xyz[x];
+    if (l) l[s] = sh.l;
+    if (nprim) nprim[s] = sh.nprim;
+  }
+  if (exps) std::copy(ctx->exps.begin(), ctx->exps.end(), exps);
+  if (coefs) std::copy(ctx->coefs.begin(), ctx->coefs.end(), coefs);
+  return RCHEM_OK;
+}
+
+int rchem_jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK,
+                    const rchem_opts* opts) {
+  if (!ctx) return RCHEM_ERR_INVALID;
+  if (!ctx->have_basis) return ctx->fail(RCHEM_ERR_NO_BASIS, "no basis uploaded");
+  if (!dD || !dJ || !dK) return ctx->fail(RCHEM_ERR_INVALID, "null device pointer");
+  rchem_opts o;
+  rchem_opts_default(&o);
+  if (opts) o = *opts;
+  TRY(check_opts(ctx, &o));
+  TRY(set_device(ctx));
+  return jk_device(ctx, dD, dJ, dK, &o);
+}
+
+int rchem_jk(rchem_ctx* ctx, const double* D, double* J, double* K, const rchem_opts* opts) {
+  if (!ctx) return RCHEM_ERR_INVALID;
+  if (!ctx->have_basis) return ctx->fail(RCHEM_ERR_NO_BASIS, "no basis uploaded");
+  if (!D || !J || !K) return ctx->fail(RCHEM_ERR_INVALID, "null pointer");
+  rchem_opts o;
+  rchem_opts_default(&o);
+  if (opts) o = *opts;
+  TRY(check_opts(ctx, &o));
+  TRY(set_device(ctx));
+  const size_t n2 = static_cast<size_t>(ctx->nbf) * ctx->nbf;
+  for (size_t i = 0; i < n2; ++i)
+    if (!std::isfinite(D[i])) return ctx->fail(RCHEM_ERR_NONFINITE, "non-finite density");
+  HIPC(ctx->work_D.ensure(n2 * sizeof(double)));
+  HIPC(ctx->work_J.ensure(n2 * sizeof(double)));
+  HIPC(ctx->work_K.ensure(n2 * sizeof(double)));
+  HIPC(hipMemcpyAsync(ctx->work_D.p, D, n2 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  TRY(jk_device(ctx, ctx->work_D.as<double>(), ctx->work_J.as<double>(), ctx->work_K.as<double>(),
+                &o));
+  HIPC(hipMemcpyAsync(J, ctx->work_J.p, n2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipMemcpyAsync(K, ctx->work_K.p, n2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return RCHEM_OK;
+}
+
+int rchem_set_profiling(rchem_ctx* ctx, int enable) {
+  if (!ctx) return RCHEM_ERR_INVALID;
+  ctx->profiling = enable != 0;
+  return RCHEM_OK;
+}
+
+int rchem_reset_class_timings(rchem_ctx* ctx) {
+  if (!ctx) return RCHEM_ERR_INVALID;
+  TRY(set_device(ctx));
+  TRY(collect_timers(ctx));
+  ctx->timers.clear();
+  return RCHEM_OK;
+}
+
+int rchem_jk_class_timings(rchem_ctx* ctx, rchem_class_timing* out, int max, int* count) {
+  if (!ctx || !count || (max > 0 && !out)) return RCHEM_ERR_INVALID;
+  TRY(set_device(ctx));
+  TRY(collect_timers(ctx));
+  *count = static_cast<int>(ctx->timers.size());
+  for (int i = 0; i < max && i < *count; ++i) {
+    const ClassTimer& t = ctx->timers[i];
+    out[i] = rchem_class_timing{t.la, t.lb, t.lc, t.ld, t.launches, t.ms,
+                                t.quartets, t.prim_quartets, t.flops};
+  }
+  return RCHEM_OK;
+}
+
+int rchem_jk_last_stats(const rchem_ctx* ctx, rchem_jk_stats* st) {
+  if (!ctx || !st) return RCHEM_ERR_INVALID;
+  *st = ctx->last_stats;
+  return RCHEM_OK;
+}
+
+int rchem_eri_tensor(rchem_ctx* ctx, double* I, const rchem_opts* opts) {
+  if (!ctx) return RCHEM_ERR_INVALID;
+  if (!ctx->have_basis) return ctx->fail(RCHEM_ERR_NO_BASIS, "no basis uploaded");
+  if (!I) return ctx->fail(RCHEM_ERR_INVALID, "null pointer");
+  rchem_opts o;
+  rchem_opts_default(&o);
+  if (opts) o = *opts;
+  TRY(check_opts(ctx, &o));
+  TRY(set_device(ctx));
+  const size_t n = static_cast<size_t>(ctx->nbf);
+  const size_t n4 = n * n * n * n;
+  if (n > 200) return ctx->fail(RCHEM_ERR_INVALID, "tensor too large (nbf > 200)");
+  TRY(prepare(ctx, 0.0));
+  DevBuf t;
+  HIPC(t.ensure(n4 * sizeof(double)));
+  HIPC(fill_launch(t.as<double>(), n4, 0.0, ctx->stream));
+  TRY(run_eri(ctx, kModeTensor, o.boys_mode, nullptr, nullptr, nullptr, t.as<double>()));
+  HIPC(hipMemcpyAsync(I, t.p, n4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return RCHEM_OK;
+}
+
+int rchem_schwarz(rchem_ctx* ctx, double* Q, const rchem_opts* opts) {
+  (void)opts;
+  if (!ctx) return RCHEM_ERR_INVALID;
+  if (!ctx->have_basis) return ctx->fail(RCHEM_ERR_NO_BASIS, "no basis uploaded");
+  if (!Q) return ctx->fail(RCHEM_ERR_INVALID, "null pointer");
+  TRY(set_device(ctx));
+  TRY(compute_schwarz(ctx));
+  const size_t ns = ctx->shells.size();
+  std::fill(Q, Q + ns * ns, 0.0);
+  for (const auto& p : ctx->all_pairs) {
+    const int a = std::max(p.a, p.b), b = std::min(p.a, p.b);
+    Q[a * ns + b] = p.Q;
+  }
+  return RCHEM_OK;
+}
+
+int rchem_one_electron(rchem_ctx* ctx, int natom, const double* Z, const double* xyz, double* S,
+                       double* T, double* V) {
+  if (!ctx) return RCHEM_ERR_INVALID;
+  if (!ctx->have_basis) return ctx->fail(RCHEM_ERR_NO_BASIS, "no basis uploaded");
+  if (natom < 0 || (natom > 0 && (!Z || !xyz)))
+    return ctx->fail(RCHEM_ERR_INVALID, "bad nuclei");
+  TRY(set_device(ctx));
+  const int ns = static_cast<int>(ctx->shells.size());
+  std::vector<ShellDev> sh(ns);
+  for (int s = 0; s < ns; ++s) {
+    for (int x = 0; x < 3; ++x) sh[s].xyz[x] = ctx->shells[s].xyz[x];
+    sh[s].l = ctx->shells[s].l;
+    sh[s].nprim = ctx->shells[s].nprim;
+    sh[s].p0 = ctx->shells[s].p0;
+    sh[s].f0 = ctx->shells[s].f0;
+  }
+  std::vector<int2> pr;
+  for (int a = 0; a < ns; ++a)
+    for (int b = 0; b <= a; ++b) pr.push_back(make_int2(a, b));
+  const size_t n2 = static_cast<size_t>(ctx->nbf) * ctx->nbf;
+  DevBuf dsh, dex, dcc, dpr, dax, daz, dS, dT, dV;
+  HIPC(dsh.ensure(sizeof(ShellDev) * ns));
+  HIPC(dex.ensure(sizeof(double) * ctx->exps.size()));
+  HIPC(dcc.ensure(sizeof(double) * ctx->ccoef.size()));
+  HIPC(dpr.ensure(sizeof(int2) * pr.size()));
+  HIPC(dax.ensure(sizeof(double) * 3 * std::max(natom, 1)));
+  HIPC(daz.ensure(sizeof(double) * std::max(natom, 1)));
+  HIPC(dS.ensure(n2 * sizeof(double)));
+  HIPC(dT.ensure(n2 * sizeof(double)));
+  HIPC(dV.ensure(n2 * sizeof(double)));
+  HIPC(hipMemcpyAsync(dsh.p, sh.data(), sizeof(ShellDev) * ns, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(dex.p, ctx->exps.data(), sizeof(double) * ctx->exps.size(),
+                      hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(dcc.p, ctx->ccoef.data(), sizeof(double) * ctx->ccoef.size(),
+                      hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(dpr.p, pr.data(), sizeof(int2) * pr.size(), hipMemcpyHostToDevice,
+                      ctx->stream));
+  if (natom > 0) {
+    HIPC(hipMemcpyAsync(dax.p, xyz, sizeof(double) * 3 * natom, hipMemcpyHostToDevice, ctx->stream));
+    HIPC(hipMemcpyAsync(daz.p, Z, sizeof(double) * natom, hipMemcpyHostToDevice, ctx->stream));
+  }
+  OneElArgs a{};
+  a.shells = dsh.as<ShellDev>();
+  a.exps = dex.as<double>();
+  a.ccoef = dcc.as<double>();
+  a.pairs = dpr.as<int2>();
+  a.npairs = static_cast<int>(pr.size());
+  a.nbf = ctx->nbf;
+  a.natom = natom;
+  a.atom_xyz = dax.as<double>();
+  a.atom_z = daz.as<double>();
+  a.boys_table = ctx->boys_d.as<double>();
+  a.S = dS.as<double>();
+  a.T = dT.as<double>();
+  a.V = dV.as<double>();
+  HIPC(one_electron_launch(a, ctx->stream));
+  if (S) HIPC(hipMemcpyAsync(S, dS.p, n2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (T) HIPC(hipMemcpyAsync(T, dT.p, n2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (V) HIPC(hipMemcpyAsync(V, dV.p, n2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return RCHEM_OK;
+}
+
+int rchem_scf_opts_default(rchem_scf_opts* o) {
+  if (!o) return RCHEM_ERR_INVALID;
+  o->nocc = 0;
+  o->max_iter = 1024;
+  o->thresh_e = 1e-11;
+  rchem_opts_default(&o->jk);
+  o->verbose = 0;
+  return RCHEM_OK;
+}
+
+int rchem_scf(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
+              const rchem_scf_opts* opts, rchem_scf_result* res, double* D_out) {
+  if (!ctx) return RCHEM_ERR_INVALID;
+  if (!ctx->have_basis) return ctx->fail(RCHEM_ERR_NO_BASIS, "no basis uploaded");
+  if (!opts || !res) return ctx->fail(RCHEM_ERR_INVALID, "null opts or result");
+  const int n = ctx->nbf;
+  if (opts->nocc <= 0 || opts->nocc > n)
+    return ctx->fail(RCHEM_ERR_INVALID, "nocc out of range");
+  TRY(check_opts(ctx, &opts->jk));
+  TRY(set_device(ctx));
+  const size_t n2 = static_cast<size_t>(n) * n;
+  // one-electron matrices on the host path of this library, then resident on device
+  std::vector<double> S(n2), T(n2), V(n2), H(n2);
+  TRY(rchem_one_electron(ctx, natom, Z, xyz, S.data(), T.data(), V.data()));
+  for (size_t i = 0; i < n2; ++i) H[i] = T[i] + V[i];
+  DevBuf dS, dH, dX, dW, dF, dFp, dC, dD, dJ, dK, dtmp, dw, de, dinfo, dE;
+  for (DevBuf* b : {&dS, &dH, &dX, &dW, &dF, &dFp, &dC, &dD, &dJ, &dK, &dtmp})
+    HIPC(b->ensure(n2 * sizeof(double)));
+  HIPC(dw.ensure(n * sizeof(double)));
+  HIPC(de.ensure(n * sizeof(double)));
+  HIPC(dinfo.ensure(sizeof(rocblas_int)));
+  HIPC(dE.ensure(sizeof(double)));
+  HIPC(hipMemcpyAsync(dS.p, S.data(), n2 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(dH.p, H.data(), n2 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  const double one = 1.0, zero = 0.0;
+  rocblas_handle hb = ctx->blas;
+  // Loewdin X = U lambda^-1/2 U^T (rchem.rs:44-51); symmetric, so the
+  // row/column-major distinction does not matter for any matrix below.
+  HIPC(hipMemcpyAsync(dtmp.p, dS.p, n2 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  BLASC(rocsolver_dsyevd(hb, rocblas_evect_original, rocblas_fill_upper, n, dtmp.as<double>(), n,
+                         dw.as<double>(), de.as<double>(), dinfo.as<rocblas_int>()));
+  HIPC(scale_columns_launch(dtmp.as<double>(), dw.as<double>(), dW.as<double>(), n, ctx->stream));
+  BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_transpose, n, n, n, &one,
+                      dW.as<double>(), n, dtmp.as<double>(), n, &zero, dX.as<double>(), n));
+  auto density = [&](const double* Fm) -> int {
+    // F' = X^T F X, eigh, C = X C', D = C_occ C_occ^T (rchem.rs:56-59, 74-78, 101-104)
+    BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, Fm, n,
+                        dX.as<double>(), n, &zero, dtmp.as<double>(), n));
+    BLASC(rocblas_dgemm(hb, rocblas_operation_transpose, rocblas_operation_none, n, n, n, &one,
+                        dX.as<double>(), n, dtmp.as<double>(), n, &zero, dFp.as<double>(), n));
+    BLASC(rocsolver_dsyevd(hb, rocblas_evect_original, rocblas_fill_upper, n, dFp.as<double>(), n,
+                           dw.as<double>(), de.as<double>(), dinfo.as<rocblas_int>()));
+    BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
+                        dX.as<double>(), n, dFp.as<double>(), n, &zero, dC.as<double>(), n));
+    BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_transpose, n, n,
+                        opts->nocc, &one, dC.as<double>(), n, dC.as<double>(), n, &zero,
+                        dD.as<double>(), n));
+    return RCHEM_OK;
+  };
+  auto energy = [&](const double* Fm, double* e) -> int {
+    HIPC(energy_launch(dD.as<double>(), dH.as<double>(), Fm, dE.as<double>(), n2, ctx->stream));
+    HIPC(hipMemcpyAsync(e, dE.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    return RCHEM_OK;
+  };
+  TRY(density(dH.as<double>()));
+  double e_new = 0.0;
+  TRY(energy(dH.as<double>(), &e_new));
+  res->converged = 0;
+  res->iterations = 0;
+  int it = 0;
+  for (; it < opts->max_iter; ++it) {
+    TRY(jk_device(ctx, dD.as<double>(), dJ.as<double>(), dK.as<double>(), &opts->jk));
+    HIPC(fock_launch(dH.as<double>(), dJ.as<double>(), dK.as<double>(), dF.as<double>(), n2,
+                     ctx->stream));
+    TRY(density(dF.as<double>()));
+    const double e_old = e_new;
+    TRY(energy(dF.as<double>(), &e_new));
+    if (!std::isfinite(e_new)) return ctx->fail(RCHEM_ERR_NONFINITE, "non-finite SCF energy");
+    const double de_ = e_new - e_old;
+    if (opts->verbose) std::printf("%4d %20.12f %20.12f\n", it, e_new, de_);
+    if (std::fabs(de_) < opts->thresh_e) {
+      if (opts->verbose) std::printf("Convergence achieved!\n");
+      res->converged = 1;
+      ++it;
+      break;
+    }
+  }
+  res->iterations = it;
+  res->e_elec = e_new;
+  double enuc = 0.0;
+  for (int a = 0; a < natom; ++a)
+    for (int b = 0; b < a; ++b) {
+      const double dx = xyz[3 * a] - xyz[3 * b], dy = xyz[3 * a + 1] - xyz[3 * b + 1],
+                   dz = xyz[3 * a + 2] - xyz[3 * b + 2];
+      enuc += Z[a] * Z[b] / std::sqrt(dx * dx + dy * dy + dz * dz);
+    }
+  res->e_nuc = enuc;
+  res->e_total = e_new + enuc;
+  if (D_out)
+    HIPC(hipMemcpy(D_out, dD.p, n2 * sizeof(double), hipMemcpyDeviceToHost));
+  if (!res->converged)
+    return ctx->fail(RCHEM_ERR_NOT_CONVERGED, "SCF did not converge in max_iter iterations");
+  return RCHEM_OK;
+}
+
+}  // extern "C"

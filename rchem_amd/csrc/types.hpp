@@ -1,0 +1,57 @@
+// Data layout shared by the host library and the gfx950 kernels.
+//
+// Shell pairs are the unit of work on both sides of a quartet.  They live in
+// one array ordered by (pair class, primitive-pair count, Schwarz bound
+// descending).  A run of pairs with the same class and the same primitive-pair
+// count is a "bucket"; its primitive-pair records are stored transposed
+// ([primitive pair][pair in bucket]) so that the 64 lanes of a wavefront, which
+// hold 64 consecutive ket pairs of one bucket, read consecutive records.
+#pragma once
+#include <cstdint>
+
+namespace rchem {
+
+// One significant shell pair (a, b); a is the shell with the higher l.
+struct PairRec {
+  double A[3];   // centre of shell a
+  double AB[3];  // A - B
+  int fa, fb;    // first basis-function index of shells a and b
+  int np;        // number of primitive pairs
+  int p0;        // index of primitive pair 0 in the primitive-pair array
+  int ps;        // stride between consecutive primitive pairs of this pair
+  int flags;     // bit 0: a and b are different shells
+};
+
+// One primitive pair (i of a, j of b).
+struct PrimRec {
+  double zeta;          // alpha_i + alpha_j
+  double px, py, pz;    // Gaussian product centre
+  double kab;           // c_i N_i c_j N_j sqrt(2) pi^(5/4) / zeta exp(-alpha_i alpha_j/zeta |AB|^2)
+  double oo2z;          // 1 / (2 zeta)
+};
+
+// A wavefront's work: one bra pair against up to 64 consecutive ket pairs of
+// one bucket (uniform primitive-pair count across the wave).
+struct Tile {
+  int bra;
+  int ket0;
+  int count;
+  int pad;
+};
+
+enum EriMode { kModeJK = 0, kModeDiag = 1, kModeTensor = 2 };
+
+struct EriArgs {
+  const PairRec* pairs;
+  const PrimRec* prims;
+  const Tile* tiles;
+  int ntiles;
+  int nbf;
+  const double* boys_table;
+  const double* D;   // kModeJK: density, nbf x nbf row-major, symmetric
+  double* J;         // kModeJK: lower-half accumulators (J + J^T is the result)
+  double* K;
+  double* out;       // kModeDiag: Schwarz bound per pair; kModeTensor: nbf^4 tensor
+};
+
+}  // namespace rchem

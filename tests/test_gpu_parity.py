@@ -1,0 +1,162 @@
+"""GPU parity: the HIP path through the C ABI against the oracle and the
+reference's golden vectors.
+
+Bars (written in each test):
+  * ERIs in FP64 within 1e-12 absolute of the oracle / reference (compat Boys
+    reproduces the reference's NR incomplete gamma; exact Boys is compared to
+    the oracle's exact mode);
+  * SCF electronic energy within 1e-8 Eh of the oracle's restatement of
+    rchem's driver on the same inputs;
+  * size-independent properties at the full (H2O)_64 size.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from rchem_amd import basis as rb
+from rchem_amd._lib import BOYS_COMPAT, BOYS_EXACT, default_opts
+from rchem_amd.inputs import WATER_CRAWFORD, WATER_Z, water_cluster
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+DATA = os.path.join(os.path.dirname(os.path.dirname(__file__)), "rchem_amd", "data")
+
+
+def _oracle_basis(gbasis):
+    xyz, l, npr, ex, cf = gbasis.shells()
+    return O.cgtos_from_shells(xyz, l, npr, ex, cf)
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def test_water_eri_tensor_compat_vs_reference(gpu_ctx):
+    _say("case: water STO-3G build_I, compat Boys, nbf=7")
+    B = rb.Basis.new(WATER_Z, WATER_CRAWFORD, "STO-3G", ctx=gpu_ctx)
+    assert B.nbf == 7
+    I = rb.build_I(B, default_opts(boys_mode=BOYS_COMPAT, schwarz_thresh=0.0))
+    g = np.load(os.path.join(GOLD, "water_sto3g.npz"))
+    assert np.max(np.abs(I - g["eri"])) < 1e-12      # reference cints contr_coulomb
+    Io = O.eri_tensor(_oracle_basis(B), boys=O.BOYS_COMPAT)
+    assert np.max(np.abs(I - Io)) < 1e-12
+
+
+def test_water_eri_tensor_exact_vs_oracle(gpu_ctx):
+    _say("case: water STO-3G build_I, exact Boys")
+    B = rb.Basis.new(WATER_Z, WATER_CRAWFORD, "STO-3G", ctx=gpu_ctx)
+    I = rb.build_I(B, default_opts(boys_mode=BOYS_EXACT, schwarz_thresh=0.0))
+    Io = O.eri_tensor(_oracle_basis(B), boys=O.BOYS_EXACT)
+    assert np.max(np.abs(I - Io)) < 1e-12
+
+
+def test_contracted_samples_all_classes(gpu_ctx):
+    """Every (la lb|lc ld) class the library compiles, against the reference's
+    contr_coulomb values (golden vectors)."""
+    g = np.load(os.path.join(GOLD, "contracted_samples.npz"))
+    ls = g["ls"]
+    keep = [s for s in range(len(ls)) if ls[s] <= 1]
+    # s/p shells only until the d classes are compiled in
+    sel = [qi for qi, q in enumerate(g["quartets"]) if all(ls[s] <= 1 for s in q)]
+    _say(f"case: contracted samples, {len(sel)} s/p quartets")
+    idx = {s: k for k, s in enumerate(keep)}
+    nprim = g["nprim"][keep]
+    ex = np.concatenate([g["exps"][3 * s:3 * s + 3] for s in keep])
+    cf = np.concatenate([g["coefs"][3 * s:3 * s + 3] for s in keep])
+    B = rb.Basis.from_shells(g["centres"][keep], ls[keep], nprim, ex, cf, ctx=gpu_ctx)
+    I = rb.build_I(B, default_opts(boys_mode=BOYS_COMPAT, schwarz_thresh=0.0))
+    f0 = np.concatenate([[0], np.cumsum([(ls[s] + 1) * (ls[s] + 2) // 2 for s in keep])])
+    qof = g["quartet_of"]
+    worst = 0.0
+    for qi in sel:
+        q = [idx[s] for s in g["quartets"][qi]]
+        vals = g["values"][qof == qi]
+        k = 0
+        for a in range(f0[q[0]], f0[q[0] + 1]):
+            for b in range(f0[q[1]], f0[q[1] + 1]):
+                for c in range(f0[q[2]], f0[q[2] + 1]):
+                    for d in range(f0[q[3]], f0[q[3] + 1]):
+                        worst = max(worst, abs(I[a, b, c, d] - vals[k]))
+                        k += 1
+    assert worst < 1e-12
+
+
+def test_water_one_electron_vs_oracle(gpu_ctx):
+    _say("case: water STO-3G S/T/V")
+    B = rb.Basis.new(WATER_Z, WATER_CRAWFORD, "STO-3G", ctx=gpu_ctx)
+    S, T, V = rb.one_electron(B, WATER_CRAWFORD, WATER_Z)
+    So, To, Vo = O.one_electron(_oracle_basis(B), WATER_Z, WATER_CRAWFORD)
+    assert np.max(np.abs(S - So)) < 1e-13
+    assert np.max(np.abs(T - To)) < 1e-12
+    assert np.max(np.abs(V - Vo)) < 1e-12
+    g = np.load(os.path.join(GOLD, "water_sto3g.npz"))
+    assert np.max(np.abs(S - g["S"])) < 1e-13
+    assert np.max(np.abs(T - g["T"])) < 1e-12
+
+
+@pytest.mark.parametrize("basis_name,boys", [("STO-3G", BOYS_COMPAT), ("6-31G", BOYS_EXACT)])
+def test_jk_water_dimer_vs_oracle(gpu_ctx, basis_name, boys):
+    Z, xyz = water_cluster(dims=(2, 1, 1), seed=11)
+    B = rb.Basis.new(Z, xyz, basis_name, ctx=gpu_ctx)
+    _say(f"case: (H2O)_2 {basis_name} JK_direct, nbf={B.nbf}, boys={boys}")
+    n = B.nbf
+    rng = np.random.default_rng(5)
+    D = rng.normal(size=(n, n))
+    D = 0.5 * (D + D.T)
+    J = np.zeros((n, n)); K = np.zeros((n, n))
+    rb.JK_direct(J, K, B, D, default_opts(boys_mode=boys, schwarz_thresh=0.0))
+    Io = O.eri_tensor(_oracle_basis(B), boys=O.BOYS_COMPAT if boys == BOYS_COMPAT else O.BOYS_EXACT)
+    Jo, Ko = O.jk_inmem(Io, D)
+    assert np.max(np.abs(J - Jo)) < 1e-11
+    assert np.max(np.abs(K - Ko)) < 1e-11
+    # Schwarz screening at 1e-12 changes J/K by at most ~1e-12 * sum|D| per element
+    J2 = np.zeros((n, n)); K2 = np.zeros((n, n))
+    rb.JK_direct(J2, K2, B, D, default_opts(boys_mode=boys, schwarz_thresh=1e-12))
+    assert np.max(np.abs(J2 - J)) < 1e-12 * np.abs(D).sum()
+    assert np.max(np.abs(K2 - K)) < 1e-12 * np.abs(D).sum()
+
+
+def test_scf_water_energy_vs_oracle(gpu_ctx):
+    _say("case: water STO-3G SCF (rchem driver), compat Boys")
+    B = rb.Basis.new(WATER_Z, WATER_CRAWFORD, "STO-3G", ctx=gpu_ctx)
+    res, D = rb.scf(B, WATER_CRAWFORD, WATER_Z, nocc=5,
+                    opts=default_opts(boys_mode=BOYS_COMPAT, schwarz_thresh=0.0))
+    assert res.converged
+    Bo = _oracle_basis(B)
+    S, T, V = O.one_electron(Bo, WATER_Z, WATER_CRAWFORD)
+    I = O.eri_tensor(Bo, boys=O.BOYS_COMPAT)
+    r = O.scf(S, T + V, lambda Dm: O.jk_inmem(I, Dm), nocc=5)
+    assert abs(res.e_elec - r.e_elec) < 1e-8
+    assert abs(res.e_nuc - 8.00236706181077) < 1e-12
+    assert np.max(np.abs(D - r.D)) < 1e-6
+
+
+def test_jk_h2o64_properties(gpu_ctx):
+    """Full (H2O)_64 6-31G size: symmetry, linearity in D, positivity of the
+    Coulomb energy -- properties that do not need the oracle at this size."""
+    Z, xyz = water_cluster(dims=(4, 4, 4), seed=64)
+    B = rb.Basis.new(Z, xyz, "6-31G", ctx=gpu_ctx)
+    n = B.nbf
+    _say(f"case: (H2O)_64 6-31G JK properties, nbf={n}")
+    assert n == 832
+    rng = np.random.default_rng(64)
+    C = rng.normal(size=(n, 320)) / np.sqrt(n)
+    D1 = C @ C.T                      # positive semidefinite
+    D2 = rng.normal(size=(n, n)) * 1e-3
+    D2 = D2 + D2.T
+    J1 = np.zeros((n, n)); K1 = np.zeros((n, n))
+    J2 = np.zeros((n, n)); K2 = np.zeros((n, n))
+    J3 = np.zeros((n, n)); K3 = np.zeros((n, n))
+    rb.JK_direct(J1, K1, B, D1)
+    rb.JK_direct(J2, K2, B, D2)
+    rb.JK_direct(J3, K3, B, 2.0 * D1 - 3.0 * D2)
+    st = rb.jk_last_stats(B)
+    _say(f"  quartets={st.quartets:.4g} prim_quartets={st.primitive_quartets:.4g}")
+    assert np.max(np.abs(J1 - J1.T)) == 0.0 and np.max(np.abs(K1 - K1.T)) == 0.0
+    scale = np.abs(J1).max()
+    assert np.max(np.abs(J3 - (2 * J1 - 3 * J2))) < 1e-10 * scale
+    assert np.max(np.abs(K3 - (2 * K1 - 3 * K2))) < 1e-10 * np.abs(K1).max()
+    assert np.sum(D1 * J1) > 0.0      # Coulomb self-energy of a PSD density
+    assert np.sum(D1 * K1) > 0.0
