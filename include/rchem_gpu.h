@@ -137,7 +137,7 @@ typedef struct rchem_jk_stats {
   double flops;              /* algorithmic FP operations (FMA = 2) */
   double quartets_fp32;      /* of `quartets`, computed in FP32 */
 } rchem_jk_stats;
-int rchem_jk_last_stats(const rchem_ctx *ctx, rchem_jk_stats *st);
+int rchem_jk_last_stats(rchem_ctx *ctx, rchem_jk_stats *st);
 
 /* Per-class kernel timing.  With profiling on, every rchem_jk / rchem_jk_device
  * call records a HIP event pair around each (la lb|lc ld) class's launch on the

@@ -38,16 +38,17 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-template <int LA, int LB, int LC, int LD, int MODE, int BM>
-__global__ __launch_bounds__(256) void eri_kernel(const EriArgs a) {
+// One tile (one bra pair x up to 64 ket pairs) in arithmetic type R.  With
+// R = float the primitive work (Boys, VRR, HRR, contraction) runs in FP32:
+// geometric differences are formed in FP64 and rounded once, and the
+// integrals are widened to FP64 before digestion, whose sums and atomics stay
+// in FP64.
+template <int LA, int LB, int LC, int LD, int MODE, int BM, typename R>
+__device__ __forceinline__ void eri_tile(const EriArgs& a, const Tile t, const int lane) {
   using C = EriClass<LA, LB, LC, LD>;
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int L = C::L;
-
-  const int lane = threadIdx.x & 63;
-  const int tile_id = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (tile_id >= a.ntiles) return;  // uniform per wave
-  const Tile t = a.tiles[tile_id];
+  static_assert(BM == kBoysExact || sizeof(R) == 8, "compat Boys is FP64 only");
   const bool active = lane < t.count;
   const int kidx = t.ket0 + (active ? lane : 0);
   const int bidx = (MODE == kModeDiag) ? kidx : t.bra;
@@ -55,34 +56,44 @@ __global__ __launch_bounds__(256) void eri_kernel(const EriArgs a) {
   const PairRec bp = a.pairs[bidx];
   const PairRec kp = a.pairs[kidx];
 
-  double acc[C::NACC];
+  R acc[C::NACC];
 #pragma unroll
-  for (int i = 0; i < C::NACC; ++i) acc[i] = 0.0;
+  for (int i = 0; i < C::NACC; ++i) acc[i] = R(0);
+
+  const R* __restrict__ table;
+  if constexpr (sizeof(R) == 8) table = reinterpret_cast<const R*>(a.boys_table);
+  else table = reinterpret_cast<const R*>(a.boys_table_f);
 
   for (int i = 0; i < bp.np; ++i) {
     const PrimRec pb = a.prims[bp.p0 + i * bp.ps];
-    const double PAx = pb.px - bp.A[0], PAy = pb.py - bp.A[1], PAz = pb.pz - bp.A[2];
+    const R PAx = R(pb.px - bp.A[0]), PAy = R(pb.py - bp.A[1]), PAz = R(pb.pz - bp.A[2]);
+    const R zb = R(pb.zeta), kb = R(pb.kab), oo2z = R(pb.oo2z);
     for (int j = 0; j < kp.np; ++j) {
       const PrimRec pk = a.prims[kp.p0 + j * kp.ps];
-      const double ooze = 1.0 / (pb.zeta + pk.zeta);
-      const double roz = pk.zeta * ooze;  // rho / zeta
-      const double roe = pb.zeta * ooze;  // rho / eta
-      const double PQx = pb.px - pk.px, PQy = pb.py - pk.py, PQz = pb.pz - pk.pz;
-      const double T = pb.zeta * roz * (PQx * PQx + PQy * PQy + PQz * PQz);
-      double F[L + 1];
-      boys<L, BM, double>(T, a.boys_table, F);
-      const double pref = pb.kab * pk.kab * sqrt(ooze);
-      double G[L + 1];
+      const R zk = R(pk.zeta);
+      const R ooze = R(1) / (zb + zk);
+      const R roz = zk * ooze;  // rho / zeta
+      const R roe = zb * ooze;  // rho / eta
+      const R PQx = R(pb.px - pk.px), PQy = R(pb.py - pk.py), PQz = R(pb.pz - pk.pz);
+      const R T = zb * roz * (PQx * PQx + PQy * PQy + PQz * PQz);
+      R F[L + 1];
+      boys<L, BM, R>(T, table, F);
+      const R pref = kb * R(pk.kab) * sqrt(ooze);
+      R G[L + 1];
 #pragma unroll
       for (int m = 0; m <= L; ++m) G[m] = pref * F[m];
-      C::template vrr<double>(G, PAx, PAy, PAz, -roz * PQx, -roz * PQy, -roz * PQz,
-                              pk.px - kp.A[0], pk.py - kp.A[1], pk.pz - kp.A[2], roe * PQx,
-                              roe * PQy, roe * PQz, pb.oo2z, pk.oo2z, 0.5 * ooze, roz, roe, acc);
+      C::template vrr<R>(G, PAx, PAy, PAz, -roz * PQx, -roz * PQy, -roz * PQz,
+                         R(pk.px - kp.A[0]), R(pk.py - kp.A[1]), R(pk.pz - kp.A[2]), roe * PQx,
+                         roe * PQy, roe * PQz, oo2z, R(pk.oo2z), R(0.5) * ooze, roz, roe, acc);
     }
   }
 
+  R outr[C::NOUT];
+  C::template hrr<R>(acc, R(bp.AB[0]), R(bp.AB[1]), R(bp.AB[2]), R(kp.AB[0]), R(kp.AB[1]),
+                     R(kp.AB[2]), outr);
   double out[C::NOUT];
-  C::template hrr<double>(acc, bp.AB[0], bp.AB[1], bp.AB[2], kp.AB[0], kp.AB[1], kp.AB[2], out);
+#pragma unroll
+  for (int k = 0; k < C::NOUT; ++k) out[k] = static_cast<double>(outr[k]);
 
   const int fa = bp.fa, fb = bp.fb, fc = kp.fa, fd = kp.fb;
   const size_t n = static_cast<size_t>(a.nbf);
@@ -219,6 +230,23 @@ __global__ __launch_bounds__(256) void eri_kernel(const EriArgs a) {
   }
 }
 
+
+template <int LA, int LB, int LC, int LD, int MODE, int BM, typename R>
+__global__ __launch_bounds__(256) void eri_kernel(const EriArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (a.tile_range) {
+    // persistent: a fixed grid drains tiles [range[0], range[1]); every wave
+    // leaves the loop once its next index passes the end
+    const int begin = a.tile_range[0], end = a.tile_range[1];
+    for (int t = begin + wave; t < end; t += gridDim.x * 4)
+      eri_tile<LA, LB, LC, LD, MODE, BM, R>(a, a.tiles[t], lane);
+  } else {
+    if (wave >= a.ntiles) return;  // uniform per wave
+    eri_tile<LA, LB, LC, LD, MODE, BM, R>(a, a.tiles[wave], lane);
+  }
+}
+
 // J = Jh + Jh^T, K = Kh + Kh^T (the lower-half accumulators of eri_kernel).
 __global__ void symmetrize_kernel(const double* __restrict__ Jh, const double* __restrict__ Kh,
                                   double* __restrict__ J, double* __restrict__ K, int n) {
@@ -230,31 +258,43 @@ __global__ void symmetrize_kernel(const double* __restrict__ Jh, const double* _
   K[ij] = Kh[ij] + Kh[ji];
 }
 
-using EriLaunchFn = hipError_t (*)(const EriArgs&, hipStream_t);
+using EriLaunchFn = hipError_t (*)(const EriArgs&, int, hipStream_t);
 
-template <int LA, int LB, int LC, int LD, int MODE, int BM>
-hipError_t eri_launch(const EriArgs& a, hipStream_t s) {
-  if (a.ntiles <= 0) return hipSuccess;
-  const unsigned blocks = static_cast<unsigned>((a.ntiles + 3) / 4);
-  hipLaunchKernelGGL((eri_kernel<LA, LB, LC, LD, MODE, BM>), dim3(blocks), dim3(256), 0, s, a);
+// grid: ntiles/4 blocks for a host-sized launch, `blocks` for a persistent one
+template <int LA, int LB, int LC, int LD, int MODE, int BM, typename R>
+hipError_t eri_launch(const EriArgs& a, int blocks, hipStream_t s) {
+  unsigned grid;
+  if (a.tile_range) {
+    if (blocks <= 0) return hipSuccess;
+    grid = static_cast<unsigned>(blocks);
+  } else {
+    if (a.ntiles <= 0) return hipSuccess;
+    grid = static_cast<unsigned>((a.ntiles + 3) / 4);
+  }
+  hipLaunchKernelGGL((eri_kernel<LA, LB, LC, LD, MODE, BM, R>), dim3(grid), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
-// Launcher lookup for (class, mode, Boys mode); nullptr when the class is not
-// compiled into this library.
-EriLaunchFn eri_launcher(int la, int lb, int lc, int ld, int mode, int bm) {
-#define RCHEM_ERI_CASE(A, B, C, D)                                                      \
-  if (la == A && lb == B && lc == C && ld == D) {                                      \
-    if (bm == kBoysExact) {                                                            \
-      if (mode == kModeJK) return eri_launch<A, B, C, D, kModeJK, kBoysExact>;         \
-      if (mode == kModeDiag) return eri_launch<A, B, C, D, kModeDiag, kBoysExact>;     \
-      if (mode == kModeTensor) return eri_launch<A, B, C, D, kModeTensor, kBoysExact>; \
-    } else {                                                                           \
-      if (mode == kModeJK) return eri_launch<A, B, C, D, kModeJK, kBoysCompat>;        \
-      if (mode == kModeDiag) return eri_launch<A, B, C, D, kModeDiag, kBoysCompat>;    \
-      if (mode == kModeTensor) return eri_launch<A, B, C, D, kModeTensor, kBoysCompat>; \
-    }                                                                                  \
-    return nullptr;                                                                    \
+// Launcher lookup for (class, mode, Boys mode, precision); nullptr when that
+// combination is not compiled into this library.  fp32 = 1 selects the FP32
+// arithmetic path (JK mode, exact Boys only).
+EriLaunchFn eri_launcher(int la, int lb, int lc, int ld, int mode, int bm, int fp32) {
+#define RCHEM_ERI_CASE(A, B, C, D)                                                             \
+  if (la == A && lb == B && lc == C && ld == D) {                                             \
+    if (fp32) {                                                                               \
+      if (mode == kModeJK && bm == kBoysExact) return eri_launch<A, B, C, D, kModeJK, kBoysExact, float>; \
+      return nullptr;                                                                         \
+    }                                                                                         \
+    if (bm == kBoysExact) {                                                                   \
+      if (mode == kModeJK) return eri_launch<A, B, C, D, kModeJK, kBoysExact, double>;        \
+      if (mode == kModeDiag) return eri_launch<A, B, C, D, kModeDiag, kBoysExact, double>;    \
+      if (mode == kModeTensor) return eri_launch<A, B, C, D, kModeTensor, kBoysExact, double>; \
+    } else {                                                                                  \
+      if (mode == kModeJK) return eri_launch<A, B, C, D, kModeJK, kBoysCompat, double>;       \
+      if (mode == kModeDiag) return eri_launch<A, B, C, D, kModeDiag, kBoysCompat, double>;   \
+      if (mode == kModeTensor) return eri_launch<A, B, C, D, kModeTensor, kBoysCompat, double>; \
+    }                                                                                         \
+    return nullptr;                                                                           \
   }
   RCHEM_ERI_CLASSES_SP(RCHEM_ERI_CASE)
 #undef RCHEM_ERI_CASE

@@ -25,8 +25,12 @@ namespace rchem {
 #include "generated/basis_tables.inc"
 #include "generated/eri_flops.inc"
 
-using EriLaunchFn = hipError_t (*)(const EriArgs&, hipStream_t);
-EriLaunchFn eri_launcher(int la, int lb, int lc, int ld, int mode, int bm);
+using EriLaunchFn = hipError_t (*)(const EriArgs&, int, hipStream_t);
+EriLaunchFn eri_launcher(int la, int lb, int lc, int ld, int mode, int bm, int fp32);
+hipError_t dmax_launch(const double* D, size_t n2, unsigned long long* out, hipStream_t s);
+size_t plan_scan_bytes(int ntasks);
+hipError_t plan_launch(const PlanArgs& a, int* off64, int* off32, Tile* tiles64, Tile* tiles32,
+                       int cap, int* ranges, void* scan_tmp, size_t scan_bytes, hipStream_t s);
 hipError_t symmetrize_launch(const double* Jh, const double* Kh, double* J, double* K, int n,
                              hipStream_t s);
 hipError_t fill_launch(double* p, size_t n, double v, hipStream_t s);
@@ -110,12 +114,13 @@ struct Bucket {
 struct QClass {
   int X, Y;           // bra / ket pair classes
   int la, lb, lc, ld;
-  int tile_begin, ntiles;
-  double quartets, prim_quartets, flops;
+  int tile_begin, ntiles;  // host-tiled launches
+  int cap;                 // planner: upper bound on tiles of one precision
+  double quartets, prim_quartets, flops;  // all-FP64 plan
 };
 
 struct ClassTimer {
-  int la, lb, lc, ld;
+  int la, lb, lc, ld, fp32;
   double quartets, prim_quartets, flops;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
   double ms = 0.0;
@@ -150,6 +155,9 @@ struct rchem_ctx {
   // laid-out pair list for a screening threshold
   double pairs_tau = -1.0;
   std::vector<PairRec> pairs;
+  std::vector<double> pairQ;                      // Schwarz bound, layout order
+  std::vector<std::pair<int, int>> pair_shells;   // (a, b), layout order
+  size_t nprims = 0;
   std::vector<Bucket> buckets[kNumPairClasses];
   int class_begin[kNumPairClasses + 1] = {0};
   DevBuf pairs_d, prims_d;
@@ -161,9 +169,19 @@ struct rchem_ctx {
   DevBuf tiles_d;
   rchem_jk_stats planned{};  // statistics of the planned tile list
 
-  DevBuf boys_d;
+  // device planner
+  double plan_tau = -1.0;
+  std::vector<PlanClass> pclasses;
+  int plan_tasks = 0, plan_cap = 0;
+  size_t plan_scan_bytes = 0;
+  DevBuf plan_classes_d, plan_buckets_d, plan_q_d, plan_cnt64, plan_cnt32, plan_off64,
+      plan_off32, plan_lo, plan_mid, plan_hi, plan_tiles64, plan_tiles32, plan_ranges,
+      plan_stats, plan_dmax, plan_scan_tmp;
+
+  DevBuf boys_d, boys_f_d;
   DevBuf work_Jh, work_Kh, work_D, work_J, work_K;
   rchem_jk_stats last_stats{};
+  bool last_mixed = false;
 
   bool profiling = false;
   std::vector<ClassTimer> timers;
@@ -251,20 +269,60 @@ int finish_basis(rchem_ctx* ctx) {
   ctx->have_schwarz = false;
   ctx->pairs_tau = -1.0;
   ctx->tiles_tau = -1.0;
+  ctx->plan_tau = -1.0;
   ctx->all_pairs.clear();
   return RCHEM_OK;
 }
 
-// Lays out the pairs selected by `keep` (sorted within buckets by Q
-// descending) and uploads pair and primitive-pair records.
-int layout_pairs(rchem_ctx* ctx, const std::vector<PairHost>& sel) {
-  std::vector<PairHost> v = sel;
+// Primitive pairs of shell pair (a, b) whose |K_ab| reaches eps (all of them
+// when eps <= 0), in (i of a, j of b) order.
+void make_prims(const rchem_ctx* ctx, int a, int b, double eps, std::vector<PrimRec>* out) {
+  const Shell& sa = ctx->shells[a];
+  const Shell& sb = ctx->shells[b];
+  double AB2 = 0.0;
+  for (int x = 0; x < 3; ++x) AB2 += (sa.xyz[x] - sb.xyz[x]) * (sa.xyz[x] - sb.xyz[x]);
+  out->clear();
+  for (int ip = 0; ip < sa.nprim; ++ip)
+    for (int jp = 0; jp < sb.nprim; ++jp) {
+      const double al = ctx->exps[sa.p0 + ip], be = ctx->exps[sb.p0 + jp];
+      const double z = al + be;
+      PrimRec pr;
+      pr.zeta = z;
+      pr.px = (al * sa.xyz[0] + be * sb.xyz[0]) / z;
+      pr.py = (al * sa.xyz[1] + be * sb.xyz[1]) / z;
+      pr.pz = (al * sa.xyz[2] + be * sb.xyz[2]) / z;
+      pr.kab = ctx->ccoef[sa.p0 + ip] * ctx->ccoef[sb.p0 + jp] * std::sqrt(2.0) *
+               std::pow(M_PI, 1.25) / z * std::exp(-al * be / z * AB2);
+      pr.oo2z = 0.5 / z;
+      if (eps > 0.0 && std::fabs(pr.kab) < eps) continue;
+      out->push_back(pr);
+    }
+}
+
+// Lays out the selected pairs -- sorted by (class, primitive-pair count,
+// Schwarz bound descending) -- with their primitive pairs transposed per
+// bucket, uploads both, and records the bounds in layout order in ctx->pairQ.
+// Primitive pairs with |K_ab| < prim_eps are dropped (prim_eps <= 0 keeps all);
+// a pair left with none is dropped.
+int layout_pairs(rchem_ctx* ctx, const std::vector<PairHost>& sel, double prim_eps) {
+  std::vector<PairHost> v;
+  v.reserve(sel.size());
+  std::vector<PrimRec> tmp;
+  for (const PairHost& p : sel) {
+    make_prims(ctx, p.a, p.b, prim_eps, &tmp);
+    if (tmp.empty()) continue;
+    PairHost q = p;
+    q.np = static_cast<int>(tmp.size());
+    v.push_back(q);
+  }
   std::stable_sort(v.begin(), v.end(), [](const PairHost& x, const PairHost& y) {
     if (x.cls != y.cls) return x.cls < y.cls;
     if (x.np != y.np) return x.np > y.np;
     return x.Q > y.Q;
   });
   ctx->pairs.assign(v.size(), PairRec{});
+  ctx->pairQ.resize(v.size());
+  ctx->pair_shells.resize(v.size());
   std::vector<PrimRec> prims;
   for (int c = 0; c < kNumPairClasses; ++c) ctx->buckets[c].clear();
   size_t i = 0;
@@ -285,11 +343,9 @@ int layout_pairs(rchem_ctx* ctx, const std::vector<PairHost>& sel) {
       const Shell& sa = ctx->shells[ph.a];
       const Shell& sb = ctx->shells[ph.b];
       PairRec& r = ctx->pairs[pos + q];
-      double AB2 = 0.0;
       for (int x = 0; x < 3; ++x) {
         r.A[x] = sa.xyz[x];
         r.AB[x] = sa.xyz[x] - sb.xyz[x];
-        AB2 += r.AB[x] * r.AB[x];
       }
       r.fa = sa.f0;
       r.fb = sb.f0;
@@ -297,23 +353,14 @@ int layout_pairs(rchem_ctx* ctx, const std::vector<PairHost>& sel) {
       r.p0 = static_cast<int>(base) + q;
       r.ps = n;
       r.flags = (ph.a != ph.b) ? 1 : 0;
-      int k = 0;
-      for (int ip = 0; ip < sa.nprim; ++ip)
-        for (int jp = 0; jp < sb.nprim; ++jp, ++k) {
-          const double al = ctx->exps[sa.p0 + ip], be = ctx->exps[sb.p0 + jp];
-          const double z = al + be;
-          PrimRec& pr = prims[base + static_cast<size_t>(k) * n + q];
-          pr.zeta = z;
-          pr.px = (al * sa.xyz[0] + be * sb.xyz[0]) / z;
-          pr.py = (al * sa.xyz[1] + be * sb.xyz[1]) / z;
-          pr.pz = (al * sa.xyz[2] + be * sb.xyz[2]) / z;
-          pr.kab = ctx->ccoef[sa.p0 + ip] * ctx->ccoef[sb.p0 + jp] * std::sqrt(2.0) *
-                   std::pow(M_PI, 1.25) / z * std::exp(-al * be / z * AB2);
-          pr.oo2z = 0.5 / z;
-        }
+      ctx->pairQ[pos + q] = ph.Q;
+      ctx->pair_shells[pos + q] = std::make_pair(ph.a, ph.b);
+      make_prims(ctx, ph.a, ph.b, prim_eps, &tmp);
+      for (int k = 0; k < np; ++k) prims[base + static_cast<size_t>(k) * n + q] = tmp[k];
     }
     pos = end;
   }
+  ctx->nprims = prims.size();
   HIPC(ctx->pairs_d.ensure(sizeof(PairRec) * std::max<size_t>(ctx->pairs.size(), 1)));
   HIPC(ctx->prims_d.ensure(sizeof(PrimRec) * std::max<size_t>(prims.size(), 1)));
   if (!ctx->pairs.empty())
@@ -357,7 +404,7 @@ int compute_schwarz(rchem_ctx* ctx) {
       all.push_back(PairHost{x, y, pair_class(ctx->shells[x].l, ctx->shells[y].l),
                              ctx->shells[x].nprim * ctx->shells[y].nprim, 0.0});
     }
-  TRY(layout_pairs(ctx, all));
+  TRY(layout_pairs(ctx, all, 0.0));
   // diagonal tiles: 64 consecutive pairs of one bucket, bra = ket per lane
   ctx->tiles.clear();
   std::vector<std::pair<int, int>> ranges;  // per class: tile range
@@ -375,7 +422,7 @@ int compute_schwarz(rchem_ctx* ctx) {
     if (ranges[X].second == 0) continue;
     int l1, l2;
     class_ls(X, &l1, &l2);
-    EriLaunchFn fn = eri_launcher(l1, l2, l1, l2, kModeDiag, kBoysExact);
+    EriLaunchFn fn = eri_launcher(l1, l2, l1, l2, kModeDiag, kBoysExact, 0);
     if (!fn) return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class not compiled in");
     EriArgs a{};
     a.pairs = ctx->pairs_d.as<PairRec>();
@@ -384,30 +431,29 @@ int compute_schwarz(rchem_ctx* ctx) {
     a.ntiles = ranges[X].second;
     a.nbf = ctx->nbf;
     a.boys_table = ctx->boys_d.as<double>();
+    a.boys_table_f = ctx->boys_f_d.as<float>();
     a.out = q.as<double>();
-    HIPC(fn(a, ctx->stream));
+    HIPC(fn(a, 0, ctx->stream));
   }
   std::vector<double> Q(ctx->pairs.size());
   if (!Q.empty())
     HIPC(hipMemcpyAsync(Q.data(), q.p, sizeof(double) * Q.size(), hipMemcpyDeviceToHost,
                         ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
-  // map back: layout_pairs sorted `all`; recover (a, b) from the records' order
-  std::vector<PairHost> sorted = all;
-  std::stable_sort(sorted.begin(), sorted.end(), [](const PairHost& x, const PairHost& y) {
-    if (x.cls != y.cls) return x.cls < y.cls;
-    if (x.np != y.np) return x.np > y.np;
-    return x.Q > y.Q;
-  });
+  // the records are in layout order; ctx->pair_shells names each one's shells
+  std::vector<PairHost> sorted(ctx->pairs.size());
   for (size_t i = 0; i < sorted.size(); ++i) {
     if (!std::isfinite(Q[i]))
       return ctx->fail(RCHEM_ERR_NONFINITE, "non-finite Schwarz bound");
-    sorted[i].Q = Q[i];
+    const int a = ctx->pair_shells[i].first, b = ctx->pair_shells[i].second;
+    sorted[i] = PairHost{a, b, pair_class(ctx->shells[a].l, ctx->shells[b].l),
+                         ctx->shells[a].nprim * ctx->shells[b].nprim, Q[i]};
   }
   ctx->all_pairs = std::move(sorted);
   ctx->have_schwarz = true;
   ctx->pairs_tau = -1.0;
   ctx->tiles_tau = -1.0;
+  ctx->plan_tau = -1.0;
   return RCHEM_OK;
 }
 
@@ -422,100 +468,208 @@ int flops_for(int la, int lb, int lc, int ld, int* vrr, int* hrr) {
   return 0;
 }
 
-// Pair list and quartet tiles for screening threshold tau.
-int prepare(rchem_ctx* ctx, double tau) {
+// Pair list (Schwarz- and primitive-screened) for threshold tau.
+int prepare_pairs(rchem_ctx* ctx, double tau) {
   TRY(compute_schwarz(ctx));
-  if (ctx->pairs_tau != tau) {
-    double qmax = 0.0;
-    for (const auto& p : ctx->all_pairs) qmax = std::max(qmax, p.Q);
-    std::vector<PairHost> sel;
-    sel.reserve(ctx->all_pairs.size());
-    for (const auto& p : ctx->all_pairs)
-      if (tau <= 0.0 || p.Q * qmax >= tau) sel.push_back(p);
-    TRY(layout_pairs(ctx, sel));
-    ctx->pairs_tau = tau;
-    ctx->tiles_tau = -1.0;
-  }
-  if (ctx->tiles_tau == tau) return RCHEM_OK;
-  // Q of the laid-out pairs, in order
-  std::vector<double> Q(ctx->pairs.size());
-  {
-    std::vector<PairHost> sel;
-    double qmax = 0.0;
-    for (const auto& p : ctx->all_pairs) qmax = std::max(qmax, p.Q);
-    for (const auto& p : ctx->all_pairs)
-      if (tau <= 0.0 || p.Q * qmax >= tau) sel.push_back(p);
-    std::stable_sort(sel.begin(), sel.end(), [](const PairHost& x, const PairHost& y) {
-      if (x.cls != y.cls) return x.cls < y.cls;
-      if (x.np != y.np) return x.np > y.np;
-      return x.Q > y.Q;
-    });
-    for (size_t i = 0; i < sel.size(); ++i) Q[i] = sel[i].Q;
-  }
-  ctx->tiles.clear();
-  ctx->qclasses.clear();
-  rchem_jk_stats st{};
-  for (int X = 0; X < kNumPairClasses; ++X) {
-    for (int Y = 0; Y <= X; ++Y) {
-      QClass qc{};
-      qc.X = X;
-      qc.Y = Y;
-      class_ls(X, &qc.la, &qc.lb);
-      class_ls(Y, &qc.lc, &qc.ld);
-      qc.tile_begin = static_cast<int>(ctx->tiles.size());
-      int fv = 0, fh = 0;
-      flops_for(qc.la, qc.lb, qc.lc, qc.ld, &fv, &fh);
-      const int L = qc.la + qc.lb + qc.lc + qc.ld;
-      const double per_prim = fv + 68.0 + 4.0 * L;
-      const double nint = static_cast<double>(ncart(qc.la) * ncart(qc.lb) * ncart(qc.lc) *
-                                              ncart(qc.ld));
-      const rchem_jk_stats before = st;
-      for (int i = ctx->class_begin[X]; i < ctx->class_begin[X + 1]; ++i) {
-        const double need = (tau > 0.0 && Q[i] > 0.0) ? tau / Q[i] : 0.0;
-        if (tau > 0.0 && Q[i] <= 0.0) continue;
-        for (const Bucket& bk : ctx->buckets[Y]) {
-          int end = bk.end;
-          if (X == Y) {
-            if (bk.begin > i) continue;
-            end = std::min(end, i + 1);
-          }
-          if (tau > 0.0) {
-            // Q descending within the bucket: count j with Q_j >= need
-            int lo = bk.begin, hi = end;
-            while (lo < hi) {
-              const int mid = (lo + hi) / 2;
-              if (Q[mid] >= need) lo = mid + 1; else hi = mid;
-            }
-            end = lo;
-          }
-          const int cnt = end - bk.begin;
-          if (cnt <= 0) continue;
-          for (int j = bk.begin; j < end; j += 64)
-            ctx->tiles.push_back(Tile{i, j, std::min(64, end - j), 0});
-          st.quartets += cnt;
-          st.primitive_quartets += static_cast<double>(cnt) * ctx->pairs[i].np * bk.np;
-          st.integrals += cnt * nint;
-          st.flops += static_cast<double>(cnt) * ctx->pairs[i].np * bk.np * per_prim +
-                      static_cast<double>(cnt) * fh;
-        }
-      }
-      qc.ntiles = static_cast<int>(ctx->tiles.size()) - qc.tile_begin;
-      qc.quartets = st.quartets - before.quartets;
-      qc.prim_quartets = st.primitive_quartets - before.primitive_quartets;
-      qc.flops = st.flops - before.flops;
-      if (qc.ntiles > 0) {
-        if (!eri_launcher(qc.la, qc.lb, qc.lc, qc.ld, kModeJK, kBoysExact))
-          return ctx->fail(RCHEM_ERR_UNSUPPORTED,
-                           "ERI class (" + std::to_string(qc.la) + std::to_string(qc.lb) + "|" +
-                               std::to_string(qc.lc) + std::to_string(qc.ld) +
-                               ") not compiled into this library");
-        ctx->qclasses.push_back(qc);
-      }
+  if (ctx->pairs_tau == tau) return RCHEM_OK;
+  double qmax = 0.0;
+  for (const auto& p : ctx->all_pairs) qmax = std::max(qmax, p.Q);
+  std::vector<PairHost> sel;
+  sel.reserve(ctx->all_pairs.size());
+  for (const auto& p : ctx->all_pairs)
+    if (tau <= 0.0 || p.Q * qmax >= tau) sel.push_back(p);
+  // A primitive pair enters an integral through K_ab K_cd / sqrt(zeta + eta)
+  // times factors of order one; dropping |K_ab| < 1e-3 tau / K_max keeps every
+  // dropped primitive quartet three orders below the quartet threshold.
+  double kmax = 0.0;
+  if (tau > 0.0) {
+    std::vector<PrimRec> tmp;
+    for (const auto& p : sel) {
+      make_prims(ctx, p.a, p.b, 0.0, &tmp);
+      for (const auto& pr : tmp) kmax = std::max(kmax, std::fabs(pr.kab));
     }
   }
+  const double prim_eps = (tau > 0.0 && kmax > 0.0) ? 1e-3 * tau / kmax : 0.0;
+  TRY(layout_pairs(ctx, sel, prim_eps));
+  ctx->pairs_tau = tau;
+  ctx->tiles_tau = -1.0;
+  ctx->plan_tau = -1.0;
+  return RCHEM_OK;
+}
+
+// Walks every (quartet class, bra pair, ket bucket) task of threshold tau and
+// calls f(class, bra, bucket, ket_begin, ket_end) for its Schwarz survivors.
+template <typename Fn>
+void for_each_task(const rchem_ctx* ctx, double tau, Fn f) {
+  const std::vector<double>& Q = ctx->pairQ;
+  for (int X = 0; X < kNumPairClasses; ++X)
+    for (int Y = 0; Y <= X; ++Y)
+      for (int i = ctx->class_begin[X]; i < ctx->class_begin[X + 1]; ++i)
+        for (size_t b = 0; b < ctx->buckets[Y].size(); ++b) {
+          const Bucket& bk = ctx->buckets[Y][b];
+          int lo = bk.begin, hi = bk.end;
+          if (X == Y) hi = (lo > i) ? lo : std::min(hi, i + 1);
+          if (tau > 0.0) {
+            if (Q[i] <= 0.0) {
+              hi = lo;
+            } else {
+              const double need = tau / Q[i];
+              int l = lo, h = hi;
+              while (l < h) {
+                const int m = (l + h) / 2;
+                if (Q[m] >= need) l = m + 1; else h = m;
+              }
+              hi = l;
+            }
+          }
+          f(X, Y, i, static_cast<int>(b), lo, hi);
+        }
+}
+
+QClass make_qclass(int X, int Y) {
+  QClass qc{};
+  qc.X = X;
+  qc.Y = Y;
+  class_ls(X, &qc.la, &qc.lb);
+  class_ls(Y, &qc.lc, &qc.ld);
+  return qc;
+}
+
+double per_prim_flops(const QClass& qc, int* hrr) {
+  int fv = 0, fh = 0;
+  flops_for(qc.la, qc.lb, qc.lc, qc.ld, &fv, &fh);
+  *hrr = fh;
+  return fv + 68.0 + 4.0 * (qc.la + qc.lb + qc.lc + qc.ld);
+}
+
+// Host-built tile list (used for the dense tensor, where tau = 0 and systems
+// are small).
+int prepare_host_tiles(rchem_ctx* ctx, double tau) {
+  TRY(prepare_pairs(ctx, tau));
+  if (ctx->tiles_tau == tau) return RCHEM_OK;
+  ctx->tiles.clear();
+  ctx->qclasses.clear();
+  int curX = -1, curY = -1;
+  QClass qc{};
+  auto flush = [&]() {
+    qc.ntiles = static_cast<int>(ctx->tiles.size()) - qc.tile_begin;
+    if (curX >= 0 && qc.ntiles > 0) ctx->qclasses.push_back(qc);
+  };
+  for_each_task(ctx, tau, [&](int X, int Y, int i, int, int lo, int hi) {
+    if (X != curX || Y != curY) {
+      flush();
+      curX = X;
+      curY = Y;
+      qc = make_qclass(X, Y);
+      qc.tile_begin = static_cast<int>(ctx->tiles.size());
+    }
+    for (int j = lo; j < hi; j += 64) ctx->tiles.push_back(Tile{i, j, std::min(64, hi - j), 0});
+  });
+  flush();
+  for (const QClass& q : ctx->qclasses)
+    if (!eri_launcher(q.la, q.lb, q.lc, q.ld, kModeTensor, kBoysExact, 0))
+      return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class (" + std::to_string(q.la) +
+                                                  std::to_string(q.lb) + "|" +
+                                                  std::to_string(q.lc) + std::to_string(q.ld) +
+                                                  ") not compiled into this library");
   TRY(upload_tiles(ctx));
-  ctx->planned = st;
   ctx->tiles_tau = tau;
+  return RCHEM_OK;
+}
+
+// Device-planner tables for threshold tau: classes, buckets, capacities and
+// the FP64 statistics (exact for all-FP64 builds).
+int prepare_plan(rchem_ctx* ctx, double tau) {
+  TRY(prepare_pairs(ctx, tau));
+  if (ctx->plan_tau == tau) return RCHEM_OK;
+  ctx->pclasses.clear();
+  ctx->qclasses.clear();
+  std::vector<int4> bks;
+  int tasks = 0;
+  for (int X = 0; X < kNumPairClasses; ++X)
+    for (int Y = 0; Y <= X; ++Y) {
+      const int nbra = ctx->class_begin[X + 1] - ctx->class_begin[X];
+      const int nbk = static_cast<int>(ctx->buckets[Y].size());
+      if (nbra == 0 || nbk == 0) continue;
+      PlanClass pc{};
+      pc.bra_begin = ctx->class_begin[X];
+      pc.bra_end = ctx->class_begin[X + 1];
+      pc.bk_off = static_cast<int>(bks.size());
+      pc.nbk = nbk;
+      pc.task_begin = tasks;
+      pc.same = X == Y;
+      for (const Bucket& bk : ctx->buckets[Y]) bks.push_back(make_int4(bk.begin, bk.end, bk.np, 0));
+      tasks += nbra * nbk;
+      ctx->pclasses.push_back(pc);
+      QClass qc = make_qclass(X, Y);
+      ctx->qclasses.push_back(qc);
+    }
+  // capacities (one partial tile more per task covers the FP64/FP32 split) and stats
+  std::vector<double> cap(ctx->qclasses.size(), 0.0);
+  rchem_jk_stats st{};
+  std::vector<int> cls_index(kNumPairClasses * kNumPairClasses, -1);
+  for (size_t c = 0; c < ctx->qclasses.size(); ++c)
+    cls_index[ctx->qclasses[c].X * kNumPairClasses + ctx->qclasses[c].Y] = static_cast<int>(c);
+  for (auto& qc : ctx->qclasses) qc.quartets = qc.prim_quartets = qc.flops = 0.0;
+  for_each_task(ctx, tau, [&](int X, int Y, int i, int b, int lo, int hi) {
+    const int c = cls_index[X * kNumPairClasses + Y];
+    if (c < 0) return;
+    const int cnt = hi - lo;
+    cap[c] += (cnt + 63) / 64 + 1;
+    if (cnt <= 0) return;
+    QClass& qc = ctx->qclasses[c];
+    const double pq = static_cast<double>(cnt) * ctx->pairs[i].np * ctx->buckets[Y][b].np;
+    qc.quartets += cnt;
+    qc.prim_quartets += pq;
+  });
+  double total_cap = 0.0;
+  for (size_t c = 0; c < ctx->qclasses.size(); ++c) {
+    QClass& qc = ctx->qclasses[c];
+    int fh = 0;
+    const double fp = per_prim_flops(qc, &fh);
+    qc.flops = qc.prim_quartets * fp + qc.quartets * fh;
+    qc.cap = static_cast<int>(std::min(cap[c], 2.0e9));
+    total_cap += cap[c];
+    st.quartets += qc.quartets;
+    st.primitive_quartets += qc.prim_quartets;
+    st.integrals += qc.quartets * ncart(qc.la) * ncart(qc.lb) * ncart(qc.lc) * ncart(qc.ld);
+    st.flops += qc.flops;
+    if (qc.quartets > 0 && !eri_launcher(qc.la, qc.lb, qc.lc, qc.ld, kModeJK, kBoysExact, 0))
+      return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class (" + std::to_string(qc.la) +
+                                                  std::to_string(qc.lb) + "|" +
+                                                  std::to_string(qc.lc) + std::to_string(qc.ld) +
+                                                  ") not compiled into this library");
+  }
+  if (total_cap > 2.0e9) return ctx->fail(RCHEM_ERR_ALLOC, "quartet tile list too large");
+  ctx->plan_cap = static_cast<int>(total_cap) + 1;
+  ctx->plan_tasks = tasks;
+  const size_t nt = static_cast<size_t>(tasks) + 1;
+  HIPC(ctx->plan_classes_d.ensure(sizeof(PlanClass) * std::max<size_t>(ctx->pclasses.size(), 1)));
+  HIPC(ctx->plan_buckets_d.ensure(sizeof(int4) * std::max<size_t>(bks.size(), 1)));
+  HIPC(ctx->plan_q_d.ensure(sizeof(double) * std::max<size_t>(ctx->pairQ.size(), 1)));
+  for (DevBuf* b : {&ctx->plan_cnt64, &ctx->plan_cnt32, &ctx->plan_off64, &ctx->plan_off32,
+                    &ctx->plan_lo, &ctx->plan_mid, &ctx->plan_hi})
+    HIPC(b->ensure(sizeof(int) * nt));
+  HIPC(ctx->plan_tiles64.ensure(sizeof(Tile) * static_cast<size_t>(ctx->plan_cap)));
+  HIPC(ctx->plan_tiles32.ensure(sizeof(Tile) * static_cast<size_t>(ctx->plan_cap)));
+  HIPC(ctx->plan_ranges.ensure(sizeof(int) * 4 * std::max<size_t>(ctx->pclasses.size(), 1)));
+  HIPC(ctx->plan_stats.ensure(sizeof(double) * 4 * std::max<size_t>(ctx->pclasses.size(), 1)));
+  HIPC(ctx->plan_dmax.ensure(sizeof(double)));
+  ctx->plan_scan_bytes = plan_scan_bytes(tasks);
+  HIPC(ctx->plan_scan_tmp.ensure(std::max<size_t>(ctx->plan_scan_bytes, 8)));
+  if (!ctx->pclasses.empty())
+    HIPC(hipMemcpyAsync(ctx->plan_classes_d.p, ctx->pclasses.data(),
+                        sizeof(PlanClass) * ctx->pclasses.size(), hipMemcpyHostToDevice,
+                        ctx->stream));
+  if (!bks.empty())
+    HIPC(hipMemcpyAsync(ctx->plan_buckets_d.p, bks.data(), sizeof(int4) * bks.size(),
+                        hipMemcpyHostToDevice, ctx->stream));
+  if (!ctx->pairQ.empty())
+    HIPC(hipMemcpyAsync(ctx->plan_q_d.p, ctx->pairQ.data(), sizeof(double) * ctx->pairQ.size(),
+                        hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  ctx->planned = st;
+  ctx->plan_tau = tau;
   return RCHEM_OK;
 }
 
@@ -524,18 +678,22 @@ int check_opts(rchem_ctx* ctx, const rchem_opts* o) {
     return ctx->fail(RCHEM_ERR_INVALID, "unknown boys_mode");
   if (o->precision != RCHEM_PREC_FP64 && o->precision != RCHEM_PREC_MIXED)
     return ctx->fail(RCHEM_ERR_INVALID, "unknown precision");
-  if (o->precision == RCHEM_PREC_MIXED)
-    return ctx->fail(RCHEM_ERR_UNSUPPORTED, "mixed precision not available in this build");
+  if (o->precision == RCHEM_PREC_MIXED && o->boys_mode != RCHEM_BOYS_EXACT)
+    return ctx->fail(RCHEM_ERR_UNSUPPORTED, "mixed precision needs the exact Boys function");
+  if (o->precision == RCHEM_PREC_MIXED && !(o->mixed_thresh >= 0.0))
+    return ctx->fail(RCHEM_ERR_INVALID, "mixed_thresh must be >= 0");
   if (!(o->schwarz_thresh >= 0.0) || !std::isfinite(o->schwarz_thresh))
     return ctx->fail(RCHEM_ERR_INVALID, "schwarz_thresh must be finite and >= 0");
   return RCHEM_OK;
 }
 
-ClassTimer* timer_for(rchem_ctx* ctx, const QClass& qc) {
+ClassTimer* timer_for(rchem_ctx* ctx, const QClass& qc, int fp32) {
   for (auto& t : ctx->timers)
-    if (t.la == qc.la && t.lb == qc.lb && t.lc == qc.lc && t.ld == qc.ld) return &t;
+    if (t.la == qc.la && t.lb == qc.lb && t.lc == qc.lc && t.ld == qc.ld && t.fp32 == fp32)
+      return &t;
   ClassTimer t;
   t.la = qc.la; t.lb = qc.lb; t.lc = qc.lc; t.ld = qc.ld;
+  t.fp32 = fp32;
   ctx->timers.push_back(t);
   return &ctx->timers.back();
 }
@@ -567,54 +725,113 @@ int collect_timers(rchem_ctx* ctx) {
   return RCHEM_OK;
 }
 
-// Launches the ERI kernels of every quartet class in `mode`.
-int run_eri(rchem_ctx* ctx, int mode, int bm, const double* D, double* Jh, double* Kh,
-            double* out) {
+EriArgs base_args(const rchem_ctx* ctx) {
+  EriArgs a{};
+  a.pairs = ctx->pairs_d.as<PairRec>();
+  a.prims = ctx->prims_d.as<PrimRec>();
+  a.nbf = ctx->nbf;
+  a.boys_table = ctx->boys_d.as<double>();
+  a.boys_table_f = ctx->boys_f_d.as<float>();
+  return a;
+}
+
+// Host-tiled launch of every class in `mode` (diagonal / tensor modes).
+int run_eri_host(rchem_ctx* ctx, int mode, int bm, double* out) {
   for (const QClass& qc : ctx->qclasses) {
-    EriLaunchFn fn = eri_launcher(qc.la, qc.lb, qc.lc, qc.ld, mode, bm);
+    EriLaunchFn fn = eri_launcher(qc.la, qc.lb, qc.lc, qc.ld, mode, bm, 0);
     if (!fn) return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class not compiled in");
-    EriArgs a{};
-    a.pairs = ctx->pairs_d.as<PairRec>();
-    a.prims = ctx->prims_d.as<PrimRec>();
+    EriArgs a = base_args(ctx);
     a.tiles = ctx->tiles_d.as<Tile>() + qc.tile_begin;
     a.ntiles = qc.ntiles;
-    a.nbf = ctx->nbf;
-    a.boys_table = ctx->boys_d.as<double>();
-    a.D = D;
-    a.J = Jh;
-    a.K = Kh;
     a.out = out;
-    const bool timed = ctx->profiling && mode == kModeJK;
-    std::pair<hipEvent_t, hipEvent_t> ev{};
-    if (timed) {
-      TRY(take_events(ctx, &ev));
-      HIPC(hipEventRecord(ev.first, ctx->stream));
-    }
-    HIPC(fn(a, ctx->stream));
-    if (timed) {
-      HIPC(hipEventRecord(ev.second, ctx->stream));
-      ClassTimer* t = timer_for(ctx, qc);
-      t->quartets = qc.quartets;
-      t->prim_quartets = qc.prim_quartets;
-      t->flops = qc.flops;
-      t->pending.push_back(ev);
-    }
+    HIPC(fn(a, 0, ctx->stream));
+  }
+  return RCHEM_OK;
+}
+
+constexpr int kPersistentBlocks = 2048;  // 256 CUs x 8 workgroups of 4 waves
+
+int launch_timed(rchem_ctx* ctx, const QClass& qc, int fp32, EriLaunchFn fn, const EriArgs& a,
+                 int blocks) {
+  const bool timed = ctx->profiling;
+  std::pair<hipEvent_t, hipEvent_t> ev{};
+  if (timed) {
+    TRY(take_events(ctx, &ev));
+    HIPC(hipEventRecord(ev.first, ctx->stream));
+  }
+  HIPC(fn(a, blocks, ctx->stream));
+  if (timed) {
+    HIPC(hipEventRecord(ev.second, ctx->stream));
+    ClassTimer* t = timer_for(ctx, qc, fp32);
+    t->quartets = qc.quartets;
+    t->prim_quartets = qc.prim_quartets;
+    t->flops = qc.flops;
+    t->pending.push_back(ev);
   }
   return RCHEM_OK;
 }
 
 int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rchem_opts* o) {
-  TRY(prepare(ctx, o->schwarz_thresh));
+  TRY(prepare_plan(ctx, o->schwarz_thresh));
   const size_t n2 = static_cast<size_t>(ctx->nbf) * ctx->nbf;
+  const bool mixed = o->precision == RCHEM_PREC_MIXED;
   HIPC(ctx->work_Jh.ensure(n2 * sizeof(double)));
   HIPC(ctx->work_Kh.ensure(n2 * sizeof(double)));
   HIPC(fill_launch(ctx->work_Jh.as<double>(), n2, 0.0, ctx->stream));
   HIPC(fill_launch(ctx->work_Kh.as<double>(), n2, 0.0, ctx->stream));
-  TRY(run_eri(ctx, kModeJK, o->boys_mode, dD, ctx->work_Jh.as<double>(),
-              ctx->work_Kh.as<double>(), nullptr));
+  const int nc = static_cast<int>(ctx->pclasses.size());
+  HIPC(fill_launch(ctx->plan_stats.as<double>(), 4 * static_cast<size_t>(std::max(nc, 1)), 0.0,
+                   ctx->stream));
+  if (mixed) {
+    HIPC(fill_launch(ctx->plan_dmax.as<double>(), 1, 0.0, ctx->stream));
+    HIPC(dmax_launch(dD, n2, ctx->plan_dmax.as<unsigned long long>(), ctx->stream));
+  }
+  PlanArgs pa{};
+  pa.classes = ctx->plan_classes_d.as<PlanClass>();
+  pa.nclasses = nc;
+  pa.ntasks = ctx->plan_tasks;
+  pa.buckets = ctx->plan_buckets_d.as<int4>();
+  pa.Q = ctx->plan_q_d.as<double>();
+  pa.pairs = ctx->pairs_d.as<PairRec>();
+  pa.tau = o->schwarz_thresh;
+  pa.mixed_thresh = mixed ? o->mixed_thresh : 0.0;
+  pa.dmax = ctx->plan_dmax.as<double>();
+  pa.cnt64 = ctx->plan_cnt64.as<int>();
+  pa.cnt32 = ctx->plan_cnt32.as<int>();
+  pa.lo = ctx->plan_lo.as<int>();
+  pa.mid = ctx->plan_mid.as<int>();
+  pa.hi = ctx->plan_hi.as<int>();
+  pa.stats = ctx->plan_stats.as<double>();
+  if (nc > 0)
+    HIPC(plan_launch(pa, ctx->plan_off64.as<int>(), ctx->plan_off32.as<int>(),
+                     ctx->plan_tiles64.as<Tile>(), ctx->plan_tiles32.as<Tile>(), ctx->plan_cap,
+                     ctx->plan_ranges.as<int>(), ctx->plan_scan_tmp.p, ctx->plan_scan_bytes,
+                     ctx->stream));
+  for (int c = 0; c < nc; ++c) {
+    const QClass& qc = ctx->qclasses[c];
+    if (qc.quartets <= 0) continue;
+    const int blocks = std::min(kPersistentBlocks, (qc.cap + 3) / 4);
+    EriArgs a = base_args(ctx);
+    a.D = dD;
+    a.J = ctx->work_Jh.as<double>();
+    a.K = ctx->work_Kh.as<double>();
+    a.tiles = ctx->plan_tiles64.as<Tile>();
+    a.tile_range = ctx->plan_ranges.as<int>() + 4 * c;
+    EriLaunchFn fn = eri_launcher(qc.la, qc.lb, qc.lc, qc.ld, kModeJK, o->boys_mode, 0);
+    if (!fn) return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class not compiled in");
+    TRY(launch_timed(ctx, qc, 0, fn, a, blocks));
+    if (mixed) {
+      EriLaunchFn f32 = eri_launcher(qc.la, qc.lb, qc.lc, qc.ld, kModeJK, kBoysExact, 1);
+      if (!f32) return ctx->fail(RCHEM_ERR_UNSUPPORTED, "FP32 ERI class not compiled in");
+      a.tiles = ctx->plan_tiles32.as<Tile>();
+      a.tile_range = ctx->plan_ranges.as<int>() + 4 * c + 2;
+      TRY(launch_timed(ctx, qc, 1, f32, a, blocks));
+    }
+  }
   HIPC(symmetrize_launch(ctx->work_Jh.as<double>(), ctx->work_Kh.as<double>(), dJ, dK, ctx->nbf,
                          ctx->stream));
   ctx->last_stats = ctx->planned;
+  ctx->last_mixed = mixed;
   return RCHEM_OK;
 }
 
@@ -671,9 +888,13 @@ int rchem_ctx_create(int device, rchem_ctx** out) {
     return RCHEM_ERR_HIP;
   }
   std::vector<double> tab = make_boys_table();
+  std::vector<float> tabf(tab.begin(), tab.end());
   if (ctx->boys_d.ensure(tab.size() * sizeof(double)) != hipSuccess ||
       hipMemcpy(ctx->boys_d.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) !=
-          hipSuccess) {
+          hipSuccess ||
+      ctx->boys_f_d.ensure(tabf.size() * sizeof(float)) != hipSuccess ||
+      hipMemcpy(ctx->boys_f_d.p, tabf.data(), tabf.size() * sizeof(float),
+                hipMemcpyHostToDevice) != hipSuccess) {
     g_create_error = "Boys table upload failed";
     rocblas_destroy_handle(ctx->blas);
     (void)hipStreamDestroy(ctx->stream);
@@ -873,9 +1094,20 @@ int rchem_jk_class_timings(rchem_ctx* ctx, rchem_class_timing* out, int max, int
   return RCHEM_OK;
 }
 
-int rchem_jk_last_stats(const rchem_ctx* ctx, rchem_jk_stats* st) {
+int rchem_jk_last_stats(rchem_ctx* ctx, rchem_jk_stats* st) {
   if (!ctx || !st) return RCHEM_ERR_INVALID;
   *st = ctx->last_stats;
+  if (ctx->last_mixed && !ctx->pclasses.empty()) {
+    // the planner counted FP64 and FP32 quartets of the last build on the device
+    TRY(set_device(ctx));
+    std::vector<double> dv(4 * ctx->pclasses.size());
+    HIPC(hipMemcpyAsync(dv.data(), ctx->plan_stats.p, sizeof(double) * dv.size(),
+                        hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    double q32 = 0.0;
+    for (size_t c = 0; c < ctx->pclasses.size(); ++c) q32 += dv[4 * c + 1];
+    st->quartets_fp32 = q32;
+  }
   return RCHEM_OK;
 }
 
@@ -891,11 +1123,11 @@ int rchem_eri_tensor(rchem_ctx* ctx, double* I, const rchem_opts* opts) {
   const size_t n = static_cast<size_t>(ctx->nbf);
   const size_t n4 = n * n * n * n;
   if (n > 200) return ctx->fail(RCHEM_ERR_INVALID, "tensor too large (nbf > 200)");
-  TRY(prepare(ctx, 0.0));
+  TRY(prepare_host_tiles(ctx, 0.0));
   DevBuf t;
   HIPC(t.ensure(n4 * sizeof(double)));
   HIPC(fill_launch(t.as<double>(), n4, 0.0, ctx->stream));
-  TRY(run_eri(ctx, kModeTensor, o.boys_mode, nullptr, nullptr, nullptr, t.as<double>()));
+  TRY(run_eri_host(ctx, kModeTensor, o.boys_mode, t.as<double>()));
   HIPC(hipMemcpyAsync(I, t.p, n4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
   return RCHEM_OK;

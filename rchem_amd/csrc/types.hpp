@@ -45,13 +45,44 @@ struct EriArgs {
   const PairRec* pairs;
   const PrimRec* prims;
   const Tile* tiles;
-  int ntiles;
+  int ntiles;              // host-sized launch: tiles[0 .. ntiles)
+  const int* tile_range;   // if set (persistent launch): tiles[range[0] .. range[1]),
+                           // read from device memory, drained by a fixed grid
   int nbf;
   const double* boys_table;
+  const float* boys_table_f;
   const double* D;   // kModeJK: density, nbf x nbf row-major, symmetric
   double* J;         // kModeJK: lower-half accumulators (J + J^T is the result)
   double* K;
   double* out;       // kModeDiag: Schwarz bound per pair; kModeTensor: nbf^4 tensor
+};
+
+// One quartet class as the device planner sees it.
+struct PlanClass {
+  int bra_begin, bra_end;  // bra pairs (pair class X)
+  int bk_off, nbk;         // ket buckets (pair class Y) in the bucket array
+  int task_begin;          // first task; tasks = (bra - bra_begin) * nbk + bucket
+  int same;                // X == Y: keep only ket <= bra
+  int npb;                 // unused padding
+  int pad;
+};
+
+struct PlanArgs {
+  const PlanClass* classes;
+  int nclasses;
+  int ntasks;
+  const int4* buckets;     // {begin, end, np, 0}
+  const double* Q;         // Schwarz bound per pair, layout order
+  const PairRec* pairs;
+  double tau;              // Schwarz threshold
+  double mixed_thresh;     // FP32 below Q_ab Q_cd max|D| < mixed_thresh; <= 0: all FP64
+  const double* dmax;      // max |D| (device scalar)
+  int* cnt64;              // per task: tiles, then (after the scan) offsets; length ntasks + 1
+  int* cnt32;
+  int* lo;                 // per task ket range [lo, mid) in FP64, [mid, hi) in FP32
+  int* mid;
+  int* hi;
+  double* stats;           // per class: {quartets64, quartets32, prim64, prim32}
 };
 
 }  // namespace rchem

@@ -160,3 +160,31 @@ def test_jk_h2o64_properties(gpu_ctx):
     assert np.max(np.abs(K3 - (2 * K1 - 3 * K2))) < 1e-10 * np.abs(K1).max()
     assert np.sum(D1 * J1) > 0.0      # Coulomb self-energy of a PSD density
     assert np.sum(D1 * K1) > 0.0
+
+
+def test_mixed_precision_h2o64(gpu_ctx):
+    """FP32 quartets below Q_ab Q_cd max|D| < 1e-7: the two-electron energy
+    moves by less than 1e-6 Eh from the all-FP64 build."""
+    from rchem_amd._lib import PREC_MIXED
+    Z, xyz = water_cluster(dims=(4, 4, 4), seed=64)
+    B = rb.Basis.new(Z, xyz, "6-31G", ctx=gpu_ctx)
+    n = B.nbf
+    _say(f"case: (H2O)_64 6-31G mixed vs FP64, nbf={n}")
+    S, T, V = rb.one_electron(B, xyz, Z)
+    lam, U = np.linalg.eigh(S)
+    X = U @ np.diag(lam ** -0.5) @ U.T
+    _, Cp = np.linalg.eigh(X.T @ (T + V) @ X)
+    C = X @ Cp
+    D = C[:, :320] @ C[:, :320].T
+    J = np.zeros((n, n)); K = np.zeros((n, n))
+    Jm = np.zeros((n, n)); Km = np.zeros((n, n))
+    rb.JK_direct(J, K, B, D)
+    rb.JK_direct(Jm, Km, B, D, default_opts(precision=PREC_MIXED, mixed_thresh=1e-7))
+    st = rb.jk_last_stats(B)
+    _say(f"  quartets={st.quartets:.4g} fp32={st.quartets_fp32:.4g}")
+    assert st.quartets_fp32 > 0.1 * st.quartets
+    e64 = np.sum(D * (2 * J - K))
+    e32 = np.sum(D * (2 * Jm - Km))
+    _say(f"  E2(fp64)={e64:.10f} E2(mixed)={e32:.10f} diff={e32 - e64:.3e}")
+    assert abs(e32 - e64) < 1e-6
+    assert np.max(np.abs(Jm - J)) < 1e-6 and np.max(np.abs(Km - K)) < 1e-6
