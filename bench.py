@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run one device copy of this many MiB (for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the Fock build is not sharded across GPUs in this "
@@ -113,6 +115,13 @@ def main():
                                     ctypes.c_void_p(dJ.data_ptr()),
                                     ctypes.c_void_p(dK.data_ptr()), ctypes.byref(opts)))
 
+    if args.copy_calibration_mb > 0:
+        nel = args.copy_calibration_mb * (1 << 20) // 8
+        src = torch.ones(nel, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()

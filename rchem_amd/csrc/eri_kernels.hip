@@ -9,12 +9,17 @@
 //   * per primitive quartet: Boys F_0..F_L, the generated VRR into contracted
 //     accumulators [e0|f0] held in registers;
 //   * per contracted quartet: the generated HRR, then
-//       kModeJK:     digestion into lower-half J and K with 8-fold symmetry
-//                    weights; the bra-row J block is reduced across the wave
-//                    before a single atomic, the rest are FP64 atomics in L2;
+//       kModeJK:     digestion into J and K with 8-fold symmetry weights;
 //       kModeDiag:   Schwarz bound sqrt(max |(ab|ab)|) of the lane's pair;
 //       kModeTensor: all 8 permutations into a dense nbf^4 tensor (small
 //                    systems; the layout of rchem's build_I, basis.rs:430-460).
+// JK digestion has two forms.  With LDS digestion (the default whenever the
+// bra's K rows fit), a workgroup owns a chunk of tiles of ONE bra pair (a, b):
+// every K contribution lands in rows of a or b, so the block accumulates them
+// in LDS (ds_add_f64) and flushes each row once, contiguously, at the end of
+// the chunk; Coulomb contributions go to pair-indexed accumulators whose
+// layout makes a wave's 64 updates contiguous.  Otherwise every contribution
+// is a global FP64 atomic (L2).
 // The reference computes the same integrals one primitive at a time with the
 // THO formula (cints.c:72-149) inside JK_direct's 8-deep loop
 // (basis.rs:383-428); its HGP recurrences are chgp.c:44-586.
@@ -38,23 +43,31 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// One tile (one bra pair x up to 64 ket pairs) in arithmetic type R.  With
-// R = float the primitive work (Boys, VRR, HRR, contraction) runs in FP32:
-// geometric differences are formed in FP64 and rounded once, and the
-// integrals are widened to FP64 before digestion, whose sums and atomics stay
-// in FP64.
+// Integrals of one tile: (ab|cd) for the lane's quartet in FP64 (out), with R
+// the arithmetic type of the primitive work.  With R = float the primitive
+// work (Boys, VRR, HRR, contraction) runs in FP32: geometric differences are
+// formed in FP64 and rounded once, and the integrals are widened to FP64
+// before digestion, whose sums and atomics stay in FP64.
+struct TileInfo {
+  PairRec bp, kp;
+  int bidx, kidx;
+  bool active;
+};
+
 template <int LA, int LB, int LC, int LD, int MODE, int BM, typename R>
-__device__ __forceinline__ void eri_tile(const EriArgs& a, const Tile t, const int lane) {
+__device__ __forceinline__ void eri_compute(const EriArgs& a, const Tile t, const int lane,
+                                            TileInfo& ti,
+                                            double (&out)[EriClass<LA, LB, LC, LD>::NOUT]) {
   using C = EriClass<LA, LB, LC, LD>;
-  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   constexpr int L = C::L;
   static_assert(BM == kBoysExact || sizeof(R) == 8, "compat Boys is FP64 only");
-  const bool active = lane < t.count;
-  const int kidx = t.ket0 + (active ? lane : 0);
-  const int bidx = (MODE == kModeDiag) ? kidx : t.bra;
-
-  const PairRec bp = a.pairs[bidx];
-  const PairRec kp = a.pairs[kidx];
+  ti.active = lane < t.count;
+  ti.kidx = t.ket0 + (ti.active ? lane : 0);
+  ti.bidx = (MODE == kModeDiag) ? ti.kidx : t.bra;
+  ti.bp = a.pairs[ti.bidx];
+  ti.kp = a.pairs[ti.kidx];
+  const PairRec& bp = ti.bp;
+  const PairRec& kp = ti.kp;
 
   R acc[C::NACC];
 #pragma unroll
@@ -91,11 +104,129 @@ __device__ __forceinline__ void eri_tile(const EriArgs& a, const Tile t, const i
   R outr[C::NOUT];
   C::template hrr<R>(acc, R(bp.AB[0]), R(bp.AB[1]), R(bp.AB[2]), R(kp.AB[0]), R(kp.AB[1]),
                      R(kp.AB[2]), outr);
-  double out[C::NOUT];
 #pragma unroll
   for (int k = 0; k < C::NOUT; ++k) out[k] = static_cast<double>(outr[k]);
+}
 
-  const int fa = bp.fa, fb = bp.fb, fc = kp.fa, fd = kp.fb;
+// 8-fold weight: each unique (ab|cd) stands for deg of the ordered quartets;
+// the lower-half accumulators get deg/8 of each of the eight permutations'
+// contributions, and J + J^T, K + K^T restore the rest.
+__device__ __forceinline__ double symmetry_weight(const TileInfo& ti) {
+  double s = 0.125;
+  if (ti.bp.flags & 1) s *= 2.0;
+  if (ti.kp.flags & 1) s *= 2.0;
+  if (ti.bidx != ti.kidx) s *= 2.0;
+  return ti.active ? s : 0.0;
+}
+
+// The four exchange blocks of one quartet, each contracted with its density
+// block:  K_ac += s (ab|cd) D_bd,  K_ad += s (ab|cd) D_bc,
+//         K_bc += s (ab|cd) D_ad,  K_bd += s (ab|cd) D_ac.
+// add(row_is_b, row_component, column_function, value) receives each sum.
+template <int NA, int NB, int NC, int ND, typename Add>
+__device__ __forceinline__ void exchange_blocks(const double* __restrict__ D, size_t n, int fa,
+                                                int fb, int fc, int fd, const double* out,
+                                                double s, Add add) {
+  double dbd[NB * ND], dbc[NB * NC], dad[NA * ND], dac[NA * NC];
+#pragma unroll
+  for (int ib = 0; ib < NB; ++ib) {
+#pragma unroll
+    for (int id = 0; id < ND; ++id) dbd[ib * ND + id] = D[(fb + ib) * n + fd + id];
+#pragma unroll
+    for (int ic = 0; ic < NC; ++ic) dbc[ib * NC + ic] = D[(fb + ib) * n + fc + ic];
+  }
+#pragma unroll
+  for (int ia = 0; ia < NA; ++ia) {
+#pragma unroll
+    for (int id = 0; id < ND; ++id) dad[ia * ND + id] = D[(fa + ia) * n + fd + id];
+#pragma unroll
+    for (int ic = 0; ic < NC; ++ic) dac[ia * NC + ic] = D[(fa + ia) * n + fc + ic];
+  }
+#pragma unroll
+  for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+    for (int ic = 0; ic < NC; ++ic) {
+      double v = 0.0;
+#pragma unroll
+      for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+        for (int id = 0; id < ND; ++id) v += out[((ia * NB + ib) * NC + ic) * ND + id] * dbd[ib * ND + id];
+      add(false, ia, fc + ic, s * v);
+    }
+#pragma unroll
+  for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+    for (int id = 0; id < ND; ++id) {
+      double v = 0.0;
+#pragma unroll
+      for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+        for (int ic = 0; ic < NC; ++ic) v += out[((ia * NB + ib) * NC + ic) * ND + id] * dbc[ib * NC + ic];
+      add(false, ia, fd + id, s * v);
+    }
+#pragma unroll
+  for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+    for (int ic = 0; ic < NC; ++ic) {
+      double v = 0.0;
+#pragma unroll
+      for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+        for (int id = 0; id < ND; ++id) v += out[((ia * NB + ib) * NC + ic) * ND + id] * dad[ia * ND + id];
+      add(true, ib, fc + ic, s * v);
+    }
+#pragma unroll
+  for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+    for (int id = 0; id < ND; ++id) {
+      double v = 0.0;
+#pragma unroll
+      for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+        for (int ic = 0; ic < NC; ++ic) v += out[((ia * NB + ib) * NC + ic) * ND + id] * dac[ia * NC + ic];
+      add(true, ib, fd + id, s * v);
+    }
+}
+
+// Coulomb sums of one quartet: jab[ab] = sum_cd (ab|cd) D_cd and
+// jcd[cd] = sum_ab (ab|cd) D_ab (unweighted).
+template <int NA, int NB, int NC, int ND>
+__device__ __forceinline__ void coulomb_sums(const double* __restrict__ D, size_t n, int fa,
+                                             int fb, int fc, int fd, const double* out,
+                                             double* jab, double* jcd) {
+  double dab[NA * NB], dcd[NC * ND];
+#pragma unroll
+  for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+    for (int ib = 0; ib < NB; ++ib) dab[ia * NB + ib] = D[(fa + ia) * n + fb + ib];
+#pragma unroll
+  for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+    for (int id = 0; id < ND; ++id) dcd[ic * ND + id] = D[(fc + ic) * n + fd + id];
+#pragma unroll
+  for (int k = 0; k < NC * ND; ++k) jcd[k] = 0.0;
+#pragma unroll
+  for (int ab = 0; ab < NA * NB; ++ab) {
+    double v = 0.0;
+#pragma unroll
+    for (int k = 0; k < NC * ND; ++k) {
+      const double x = out[ab * NC * ND + k];
+      v += x * dcd[k];
+      jcd[k] += x * dab[ab];
+    }
+    jab[ab] = v;
+  }
+}
+
+// Tile-granular kernel body: diagonal, tensor, and JK with global atomics.
+template <int LA, int LB, int LC, int LD, int MODE, int BM, typename R>
+__device__ __forceinline__ void eri_tile(const EriArgs& a, const Tile t, const int lane) {
+  using C = EriClass<LA, LB, LC, LD>;
+  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  TileInfo ti;
+  double out[C::NOUT];
+  eri_compute<LA, LB, LC, LD, MODE, BM, R>(a, t, lane, ti, out);
+  const int fa = ti.bp.fa, fb = ti.bp.fb, fc = ti.kp.fa, fd = ti.kp.fb;
   const size_t n = static_cast<size_t>(a.nbf);
 
   if constexpr (MODE == kModeDiag) {
@@ -104,9 +235,9 @@ __device__ __forceinline__ void eri_tile(const EriArgs& a, const Tile t, const i
     for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
       for (int ib = 0; ib < NB; ++ib) m = fmax(m, fabs(out[((ia * NB + ib) * NC + ia) * ND + ib]));
-    if (active) a.out[kidx] = sqrt(m);
+    if (ti.active) a.out[ti.kidx] = sqrt(m);
   } else if constexpr (MODE == kModeTensor) {
-    if (active) {
+    if (ti.active) {
 #pragma unroll
       for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
@@ -129,107 +260,27 @@ __device__ __forceinline__ void eri_tile(const EriArgs& a, const Tile t, const i
             }
     }
   } else {
-    // 8-fold weight: each unique (ab|cd) stands for deg of the ordered
-    // quartets; the lower-half accumulators get deg/8 of each of the eight
-    // permutations' contributions, and J + J^T, K + K^T restore the rest.
-    double s = 0.125;
-    if (bp.flags & 1) s *= 2.0;
-    if (kp.flags & 1) s *= 2.0;
-    if (bidx != kidx) s *= 2.0;
-    if (!active) s = 0.0;
-    const double* __restrict__ D = a.D;
-
-    double dab[NA * NB], dcd[NC * ND];
+    const double s = symmetry_weight(ti);
+    double jab[NA * NB], jcd[NC * ND];
+    coulomb_sums<NA, NB, NC, ND>(a.D, n, fa, fb, fc, fd, out, jab, jcd);
 #pragma unroll
-    for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-      for (int ib = 0; ib < NB; ++ib) dab[ia * NB + ib] = D[(fa + ia) * n + fb + ib];
-#pragma unroll
-    for (int ic = 0; ic < NC; ++ic)
-#pragma unroll
-      for (int id = 0; id < ND; ++id) dcd[ic * ND + id] = D[(fc + ic) * n + fd + id];
-
-    // Coulomb: J_ab += 2s sum_cd (ab|cd) D_cd ; J_cd += 2s sum_ab (ab|cd) D_ab
-    double jcd[NC * ND];
-#pragma unroll
-    for (int k = 0; k < NC * ND; ++k) jcd[k] = 0.0;
-#pragma unroll
-    for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-      for (int ib = 0; ib < NB; ++ib) {
-        double jab = 0.0;
-#pragma unroll
-        for (int k = 0; k < NC * ND; ++k) {
-          const double v = out[(ia * NB + ib) * NC * ND + k];
-          jab += v * dcd[k];
-          jcd[k] += v * dab[ia * NB + ib];
-        }
-        jab = wave_sum(2.0 * s * jab);
-        if (lane == 0) unsafeAtomicAdd(&a.J[(fa + ia) * n + fb + ib], jab);
-      }
-    if (active) {
-#pragma unroll
-      for (int ic = 0; ic < NC; ++ic)
-#pragma unroll
-        for (int id = 0; id < ND; ++id)
-          unsafeAtomicAdd(&a.J[(fc + ic) * n + fd + id], 2.0 * s * jcd[ic * ND + id]);
+    for (int ab = 0; ab < NA * NB; ++ab) {
+      const double v = wave_sum(2.0 * s * jab[ab]);
+      if (lane == 0) unsafeAtomicAdd(&a.J[(fa + ab / NB) * n + fb + ab % NB], v);
     }
-
-    // Exchange: K_ac += s (ab|cd) D_bd, K_ad += s (ab|cd) D_bc,
-    //           K_bc += s (ab|cd) D_ad, K_bd += s (ab|cd) D_ac.
-    if (active) {
+    if (ti.active) {
 #pragma unroll
-      for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-        for (int ic = 0; ic < NC; ++ic) {
-          double kac = 0.0;
-#pragma unroll
-          for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-            for (int id = 0; id < ND; ++id)
-              kac += out[((ia * NB + ib) * NC + ic) * ND + id] * D[(fb + ib) * n + fd + id];
-          unsafeAtomicAdd(&a.K[(fa + ia) * n + fc + ic], s * kac);
-        }
-#pragma unroll
-      for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-        for (int id = 0; id < ND; ++id) {
-          double kad = 0.0;
-#pragma unroll
-          for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-            for (int ic = 0; ic < NC; ++ic)
-              kad += out[((ia * NB + ib) * NC + ic) * ND + id] * D[(fb + ib) * n + fc + ic];
-          unsafeAtomicAdd(&a.K[(fa + ia) * n + fd + id], s * kad);
-        }
-#pragma unroll
-      for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-        for (int ic = 0; ic < NC; ++ic) {
-          double kbc = 0.0;
-#pragma unroll
-          for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-            for (int id = 0; id < ND; ++id)
-              kbc += out[((ia * NB + ib) * NC + ic) * ND + id] * D[(fa + ia) * n + fd + id];
-          unsafeAtomicAdd(&a.K[(fb + ib) * n + fc + ic], s * kbc);
-        }
-#pragma unroll
-      for (int ib = 0; ib < NB; ++ib)
-#pragma unroll
-        for (int id = 0; id < ND; ++id) {
-          double kbd = 0.0;
-#pragma unroll
-          for (int ia = 0; ia < NA; ++ia)
-#pragma unroll
-            for (int ic = 0; ic < NC; ++ic)
-              kbd += out[((ia * NB + ib) * NC + ic) * ND + id] * D[(fa + ia) * n + fc + ic];
-          unsafeAtomicAdd(&a.K[(fb + ib) * n + fd + id], s * kbd);
-        }
+      for (int k = 0; k < NC * ND; ++k)
+        unsafeAtomicAdd(&a.J[(fc + k / ND) * n + fd + k % ND], 2.0 * s * jcd[k]);
+      double* __restrict__ K = a.K;
+      exchange_blocks<NA, NB, NC, ND>(a.D, n, fa, fb, fc, fd, out, s,
+                                      [&](bool rb, int comp, int col, double v) {
+                                        const int row = rb ? fb + comp : fa + comp;
+                                        unsafeAtomicAdd(&K[row * n + col], v);
+                                      });
     }
   }
 }
-
 
 template <int LA, int LB, int LC, int LD, int MODE, int BM, typename R>
 __global__ __launch_bounds__(256) void eri_kernel(const EriArgs a) {
@@ -244,6 +295,88 @@ __global__ __launch_bounds__(256) void eri_kernel(const EriArgs a) {
   } else {
     if (wave >= a.ntiles) return;  // uniform per wave
     eri_tile<LA, LB, LC, LD, MODE, BM, R>(a, a.tiles[wave], lane);
+  }
+}
+
+// LDS digestion: a fixed grid of workgroups drains chunks [range[0], range[1]).
+// Dynamic LDS holds the bra's K rows: (NA + NB) x nbf doubles.
+template <int LA, int LB, int LC, int LD, int BM, typename R>
+__global__ __launch_bounds__(256) void eri_jk_lds_kernel(const EriArgs a) {
+  using C = EriClass<LA, LB, LC, LD>;
+  constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  extern __shared__ double krows[];
+  const int nbf = a.nbf;
+  const size_t n = static_cast<size_t>(nbf);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nrow = (NA + NB) * nbf;
+  const int begin = a.chunk_range[0], end = a.chunk_range[1];
+  for (int ch = begin + blockIdx.x; ch < end; ch += gridDim.x) {  // block-uniform
+    const Chunk c = a.chunks[ch];
+    for (int k = threadIdx.x; k < nrow; k += blockDim.x) krows[k] = 0.0;
+    __syncthreads();
+    double jab_acc[NA * NB];
+#pragma unroll
+    for (int k = 0; k < NA * NB; ++k) jab_acc[k] = 0.0;
+    for (int t = c.tb + wave; t < c.te; t += 4) {  // wave-uniform
+      TileInfo ti;
+      double out[C::NOUT];
+      eri_compute<LA, LB, LC, LD, kModeJK, BM, R>(a, a.tiles[t], lane, ti, out);
+      const int fa = ti.bp.fa, fb = ti.bp.fb, fc = ti.kp.fa, fd = ti.kp.fb;
+      const double s = symmetry_weight(ti);
+      double jab[NA * NB], jcd[NC * ND];
+      coulomb_sums<NA, NB, NC, ND>(a.D, n, fa, fb, fc, fd, out, jab, jcd);
+#pragma unroll
+      for (int k = 0; k < NA * NB; ++k) jab_acc[k] += 2.0 * s * jab[k];
+      if (ti.active) {
+        const int kq = ti.kidx - a.jp_ket.begin;
+#pragma unroll
+        for (int k = 0; k < NC * ND; ++k)
+          unsafeAtomicAdd(&a.Jp[a.jp_ket.base + k * a.jp_ket.n + kq], 2.0 * s * jcd[k]);
+        exchange_blocks<NA, NB, NC, ND>(a.D, n, fa, fb, fc, fd, out, s,
+                                        [&](bool rb, int comp, int col, double v) {
+                                          atomicAdd(&krows[(rb ? NA + comp : comp) * nbf + col], v);
+                                        });
+      }
+    }
+    // bra Coulomb block: wave sums, one atomic per component per wave
+    const int bq = c.bra - a.jp_bra.begin;
+#pragma unroll
+    for (int k = 0; k < NA * NB; ++k) {
+      const double v = wave_sum(jab_acc[k]);
+      if (lane == 0 && v != 0.0) unsafeAtomicAdd(&a.Jp[a.jp_bra.base + k * a.jp_bra.n + bq], v);
+    }
+    __syncthreads();
+    // flush the K rows of a and b, contiguous along each row
+    const PairRec bp = a.pairs[c.bra];
+    for (int k = threadIdx.x; k < nrow; k += blockDim.x) {
+      const double v = krows[k];
+      if (v != 0.0) {
+        const int r = k / nbf, col = k - r * nbf;
+        const int row = r < NA ? bp.fa + r : bp.fb + (r - NA);
+        unsafeAtomicAdd(&a.K[row * n + col], v);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Jh[fa + ia, fb + ib] = Jp of pair p (each shell pair appears once, so the
+// writes do not collide); one thread per (pair, component).
+__global__ void jp_scatter_kernel(const PairRec* __restrict__ pairs, const double* __restrict__ Jp,
+                                  const JpClass* __restrict__ cls, const int* __restrict__ ncomp,
+                                  int ncls, int npairs, double* __restrict__ Jh, int nbf) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  int X = 0;
+  while (X + 1 < ncls && cls[X + 1].begin <= p) ++X;
+  const JpClass jc = cls[X];
+  const int nb = ncomp[2 * X + 1];
+  const int nc = ncomp[2 * X] * nb;
+  const PairRec pr = pairs[p];
+  const size_t n = static_cast<size_t>(nbf);
+  for (int k = 0; k < nc; ++k) {
+    const double v = Jp[jc.base + k * jc.n + (p - jc.begin)];
+    Jh[(pr.fa + k / nb) * n + pr.fb + k % nb] += v;
   }
 }
 
@@ -272,6 +405,37 @@ hipError_t eri_launch(const EriArgs& a, int blocks, hipStream_t s) {
     grid = static_cast<unsigned>((a.ntiles + 3) / 4);
   }
   hipLaunchKernelGGL((eri_kernel<LA, LB, LC, LD, MODE, BM, R>), dim3(grid), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int LA, int LB, int LC, int LD, int BM, typename R>
+hipError_t eri_jk_lds_launch(const EriArgs& a, int blocks, hipStream_t s) {
+  if (blocks <= 0) return hipSuccess;
+  const size_t lds = sizeof(double) * (ncart(LA) + ncart(LB)) * static_cast<size_t>(a.nbf);
+  hipLaunchKernelGGL((eri_jk_lds_kernel<LA, LB, LC, LD, BM, R>), dim3(blocks), dim3(256), lds, s,
+                     a);
+  return hipGetLastError();
+}
+
+// LDS-digestion launcher lookup (JK mode); nullptr when not compiled in.
+EriLaunchFn eri_jk_lds_launcher(int la, int lb, int lc, int ld, int bm, int fp32) {
+#define RCHEM_ERI_LDS_CASE(A, B, C, D)                                          \
+  if (la == A && lb == B && lc == C && ld == D) {                               \
+    if (fp32) return bm == kBoysExact ? eri_jk_lds_launch<A, B, C, D, kBoysExact, float> : nullptr; \
+    return bm == kBoysExact ? eri_jk_lds_launch<A, B, C, D, kBoysExact, double>            \
+                            : eri_jk_lds_launch<A, B, C, D, kBoysCompat, double>;          \
+  }
+  RCHEM_ERI_CLASSES_SP(RCHEM_ERI_LDS_CASE)
+#undef RCHEM_ERI_LDS_CASE
+  return nullptr;
+}
+
+hipError_t jp_scatter_launch(const PairRec* pairs, const double* Jp, const JpClass* cls,
+                             const int* ncomp, int ncls, int npairs, double* Jh, int nbf,
+                             hipStream_t s) {
+  if (npairs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(jp_scatter_kernel, dim3((npairs + 255) / 256), dim3(256), 0, s, pairs, Jp,
+                     cls, ncomp, ncls, npairs, Jh, nbf);
   return hipGetLastError();
 }
 

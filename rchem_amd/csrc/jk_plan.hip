@@ -10,9 +10,12 @@
 //              search), split into an FP64 prefix (Q_ab Q_cd max|D| >=
 //              mixed_thresh) and an FP32 remainder; tile counts per task;
 //   3. scan:   exclusive prefix sums of the tile counts (hipCUB);
-//   4. expand: each task writes its tiles; per class, the [begin, end) tile
-//              range of each precision is written for the persistent ERI
-//              kernels, which read it from device memory.
+//   4. expand: each task writes its tiles;
+//   5. chunks: the tiles of one (class, bra pair) unit are cut into chunks of
+//              at most chunk_tiles tiles (a workgroup's unit of work when K is
+//              digested in LDS), counted, scanned and written; per class the
+//              [begin, end) tile and chunk ranges of each precision go to device
+//              memory, where the persistent ERI kernels read them.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -47,48 +50,71 @@ __device__ __forceinline__ int first_below(const double* __restrict__ Q, int lo,
   return lo;
 }
 
-__global__ void plan_kernel(const PlanArgs a) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.ntasks) {
-    if (t == a.ntasks) {  // the extra slot makes the exclusive scan's last entry the total
-      a.cnt64[t] = 0;
-      a.cnt32[t] = 0;
-    }
-    return;
-  }
-  // class of this task (few classes: linear search)
+__device__ __forceinline__ double wave_sum_d(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ int class_of_task(const PlanArgs& a, int t) {
   int c = 0;
   while (c + 1 < a.nclasses && a.classes[c + 1].task_begin <= t) ++c;
-  const PlanClass pc = a.classes[c];
-  const int local = t - pc.task_begin;
-  const int i = pc.bra_begin + local / pc.nbk;
-  const int4 bk = a.buckets[pc.bk_off + local % pc.nbk];
-  int lo = bk.x, hi = bk.y;
-  if (pc.same) hi = (lo > i) ? lo : min(hi, i + 1);
-  const double qi = a.Q[i];
-  if (a.tau > 0.0) hi = (qi > 0.0) ? first_below(a.Q, lo, hi, a.tau / qi) : lo;
-  int mid = hi;
-  if (a.mixed_thresh > 0.0 && hi > lo) {
-    const double dm = *a.dmax;
-    const double need = (qi > 0.0 && dm > 0.0) ? a.mixed_thresh / (qi * dm) : HUGE_VAL;
-    mid = first_below(a.Q, lo, hi, need);
+  return c;
+}
+
+__global__ void plan_kernel(const PlanArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = t < a.ntasks;
+  if (t == a.ntasks) {  // the extra slot makes the exclusive scan's last entry the total
+    a.cnt64[t] = 0;
+    a.cnt32[t] = 0;
   }
-  a.lo[t] = lo;
-  a.mid[t] = mid;
-  a.hi[t] = hi;
-  a.cnt64[t] = (mid - lo + 63) >> 6;
-  a.cnt32[t] = (hi - mid + 63) >> 6;
-  if (hi > lo) {
-    const double npb = a.pairs[i].np, npk = bk.z;
+  int c = 0;
+  double q64 = 0.0, q32 = 0.0, p64 = 0.0, p32 = 0.0;
+  if (valid) {
+    c = class_of_task(a, t);
+    const PlanClass pc = a.classes[c];
+    const int local = t - pc.task_begin;
+    const int i = pc.bra_begin + local / pc.nbk;
+    const int4 bk = a.buckets[pc.bk_off + local % pc.nbk];
+    int lo = bk.x, hi = bk.y;
+    if (pc.same) hi = (lo > i) ? lo : min(hi, i + 1);
+    const double qi = a.Q[i];
+    if (a.tau > 0.0) hi = (qi > 0.0) ? first_below(a.Q, lo, hi, a.tau / qi) : lo;
+    int mid = hi;
+    if (a.mixed_thresh > 0.0 && hi > lo) {
+      const double dm = *a.dmax;
+      const double need = (qi > 0.0 && dm > 0.0) ? a.mixed_thresh / (qi * dm) : HUGE_VAL;
+      mid = first_below(a.Q, lo, hi, need);
+    }
+    a.lo[t] = lo;
+    a.mid[t] = mid;
+    a.hi[t] = hi;
+    a.cnt64[t] = (mid - lo + 63) >> 6;
+    a.cnt32[t] = (hi - mid + 63) >> 6;
+    const double npp = static_cast<double>(a.pairs[i].np) * bk.z;
+    q64 = mid - lo;
+    q32 = hi - mid;
+    p64 = q64 * npp;
+    p32 = q32 * npp;
+  }
+  // statistics: one atomic per wave when the wave's tasks share a class
+  const int c0 = __shfl(c, 0, 64);
+  const int lastv = __shfl(valid ? 1 : 0, 63, 64);
+  const int c63 = __shfl(c, 63, 64);
+  if (lastv && c0 == c63) {
+    q64 = wave_sum_d(q64);
+    q32 = wave_sum_d(q32);
+    p64 = wave_sum_d(p64);
+    p32 = wave_sum_d(p32);
+    if ((threadIdx.x & 63) == 0) {
+      double* st = a.stats + 4 * c0;
+      if (q64 > 0) { unsafeAtomicAdd(st + 0, q64); unsafeAtomicAdd(st + 2, p64); }
+      if (q32 > 0) { unsafeAtomicAdd(st + 1, q32); unsafeAtomicAdd(st + 3, p32); }
+    }
+  } else if (valid) {
     double* st = a.stats + 4 * c;
-    if (mid > lo) {
-      unsafeAtomicAdd(st + 0, double(mid - lo));
-      unsafeAtomicAdd(st + 2, double(mid - lo) * npb * npk);
-    }
-    if (hi > mid) {
-      unsafeAtomicAdd(st + 1, double(hi - mid));
-      unsafeAtomicAdd(st + 3, double(hi - mid) * npb * npk);
-    }
+    if (q64 > 0) { unsafeAtomicAdd(st + 0, q64); unsafeAtomicAdd(st + 2, p64); }
+    if (q32 > 0) { unsafeAtomicAdd(st + 1, q32); unsafeAtomicAdd(st + 3, p32); }
   }
 }
 
@@ -98,9 +124,7 @@ __global__ void expand_kernel(const PlanArgs a, const int* __restrict__ off64,
                               Tile* __restrict__ tiles32, int cap) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.ntasks) return;
-  int c = 0;
-  while (c + 1 < a.nclasses && a.classes[c + 1].task_begin <= t) ++c;
-  const PlanClass pc = a.classes[c];
+  const PlanClass pc = a.classes[class_of_task(a, t)];
   const int i = pc.bra_begin + (t - pc.task_begin) / pc.nbk;
   const int lo = a.lo[t], mid = a.mid[t], hi = a.hi[t];
   int o = off64[t];
@@ -109,18 +133,80 @@ __global__ void expand_kernel(const PlanArgs a, const int* __restrict__ off64,
   for (int k = mid; k < hi && o < cap; k += 64, ++o) tiles32[o] = Tile{i, k, min(64, hi - k), 0};
 }
 
-// ranges[4c .. 4c+3] = {begin64, end64, begin32, end32} of class c; the end of
-// the last class is the scan total, clamped to the buffer capacity.
+__device__ __forceinline__ int class_of_unit(const PlanArgs& a, int u) {
+  int c = 0;
+  while (c + 1 < a.nclasses && a.classes[c + 1].unit_begin <= u) ++c;
+  return c;
+}
+
+// Unit u = (class c, bra pair): its tasks are [t0, t0 + nbk).
+__device__ __forceinline__ void unit_tasks(const PlanArgs& a, int u, int* c, int* bra, int* t0,
+                                           int* t1) {
+  *c = class_of_unit(a, u);
+  const PlanClass pc = a.classes[*c];
+  const int local = u - pc.unit_begin;
+  *bra = pc.bra_begin + local;
+  *t0 = pc.task_begin + local * pc.nbk;
+  *t1 = *t0 + pc.nbk;
+}
+
+__global__ void chunk_count_kernel(const PlanArgs a, const int* __restrict__ off64,
+                                   const int* __restrict__ off32, int* __restrict__ chc64,
+                                   int* __restrict__ chc32) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u > a.nunits) return;
+  if (u == a.nunits) {
+    chc64[u] = 0;
+    chc32[u] = 0;
+    return;
+  }
+  int c, bra, t0, t1;
+  unit_tasks(a, u, &c, &bra, &t0, &t1);
+  const int ch = a.chunk_tiles;
+  chc64[u] = (off64[t1] - off64[t0] + ch - 1) / ch;
+  chc32[u] = (off32[t1] - off32[t0] + ch - 1) / ch;
+}
+
+__global__ void chunk_expand_kernel(const PlanArgs a, const int* __restrict__ off64,
+                                    const int* __restrict__ off32, const int* __restrict__ choff64,
+                                    const int* __restrict__ choff32, Chunk* __restrict__ chunks64,
+                                    Chunk* __restrict__ chunks32, int cap) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= a.nunits) return;
+  int c, bra, t0, t1;
+  unit_tasks(a, u, &c, &bra, &t0, &t1);
+  const int ch = a.chunk_tiles;
+  {
+    const int tb = min(off64[t0], cap), te = min(off64[t1], cap);
+    int o = choff64[u];
+    for (int k = tb; k < te && o < cap; k += ch, ++o) chunks64[o] = Chunk{bra, k, min(k + ch, te), 0};
+  }
+  {
+    const int tb = min(off32[t0], cap), te = min(off32[t1], cap);
+    int o = choff32[u];
+    for (int k = tb; k < te && o < cap; k += ch, ++o) chunks32[o] = Chunk{bra, k, min(k + ch, te), 0};
+  }
+}
+
+// ranges[8c .. 8c+7] = tiles {b64, e64, b32, e32}, chunks {b64, e64, b32, e32}
+// of class c, clamped to the buffer capacity.
 __global__ void ranges_kernel(const PlanArgs a, const int* __restrict__ off64,
-                              const int* __restrict__ off32, int* __restrict__ ranges, int cap) {
+                              const int* __restrict__ off32, const int* __restrict__ choff64,
+                              const int* __restrict__ choff32, int* __restrict__ ranges, int cap) {
   const int c = threadIdx.x;
   if (c >= a.nclasses) return;
   const int tb = a.classes[c].task_begin;
   const int te = (c + 1 < a.nclasses) ? a.classes[c + 1].task_begin : a.ntasks;
-  ranges[4 * c + 0] = min(off64[tb], cap);
-  ranges[4 * c + 1] = min(off64[te], cap);
-  ranges[4 * c + 2] = min(off32[tb], cap);
-  ranges[4 * c + 3] = min(off32[te], cap);
+  const int ub = a.classes[c].unit_begin;
+  const int ue = (c + 1 < a.nclasses) ? a.classes[c + 1].unit_begin : a.nunits;
+  ranges[8 * c + 0] = min(off64[tb], cap);
+  ranges[8 * c + 1] = min(off64[te], cap);
+  ranges[8 * c + 2] = min(off32[tb], cap);
+  ranges[8 * c + 3] = min(off32[te], cap);
+  ranges[8 * c + 4] = min(choff64[ub], cap);
+  ranges[8 * c + 5] = min(choff64[ue], cap);
+  ranges[8 * c + 6] = min(choff32[ub], cap);
+  ranges[8 * c + 7] = min(choff32[ue], cap);
 }
 
 hipError_t dmax_launch(const double* D, size_t n2, unsigned long long* out, hipStream_t s) {
@@ -132,34 +218,51 @@ hipError_t dmax_launch(const double* D, size_t n2, unsigned long long* out, hipS
   return hipGetLastError();
 }
 
-// Temporary-storage bytes of the scans over n + 1 entries.
-size_t plan_scan_bytes(int ntasks) {
+// Temporary-storage bytes of a scan over n + 1 entries.
+size_t plan_scan_bytes(int n) {
   size_t bytes = 0;
   if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, static_cast<int*>(nullptr),
-                                       static_cast<int*>(nullptr), ntasks + 1) != hipSuccess)
+                                       static_cast<int*>(nullptr), n + 1) != hipSuccess)
     return 0;
   return bytes;
 }
 
-hipError_t plan_launch(const PlanArgs& a, int* off64, int* off32, Tile* tiles64, Tile* tiles32,
-                       int cap, int* ranges, void* scan_tmp, size_t scan_bytes, hipStream_t s) {
-  const unsigned blocks = static_cast<unsigned>((a.ntasks + 1 + 255) / 256);
-  hipLaunchKernelGGL(plan_kernel, dim3(blocks), dim3(256), 0, s, a);
+hipError_t plan_launch(const PlanArgs& a, const PlanBuffers& b, hipStream_t s) {
+  hipLaunchKernelGGL(plan_kernel, dim3(static_cast<unsigned>((a.ntasks + 1 + 255) / 256)),
+                     dim3(256), 0, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  size_t b = scan_bytes;
-  e = hipcub::DeviceScan::ExclusiveSum(scan_tmp, b, a.cnt64, off64, a.ntasks + 1, s);
+  size_t sb = b.scan_bytes;
+  e = hipcub::DeviceScan::ExclusiveSum(b.scan_tmp, sb, a.cnt64, b.off64, a.ntasks + 1, s);
   if (e != hipSuccess) return e;
-  b = scan_bytes;
-  e = hipcub::DeviceScan::ExclusiveSum(scan_tmp, b, a.cnt32, off32, a.ntasks + 1, s);
+  sb = b.scan_bytes;
+  e = hipcub::DeviceScan::ExclusiveSum(b.scan_tmp, sb, a.cnt32, b.off32, a.ntasks + 1, s);
   if (e != hipSuccess) return e;
   if (a.ntasks > 0) {
     hipLaunchKernelGGL(expand_kernel, dim3(static_cast<unsigned>((a.ntasks + 255) / 256)),
-                       dim3(256), 0, s, a, off64, off32, tiles64, tiles32, cap);
+                       dim3(256), 0, s, a, b.off64, b.off32, b.tiles64, b.tiles32, b.cap);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(ranges_kernel, dim3(1), dim3(64), 0, s, a, off64, off32, ranges, cap);
+  hipLaunchKernelGGL(chunk_count_kernel, dim3(static_cast<unsigned>((a.nunits + 1 + 255) / 256)),
+                     dim3(256), 0, s, a, b.off64, b.off32, b.chc64, b.chc32);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  sb = b.scan_bytes;
+  e = hipcub::DeviceScan::ExclusiveSum(b.scan_tmp, sb, b.chc64, b.choff64, a.nunits + 1, s);
+  if (e != hipSuccess) return e;
+  sb = b.scan_bytes;
+  e = hipcub::DeviceScan::ExclusiveSum(b.scan_tmp, sb, b.chc32, b.choff32, a.nunits + 1, s);
+  if (e != hipSuccess) return e;
+  if (a.nunits > 0) {
+    hipLaunchKernelGGL(chunk_expand_kernel, dim3(static_cast<unsigned>((a.nunits + 255) / 256)),
+                       dim3(256), 0, s, a, b.off64, b.off32, b.choff64, b.choff32, b.chunks64,
+                       b.chunks32, b.cap);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(ranges_kernel, dim3(1), dim3(64), 0, s, a, b.off64, b.off32, b.choff64,
+                     b.choff32, b.ranges, b.cap);
   return hipGetLastError();
 }
 

@@ -12,6 +12,7 @@
 #include <cctype>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -29,8 +30,11 @@ using EriLaunchFn = hipError_t (*)(const EriArgs&, int, hipStream_t);
 EriLaunchFn eri_launcher(int la, int lb, int lc, int ld, int mode, int bm, int fp32);
 hipError_t dmax_launch(const double* D, size_t n2, unsigned long long* out, hipStream_t s);
 size_t plan_scan_bytes(int ntasks);
-hipError_t plan_launch(const PlanArgs& a, int* off64, int* off32, Tile* tiles64, Tile* tiles32,
-                       int cap, int* ranges, void* scan_tmp, size_t scan_bytes, hipStream_t s);
+hipError_t plan_launch(const PlanArgs& a, const PlanBuffers& b, hipStream_t s);
+EriLaunchFn eri_jk_lds_launcher(int la, int lb, int lc, int ld, int bm, int fp32);
+hipError_t jp_scatter_launch(const PairRec* pairs, const double* Jp, const JpClass* cls,
+                             const int* ncomp, int ncls, int npairs, double* Jh, int nbf,
+                             hipStream_t s);
 hipError_t symmetrize_launch(const double* Jh, const double* Kh, double* J, double* K, int n,
                              hipStream_t s);
 hipError_t fill_launch(double* p, size_t n, double v, hipStream_t s);
@@ -174,9 +178,15 @@ struct rchem_ctx {
   std::vector<PlanClass> pclasses;
   int plan_tasks = 0, plan_cap = 0;
   size_t plan_scan_bytes = 0;
+  int plan_units = 0;
   DevBuf plan_classes_d, plan_buckets_d, plan_q_d, plan_cnt64, plan_cnt32, plan_off64,
       plan_off32, plan_lo, plan_mid, plan_hi, plan_tiles64, plan_tiles32, plan_ranges,
-      plan_stats, plan_dmax, plan_scan_tmp;
+      plan_stats, plan_dmax, plan_scan_tmp, plan_chc64, plan_chc32, plan_choff64, plan_choff32,
+      plan_chunks64, plan_chunks32;
+  // pair-indexed Coulomb accumulators
+  std::vector<JpClass> jp_cls;
+  size_t jp_size = 0;
+  DevBuf jp_d, jp_cls_d, jp_ncomp_d;
 
   DevBuf boys_d, boys_f_d;
   DevBuf work_Jh, work_Kh, work_D, work_J, work_K;
@@ -184,6 +194,7 @@ struct rchem_ctx {
   bool last_mixed = false;
 
   bool profiling = false;
+  bool lds_digestion = true;  // RCHEM_LDS_DIGESTION=0 selects global-atomic digestion
   std::vector<ClassTimer> timers;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> event_pool;
 
@@ -585,7 +596,7 @@ int prepare_plan(rchem_ctx* ctx, double tau) {
   ctx->pclasses.clear();
   ctx->qclasses.clear();
   std::vector<int4> bks;
-  int tasks = 0;
+  int tasks = 0, units = 0;
   for (int X = 0; X < kNumPairClasses; ++X)
     for (int Y = 0; Y <= X; ++Y) {
       const int nbra = ctx->class_begin[X + 1] - ctx->class_begin[X];
@@ -598,8 +609,10 @@ int prepare_plan(rchem_ctx* ctx, double tau) {
       pc.nbk = nbk;
       pc.task_begin = tasks;
       pc.same = X == Y;
+      pc.unit_begin = units;
       for (const Bucket& bk : ctx->buckets[Y]) bks.push_back(make_int4(bk.begin, bk.end, bk.np, 0));
       tasks += nbra * nbk;
+      units += nbra;
       ctx->pclasses.push_back(pc);
       QClass qc = make_qclass(X, Y);
       ctx->qclasses.push_back(qc);
@@ -643,7 +656,34 @@ int prepare_plan(rchem_ctx* ctx, double tau) {
   if (total_cap > 2.0e9) return ctx->fail(RCHEM_ERR_ALLOC, "quartet tile list too large");
   ctx->plan_cap = static_cast<int>(total_cap) + 1;
   ctx->plan_tasks = tasks;
+  ctx->plan_units = units;
   const size_t nt = static_cast<size_t>(tasks) + 1;
+  const size_t nu = static_cast<size_t>(units) + 1;
+  for (DevBuf* b : {&ctx->plan_chc64, &ctx->plan_chc32, &ctx->plan_choff64, &ctx->plan_choff32})
+    HIPC(b->ensure(sizeof(int) * nu));
+  HIPC(ctx->plan_chunks64.ensure(sizeof(Chunk) * static_cast<size_t>(ctx->plan_cap)));
+  HIPC(ctx->plan_chunks32.ensure(sizeof(Chunk) * static_cast<size_t>(ctx->plan_cap)));
+  // pair-indexed Coulomb accumulators, one block per pair class
+  ctx->jp_cls.assign(kNumPairClasses, JpClass{});
+  std::vector<int> ncomp(2 * kNumPairClasses);
+  size_t jp = 0;
+  for (int X = 0; X < kNumPairClasses; ++X) {
+    int l1, l2;
+    class_ls(X, &l1, &l2);
+    ncomp[2 * X] = ncart(l1);
+    ncomp[2 * X + 1] = ncart(l2);
+    const int nX = ctx->class_begin[X + 1] - ctx->class_begin[X];
+    ctx->jp_cls[X] = JpClass{static_cast<int>(jp), nX, ctx->class_begin[X], 0};
+    jp += static_cast<size_t>(ncart(l1) * ncart(l2)) * nX;
+  }
+  ctx->jp_size = jp;
+  HIPC(ctx->jp_d.ensure(sizeof(double) * std::max<size_t>(jp, 1)));
+  HIPC(ctx->jp_cls_d.ensure(sizeof(JpClass) * kNumPairClasses));
+  HIPC(ctx->jp_ncomp_d.ensure(sizeof(int) * ncomp.size()));
+  HIPC(hipMemcpyAsync(ctx->jp_cls_d.p, ctx->jp_cls.data(), sizeof(JpClass) * kNumPairClasses,
+                      hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(ctx->jp_ncomp_d.p, ncomp.data(), sizeof(int) * ncomp.size(),
+                      hipMemcpyHostToDevice, ctx->stream));
   HIPC(ctx->plan_classes_d.ensure(sizeof(PlanClass) * std::max<size_t>(ctx->pclasses.size(), 1)));
   HIPC(ctx->plan_buckets_d.ensure(sizeof(int4) * std::max<size_t>(bks.size(), 1)));
   HIPC(ctx->plan_q_d.ensure(sizeof(double) * std::max<size_t>(ctx->pairQ.size(), 1)));
@@ -652,10 +692,10 @@ int prepare_plan(rchem_ctx* ctx, double tau) {
     HIPC(b->ensure(sizeof(int) * nt));
   HIPC(ctx->plan_tiles64.ensure(sizeof(Tile) * static_cast<size_t>(ctx->plan_cap)));
   HIPC(ctx->plan_tiles32.ensure(sizeof(Tile) * static_cast<size_t>(ctx->plan_cap)));
-  HIPC(ctx->plan_ranges.ensure(sizeof(int) * 4 * std::max<size_t>(ctx->pclasses.size(), 1)));
+  HIPC(ctx->plan_ranges.ensure(sizeof(int) * 8 * std::max<size_t>(ctx->pclasses.size(), 1)));
   HIPC(ctx->plan_stats.ensure(sizeof(double) * 4 * std::max<size_t>(ctx->pclasses.size(), 1)));
   HIPC(ctx->plan_dmax.ensure(sizeof(double)));
-  ctx->plan_scan_bytes = plan_scan_bytes(tasks);
+  ctx->plan_scan_bytes = std::max(plan_scan_bytes(tasks), plan_scan_bytes(units));
   HIPC(ctx->plan_scan_tmp.ensure(std::max<size_t>(ctx->plan_scan_bytes, 8)));
   if (!ctx->pclasses.empty())
     HIPC(hipMemcpyAsync(ctx->plan_classes_d.p, ctx->pclasses.data(),
@@ -750,6 +790,7 @@ int run_eri_host(rchem_ctx* ctx, int mode, int bm, double* out) {
 }
 
 constexpr int kPersistentBlocks = 2048;  // 256 CUs x 8 workgroups of 4 waves
+constexpr int kChunkTiles = 32;          // tiles per LDS-digestion chunk
 
 int launch_timed(rchem_ctx* ctx, const QClass& qc, int fp32, EriLaunchFn fn, const EriArgs& a,
                  int blocks) {
@@ -790,6 +831,8 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
   pa.classes = ctx->plan_classes_d.as<PlanClass>();
   pa.nclasses = nc;
   pa.ntasks = ctx->plan_tasks;
+  pa.nunits = ctx->plan_units;
+  pa.chunk_tiles = kChunkTiles;
   pa.buckets = ctx->plan_buckets_d.as<int4>();
   pa.Q = ctx->plan_q_d.as<double>();
   pa.pairs = ctx->pairs_d.as<PairRec>();
@@ -802,32 +845,58 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
   pa.mid = ctx->plan_mid.as<int>();
   pa.hi = ctx->plan_hi.as<int>();
   pa.stats = ctx->plan_stats.as<double>();
-  if (nc > 0)
-    HIPC(plan_launch(pa, ctx->plan_off64.as<int>(), ctx->plan_off32.as<int>(),
-                     ctx->plan_tiles64.as<Tile>(), ctx->plan_tiles32.as<Tile>(), ctx->plan_cap,
-                     ctx->plan_ranges.as<int>(), ctx->plan_scan_tmp.p, ctx->plan_scan_bytes,
-                     ctx->stream));
+  PlanBuffers pb{};
+  pb.off64 = ctx->plan_off64.as<int>();
+  pb.off32 = ctx->plan_off32.as<int>();
+  pb.tiles64 = ctx->plan_tiles64.as<Tile>();
+  pb.tiles32 = ctx->plan_tiles32.as<Tile>();
+  pb.chc64 = ctx->plan_chc64.as<int>();
+  pb.chc32 = ctx->plan_chc32.as<int>();
+  pb.choff64 = ctx->plan_choff64.as<int>();
+  pb.choff32 = ctx->plan_choff32.as<int>();
+  pb.chunks64 = ctx->plan_chunks64.as<Chunk>();
+  pb.chunks32 = ctx->plan_chunks32.as<Chunk>();
+  pb.cap = ctx->plan_cap;
+  pb.ranges = ctx->plan_ranges.as<int>();
+  pb.scan_tmp = ctx->plan_scan_tmp.p;
+  pb.scan_bytes = ctx->plan_scan_bytes;
+  if (nc > 0) HIPC(plan_launch(pa, pb, ctx->stream));
+  HIPC(fill_launch(ctx->jp_d.as<double>(), std::max<size_t>(ctx->jp_size, 1), 0.0, ctx->stream));
   for (int c = 0; c < nc; ++c) {
     const QClass& qc = ctx->qclasses[c];
     if (qc.quartets <= 0) continue;
-    const int blocks = std::min(kPersistentBlocks, (qc.cap + 3) / 4);
     EriArgs a = base_args(ctx);
     a.D = dD;
     a.J = ctx->work_Jh.as<double>();
     a.K = ctx->work_Kh.as<double>();
-    a.tiles = ctx->plan_tiles64.as<Tile>();
-    a.tile_range = ctx->plan_ranges.as<int>() + 4 * c;
-    EriLaunchFn fn = eri_launcher(qc.la, qc.lb, qc.lc, qc.ld, kModeJK, o->boys_mode, 0);
-    if (!fn) return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class not compiled in");
-    TRY(launch_timed(ctx, qc, 0, fn, a, blocks));
-    if (mixed) {
-      EriLaunchFn f32 = eri_launcher(qc.la, qc.lb, qc.lc, qc.ld, kModeJK, kBoysExact, 1);
-      if (!f32) return ctx->fail(RCHEM_ERR_UNSUPPORTED, "FP32 ERI class not compiled in");
-      a.tiles = ctx->plan_tiles32.as<Tile>();
-      a.tile_range = ctx->plan_ranges.as<int>() + 4 * c + 2;
-      TRY(launch_timed(ctx, qc, 1, f32, a, blocks));
+    a.Jp = ctx->jp_d.as<double>();
+    a.jp_bra = ctx->jp_cls[qc.X];
+    a.jp_ket = ctx->jp_cls[qc.Y];
+    // K rows of the bra pair in LDS when they fit in 64 KB
+    const size_t lds = sizeof(double) * (ncart(qc.la) + ncart(qc.lb)) * static_cast<size_t>(ctx->nbf);
+    const bool use_lds = lds <= 65536 && ctx->lds_digestion;
+    for (int prec = 0; prec < (mixed ? 2 : 1); ++prec) {
+      const int bm = prec ? kBoysExact : o->boys_mode;
+      a.tiles = prec ? ctx->plan_tiles32.as<Tile>() : ctx->plan_tiles64.as<Tile>();
+      if (use_lds) {
+        EriLaunchFn fn = eri_jk_lds_launcher(qc.la, qc.lb, qc.lc, qc.ld, bm, prec);
+        if (!fn) return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class not compiled in");
+        a.chunks = prec ? ctx->plan_chunks32.as<Chunk>() : ctx->plan_chunks64.as<Chunk>();
+        a.chunk_range = ctx->plan_ranges.as<int>() + 8 * c + (prec ? 6 : 4);
+        a.tile_range = nullptr;
+        TRY(launch_timed(ctx, qc, prec, fn, a, std::min(kPersistentBlocks, qc.cap)));
+      } else {
+        EriLaunchFn fn = eri_launcher(qc.la, qc.lb, qc.lc, qc.ld, kModeJK, bm, prec);
+        if (!fn) return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class not compiled in");
+        a.tile_range = ctx->plan_ranges.as<int>() + 8 * c + (prec ? 2 : 0);
+        TRY(launch_timed(ctx, qc, prec, fn, a, std::min(kPersistentBlocks, (qc.cap + 3) / 4)));
+      }
     }
   }
+  HIPC(jp_scatter_launch(ctx->pairs_d.as<PairRec>(), ctx->jp_d.as<double>(),
+                         ctx->jp_cls_d.as<JpClass>(), ctx->jp_ncomp_d.as<int>(), kNumPairClasses,
+                         static_cast<int>(ctx->pairs.size()), ctx->work_Jh.as<double>(), ctx->nbf,
+                         ctx->stream));
   HIPC(symmetrize_launch(ctx->work_Jh.as<double>(), ctx->work_Kh.as<double>(), dJ, dK, ctx->nbf,
                          ctx->stream));
   ctx->last_stats = ctx->planned;
@@ -875,6 +944,7 @@ int rchem_ctx_create(int device, rchem_ctx** out) {
   }
   auto* ctx = new rchem_ctx();
   ctx->device = device;
+  if (const char* e = std::getenv("RCHEM_LDS_DIGESTION")) ctx->lds_digestion = std::atoi(e) != 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
     g_create_error = "stream creation failed";
     delete ctx;

@@ -39,7 +39,21 @@ struct Tile {
   int pad;
 };
 
+// A block's work in LDS-digestion mode: tiles [tb, te) of one bra pair.
+struct Chunk {
+  int bra;
+  int tb, te;
+  int pad;
+};
+
 enum EriMode { kModeJK = 0, kModeDiag = 1, kModeTensor = 2 };
+
+// Pair-indexed Coulomb accumulators: component k of pair p (pair class X) is
+// Jp[base_X + k * n_X + (p - begin_X)], so the 64 lanes of a wave, which hold
+// consecutive ket pairs, update consecutive addresses.
+struct JpClass {
+  int base, n, begin, pad;
+};
 
 struct EriArgs {
   const PairRec* pairs;
@@ -55,6 +69,11 @@ struct EriArgs {
   double* J;         // kModeJK: lower-half accumulators (J + J^T is the result)
   double* K;
   double* out;       // kModeDiag: Schwarz bound per pair; kModeTensor: nbf^4 tensor
+  // LDS digestion (chunked launches): K rows of the bra in LDS, J by pair
+  const Chunk* chunks;
+  const int* chunk_range;  // [begin, end) in chunks, device memory
+  double* Jp;
+  JpClass jp_bra, jp_ket;
 };
 
 // One quartet class as the device planner sees it.
@@ -63,7 +82,7 @@ struct PlanClass {
   int bk_off, nbk;         // ket buckets (pair class Y) in the bucket array
   int task_begin;          // first task; tasks = (bra - bra_begin) * nbk + bucket
   int same;                // X == Y: keep only ket <= bra
-  int npb;                 // unused padding
+  int unit_begin;          // first bra unit; units = bra - bra_begin
   int pad;
 };
 
@@ -83,6 +102,26 @@ struct PlanArgs {
   int* mid;
   int* hi;
   double* stats;           // per class: {quartets64, quartets32, prim64, prim32}
+  int nunits;              // bra units (class, bra pair)
+  int chunk_tiles;         // tiles per chunk at most
+};
+
+// Device buffers the planner fills (all sized by the host from upper bounds).
+struct PlanBuffers {
+  int* off64;        // tile offsets per task (exclusive scan), ntasks + 1
+  int* off32;
+  Tile* tiles64;     // cap entries each
+  Tile* tiles32;
+  int* chc64;        // chunk counts per unit, nunits + 1
+  int* chc32;
+  int* choff64;      // chunk offsets per unit (exclusive scan)
+  int* choff32;
+  Chunk* chunks64;   // cap entries each
+  Chunk* chunks32;
+  int cap;
+  int* ranges;       // 8 per class
+  void* scan_tmp;
+  size_t scan_bytes;
 };
 
 }  // namespace rchem
