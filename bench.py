@@ -31,6 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector
 HBM_PEAK_GBS = 8000.0
 
 
@@ -142,11 +143,14 @@ def main():
     for i in range(cnt.value):
         t = timings[i]
         avg = t.ms / max(t.launches, 1)
-        classes.append({"class": f"({t.la}{t.lb}|{t.lc}{t.ld})", "avg_ms": avg,
+        classes.append({"class": f"({t.la}{t.lb}|{t.lc}{t.ld})" + (" fp32" if t.fp32 else ""),
+                        "avg_ms": avg,
                         "quartets": t.quartets, "prim_quartets": t.primitive_quartets,
                         "flops": t.flops,
                         "tflops": t.flops / (avg * 1e-3) / 1e12 if avg > 0 else 0.0})
     dom = max(classes, key=lambda c: c["avg_ms"])
+    dom_fp32 = dom["class"].endswith("fp32")
+    dom_peak = FP32_PEAK_TFLOPS if dom_fp32 else FP64_PEAK_TFLOPS
     ms = elapsed / args.steps * 1e3
     value = st.quartets / (ms * 1e-3)
     total_tflops = st.flops / (ms * 1e-3) / 1e12
@@ -167,13 +171,15 @@ def main():
                                "3.1 A, Schwarz 1e-12", "nbf": n, "shells": len(B.shells()[1]),
                    "precision": args.precision},
         "quartets": st.quartets,
+        "quartets_fp32": st.quartets_fp32,
         "primitive_quartets": st.primitive_quartets,
         "integrals": st.integrals,
         "fock_tflops": total_tflops,
-        "roofline": {"bound": "mfma", "achieved": dom["tflops"], "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": dom["tflops"] / FP64_PEAK_TFLOPS,
+        "roofline": {"bound": "mfma", "achieved": dom["tflops"], "peak": dom_peak,
+                     "unit": "TFLOP/s", "frac": dom["tflops"] / dom_peak,
                      "traffic": None, "kernel": dom["class"],
-                     "note": "FP64 VALU kernel; MI355X FP64 vector peak = FP64 matrix peak"},
+                     "note": ("VALU kernel; peak is MI355X's FP32 vector peak" if dom_fp32 else
+                              "FP64 VALU kernel; MI355X FP64 vector peak = FP64 matrix peak")},
         "classes": classes,
     }
     if not args.no_cpu_baseline:

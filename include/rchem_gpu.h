@@ -85,7 +85,7 @@ int rchem_basis_from_atoms(rchem_ctx *ctx, int natom, const int *Z, const double
                            const char *basis_name);
 
 /* The same from explicit shells: shell s has centre xyz[3s..3s+2], angular
- * momentum l[s] (0..2), nprim[s] primitives whose exponents and raw contraction
+ * momentum l[s] (0..2), nprim[s] (1..8) primitives whose exponents and raw contraction
  * coefficients are exps/coefs[off_s .. off_s+nprim[s]) with off_s the running
  * sum of nprim.  Functions are numbered shell by shell, components in
  * get_ijk_list order. */
@@ -145,6 +145,7 @@ int rchem_jk_last_stats(rchem_ctx *ctx, rchem_jk_stats *st);
  * No rchem counterpart (rchem has no timing). */
 typedef struct rchem_class_timing {
   int la, lb, lc, ld;
+  int fp32;                  /* 1: the FP32 kernel of a mixed-precision build */
   int launches;              /* launches summed into ms */
   double ms;                 /* summed kernel time */
   double quartets;           /* per launch */

@@ -37,7 +37,8 @@ class JKStats(ctypes.Structure):
 
 class ClassTiming(ctypes.Structure):
     _fields_ = [("la", ctypes.c_int), ("lb", ctypes.c_int), ("lc", ctypes.c_int),
-                ("ld", ctypes.c_int), ("launches", ctypes.c_int), ("ms", ctypes.c_double),
+                ("ld", ctypes.c_int), ("fp32", ctypes.c_int), ("launches", ctypes.c_int),
+                ("ms", ctypes.c_double),
                 ("quartets", ctypes.c_double), ("primitive_quartets", ctypes.c_double),
                 ("flops", ctypes.c_double)]
 

@@ -7,7 +7,8 @@
 //     stable.  For T >= 36 the asymptote F_0 = sqrt(pi/T)/2 (error < e^{-T})
 //     and the upward recursion F_{m+1} = ((2m+1) F_m - e^{-T}) / (2T) are used.
 //     The table is filled on the host in long double from the all-positive
-//     series and lives in device memory (74 KB, read through L1/L2).
+//     series and lives in device memory (111 KB FP64, read through L1/L2),
+//     followed by exp(-k/16) for the same grid.
 //   * compat (parity with the reference): the Numerical-Recipes incomplete
 //     gamma of libpyquante2 (cints.c:302-373) evaluated per m with its own
 //     constants (EPS = 3e-7, ITMAX = 100, x clamped to >= 1e-8), so integrals
@@ -27,7 +28,25 @@ constexpr int kBoysRows = 36 * 16 + 2; // T in [0, 36] rounded to the grid, plus
 
 enum BoysMode { kBoysExact = 0, kBoysCompat = 1 };
 
-// Exact Boys F_0..F_M into F[0..M].
+// 1/sqrt(x) and 1/x from the hardware estimates refined by Newton steps
+// (two in FP64, one in FP32): ~1 ulp, a fraction of the cost of the IEEE
+// division and square root sequences.
+__device__ __forceinline__ double fast_rsqrt(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = y * __builtin_fma(-h * y, y, 1.5);
+  y = y * __builtin_fma(-h * y, y, 1.5);
+  return y;
+}
+__device__ __forceinline__ float fast_rsqrt(float x) {
+  float y = __builtin_amdgcn_rsqf(x);
+  y = y * __builtin_fmaf(-0.5f * x * y, y, 1.5f);
+  return y;
+}
+
+// Exact Boys F_0..F_M into F[0..M].  `table` holds the Boys grid followed by
+// exp(-k/16) for the same k; e^{-T} = e^{-T_k} e^{T_k - T} with an 8-term
+// Taylor series of the second factor (|T_k - T| <= 1/32).
 template <int M, typename R>
 __device__ __forceinline__ void boys_exact(R T, const R* __restrict__ table, R* __restrict__ F) {
   if (T < R(kBoysTMax)) {
@@ -45,16 +64,26 @@ __device__ __forceinline__ void boys_exact(R T, const R* __restrict__ table, R* 
     f = row[0] + d * f;
     F[M] = f;
     if constexpr (M > 0) {
-      const R e = exp(-T);
+      R p = R(1.0 / 5040.0);
+      p = R(1.0 / 720.0) + d * p;
+      p = R(1.0 / 120.0) + d * p;
+      p = R(1.0 / 24.0) + d * p;
+      p = R(1.0 / 6.0) + d * p;
+      p = R(0.5) + d * p;
+      p = R(1) + d * p;
+      p = R(1) + d * p;
+      const R e = table[kBoysRows * kBoysCols + k] * p;
       const R t2 = T + T;
 #pragma unroll
       for (int m = M - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + e) * R(1.0 / (2 * m + 1));
     }
   } else {
-    const R rt = R(1) / sqrt(T);
-    F[0] = R(0.88622692545275801365) * rt;  // sqrt(pi)/2 / sqrt(T)
+    const R rt = fast_rsqrt(T);
+    F[0] = R(0.88622692545275801365) * rt;  // sqrt(pi)/2 / sqrt(T); erf(sqrt T) = 1 - O(e^-T)
     if constexpr (M > 0) {
-      const R e = exp(-T);
+      // e^{-T} < 2.4e-16 here: it only corrects the upward recursion, so a
+      // single-precision exponential is accurate enough
+      const R e = static_cast<R>(__expf(-static_cast<float>(T)));
       const R h = R(0.5) * rt * rt;  // 1/(2T)
 #pragma unroll
       for (int m = 0; m < M; ++m) F[m + 1] = (R(2 * m + 1) * F[m] - e) * h;

@@ -18,7 +18,12 @@ Intermediates are named locals, so after inlining they live in registers.
 The generator also counts floating-point operations per class (add/mul = 1,
 FMA = 2) for the roofline model, written to ``flops_per_class.json``.
 
-Usage: gen_eri.py OUT_INC OUT_JSON [LMAX]
+It writes, into OUT_DIR: eri_classes.inc (the recurrences), one translation
+unit eri_c_<la><lb><lc><ld>.hip per class instantiating the kernel variants
+that class needs, eri_dispatch.inc (class -> launcher), eri_flops.inc and
+flops_per_class.json (operation counts).
+
+Usage: gen_eri.py OUT_DIR [LMAX [KEY ...]]   (KEYs: classes to dispatch, default all)
 """
 import json
 import math
@@ -235,9 +240,46 @@ def classes(lmax):
     return out
 
 
+def tu_source(la, lb, lc, ld):
+    """Kernel variants of one class.  s/p classes carry everything (both Boys
+    modes, FP32, LDS digestion); d classes the production and parity set."""
+    key = f"{la}{lb}{lc}{ld}"
+    sp = max(la, lb, lc, ld) <= 1
+    A = f"{la}, {lb}, {lc}, {ld}"
+    L = ["// Generated by gen_eri.py -- do not edit.", '#include "../eri_common.hpp"', "",
+         "namespace rchem {", "",
+         f"EriLaunchFn eri_launcher_c{key}(int mode, int bm, int fp32) {{",
+         "  if (fp32)",
+         f"    return (mode == kModeJK && bm == kBoysExact) ? eri_launch<{A}, kModeJK, kBoysExact, float>"
+         " : nullptr;",
+         "  if (bm == kBoysExact) {",
+         f"    if (mode == kModeJK) return eri_launch<{A}, kModeJK, kBoysExact, double>;",
+         f"    if (mode == kModeDiag) return eri_launch<{A}, kModeDiag, kBoysExact, double>;",
+         f"    if (mode == kModeTensor) return eri_launch<{A}, kModeTensor, kBoysExact, double>;",
+         "    return nullptr;",
+         "  }"]
+    if sp:
+        L += [f"  if (mode == kModeJK) return eri_launch<{A}, kModeJK, kBoysCompat, double>;",
+              f"  if (mode == kModeDiag) return eri_launch<{A}, kModeDiag, kBoysCompat, double>;"]
+    L += [f"  if (mode == kModeTensor) return eri_launch<{A}, kModeTensor, kBoysCompat, double>;",
+          "  return nullptr;", "}", "",
+          f"EriLaunchFn eri_jk_lds_launcher_c{key}(int bm, int fp32) {{"]
+    if sp:
+        L += [f"  if (fp32) return bm == kBoysExact ? eri_jk_lds_launch<{A}, kBoysExact, float> : nullptr;",
+              f"  return bm == kBoysExact ? eri_jk_lds_launch<{A}, kBoysExact, double>",
+              f"                          : eri_jk_lds_launch<{A}, kBoysCompat, double>;"]
+    else:
+        L += ["  (void)bm;", "  (void)fp32;", "  return nullptr;"]
+    L += ["}", "", "}  // namespace rchem", ""]
+    return "\n".join(L)
+
+
 def main():
-    out_inc, out_json = sys.argv[1], sys.argv[2]
-    lmax = int(sys.argv[3]) if len(sys.argv) > 3 else 2
+    import os
+    out_dir = sys.argv[1]
+    lmax = int(sys.argv[2]) if len(sys.argv) > 2 else 2
+    only = set(sys.argv[3:])
+    os.makedirs(out_dir, exist_ok=True)
     parts = ["// Generated by gen_eri.py -- do not edit.", ""]
     flops = {}
     for (la, lb, lc, ld) in classes(lmax):
@@ -268,19 +310,39 @@ def main():
         parts.append("  }")
         parts.append("};")
         parts.append("")
-    sp = [c for c in classes(lmax) if max(c) <= 1]
-    dd = [c for c in classes(lmax) if max(c) > 1]
-    parts.append("#define RCHEM_ERI_CLASSES_SP(X) " + " ".join(f"X({a}, {b}, {c}, {d})" for a, b, c, d in sp))
-    parts.append("#define RCHEM_ERI_CLASSES_D(X) " + " ".join(f"X({a}, {b}, {c}, {d})" for a, b, c, d in dd))
-    parts.append("")
-    with open(out_inc, "w") as fh:
+        tu = tu_source(la, lb, lc, ld)
+        path = os.path.join(out_dir, f"eri_c_{key}.hip")
+        old = open(path).read() if os.path.exists(path) else None
+        if old != tu:  # keep timestamps of unchanged units
+            with open(path, "w") as fh:
+                fh.write(tu)
+    with open(os.path.join(out_dir, "eri_classes.inc"), "w") as fh:
         fh.write("\n".join(parts))
-    with open(out_json, "w") as fh:
+    keys = [f"{a}{b}{c}{d}" for a, b, c, d in classes(lmax)]
+    if only:
+        keys = [k for k in keys if k in only]
+    disp = ["// Generated by gen_eri.py -- do not edit."]
+    disp += [f"EriLaunchFn eri_launcher_c{k}(int mode, int bm, int fp32);" for k in keys]
+    disp += [f"EriLaunchFn eri_jk_lds_launcher_c{k}(int bm, int fp32);" for k in keys]
+    disp += ["", "// nullptr when the class or the variant is not compiled in.",
+             "EriLaunchFn eri_launcher(int la, int lb, int lc, int ld, int mode, int bm, int fp32) {"]
+    for k in keys:
+        disp.append(f"  if (la == {k[0]} && lb == {k[1]} && lc == {k[2]} && ld == {k[3]}) "
+                    f"return eri_launcher_c{k}(mode, bm, fp32);")
+    disp += ["  return nullptr;", "}", "",
+             "EriLaunchFn eri_jk_lds_launcher(int la, int lb, int lc, int ld, int bm, int fp32) {"]
+    for k in keys:
+        disp.append(f"  if (la == {k[0]} && lb == {k[1]} && lc == {k[2]} && ld == {k[3]}) "
+                    f"return eri_jk_lds_launcher_c{k}(bm, fp32);")
+    disp += ["  return nullptr;", "}", ""]
+    with open(os.path.join(out_dir, "eri_dispatch.inc"), "w") as fh:
+        fh.write("\n".join(disp))
+    with open(os.path.join(out_dir, "flops_per_class.json"), "w") as fh:
         json.dump(flops, fh, indent=1, sort_keys=True)
     # host-side copy of the operation counts: {la, lb, lc, ld, vrr/prim, hrr/contracted}
     rows = [f"  {{{k[0]}, {k[1]}, {k[2]}, {k[3]}, {v['vrr_per_primitive_quartet']}, "
             f"{v['hrr_per_contracted_quartet']}}}," for k, v in sorted(flops.items())]
-    with open(out_inc.replace("eri_classes.inc", "eri_flops.inc"), "w") as fh:
+    with open(os.path.join(out_dir, "eri_flops.inc"), "w") as fh:
         fh.write("// Generated by gen_eri.py -- do not edit.\n"
                  "static const int kEriFlops[][6] = {\n" + "\n".join(rows) + "\n};\n"
                  f"constexpr int kEriFlopsRows = {len(rows)};\n")

@@ -125,6 +125,7 @@ struct QClass {
 
 struct ClassTimer {
   int la, lb, lc, ld, fp32;
+  int cls;  // index into the planner's classes
   double quartets, prim_quartets, flops;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
   double ms = 0.0;
@@ -192,6 +193,7 @@ struct rchem_ctx {
   DevBuf work_Jh, work_Kh, work_D, work_J, work_K;
   rchem_jk_stats last_stats{};
   bool last_mixed = false;
+  bool last_mixed_pending = false;  // the build being launched is mixed
 
   bool profiling = false;
   bool lds_digestion = true;  // RCHEM_LDS_DIGESTION=0 selects global-atomic digestion
@@ -230,9 +232,13 @@ static std::string g_create_error;
 namespace {
 
 // Boys table F_m(k/16), m = 0..kBoysCols-1, from the all-positive series
-// F_m(T) = e^{-T} sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1)) in long double.
+// F_m(T) = e^{-T} sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1)) in long double,
+// followed by exp(-k/16) for k = 0..kBoysRows-1.
 std::vector<double> make_boys_table() {
-  std::vector<double> tab(static_cast<size_t>(kBoysRows) * kBoysCols);
+  std::vector<double> tab(static_cast<size_t>(kBoysRows) * (kBoysCols + 1));
+  for (int r = 0; r < kBoysRows; ++r)
+    tab[static_cast<size_t>(kBoysRows) * kBoysCols + r] =
+        static_cast<double>(expl(-static_cast<long double>(r) / kBoysInvH));
   for (int r = 0; r < kBoysRows; ++r) {
     const long double T = static_cast<long double>(r) / kBoysInvH;
     for (int m = 0; m < kBoysCols; ++m) {
@@ -262,6 +268,8 @@ int finish_basis(rchem_ctx* ctx) {
       return ctx->fail(RCHEM_ERR_UNSUPPORTED,
                        "angular momentum " + std::to_string(s.l) + " not supported (max 2)");
     if (s.nprim <= 0) return ctx->fail(RCHEM_ERR_INVALID, "shell with no primitives");
+    if (s.nprim > kMaxPrimPerShell)
+      return ctx->fail(RCHEM_ERR_UNSUPPORTED, "more than 8 primitives in a shell");
     s.f0 = f;
     f += ncart(s.l);
     for (int p = 0; p < s.nprim; ++p) {
@@ -734,6 +742,7 @@ ClassTimer* timer_for(rchem_ctx* ctx, const QClass& qc, int fp32) {
   ClassTimer t;
   t.la = qc.la; t.lb = qc.lb; t.lc = qc.lc; t.ld = qc.ld;
   t.fp32 = fp32;
+  t.cls = static_cast<int>(&qc - ctx->qclasses.data());
   ctx->timers.push_back(t);
   return &ctx->timers.back();
 }
@@ -816,6 +825,7 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
   TRY(prepare_plan(ctx, o->schwarz_thresh));
   const size_t n2 = static_cast<size_t>(ctx->nbf) * ctx->nbf;
   const bool mixed = o->precision == RCHEM_PREC_MIXED;
+  ctx->last_mixed_pending = mixed;
   HIPC(ctx->work_Jh.ensure(n2 * sizeof(double)));
   HIPC(ctx->work_Kh.ensure(n2 * sizeof(double)));
   HIPC(fill_launch(ctx->work_Jh.as<double>(), n2, 0.0, ctx->stream));
@@ -1155,10 +1165,26 @@ int rchem_jk_class_timings(rchem_ctx* ctx, rchem_class_timing* out, int max, int
   if (!ctx || !count || (max > 0 && !out)) return RCHEM_ERR_INVALID;
   TRY(set_device(ctx));
   TRY(collect_timers(ctx));
+  if (ctx->last_mixed && !ctx->pclasses.empty()) {
+    // work per precision of the last build, counted by the planner on the device
+    std::vector<double> dv(4 * ctx->pclasses.size());
+    HIPC(hipMemcpyAsync(dv.data(), ctx->plan_stats.p, sizeof(double) * dv.size(),
+                        hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    for (auto& t : ctx->timers) {
+      if (t.cls < 0 || t.cls >= static_cast<int>(ctx->qclasses.size())) continue;
+      const QClass& qc = ctx->qclasses[t.cls];
+      int fh = 0;
+      const double fp = per_prim_flops(qc, &fh);
+      t.quartets = dv[4 * t.cls + (t.fp32 ? 1 : 0)];
+      t.prim_quartets = dv[4 * t.cls + (t.fp32 ? 3 : 2)];
+      t.flops = t.prim_quartets * fp + t.quartets * fh;
+    }
+  }
   *count = static_cast<int>(ctx->timers.size());
   for (int i = 0; i < max && i < *count; ++i) {
     const ClassTimer& t = ctx->timers[i];
-    out[i] = rchem_class_timing{t.la, t.lb, t.lc, t.ld, t.launches, t.ms,
+    out[i] = rchem_class_timing{t.la, t.lb, t.lc, t.ld, t.fp32, t.launches, t.ms,
                                 t.quartets, t.prim_quartets, t.flops};
   }
   return RCHEM_OK;

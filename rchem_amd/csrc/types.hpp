@@ -11,6 +11,11 @@
 
 namespace rchem {
 
+// Primitives per shell accepted by the library (so a pair has at most 64
+// primitive pairs, which the LDS-digestion kernel stages in LDS).
+constexpr int kMaxPrimPerShell = 8;
+constexpr int kMaxPrimPairs = kMaxPrimPerShell * kMaxPrimPerShell;
+
 // One significant shell pair (a, b); a is the shell with the higher l.
 struct PairRec {
   double A[3];   // centre of shell a
