@@ -2,13 +2,15 @@
 //
 // Two evaluations:
 //   * exact (production): F_{M}(T) from a Taylor expansion about the nearest
-//     grid point of a table (spacing 1/16, 8 terms, |error| < 3e-17 relative),
-//     then downward recursion F_m = (2T F_{m+1} + e^{-T}) / (2m+1), which is
-//     stable.  For T >= 36 the asymptote F_0 = sqrt(pi/T)/2 (error < e^{-T})
-//     and the upward recursion F_{m+1} = ((2m+1) F_m - e^{-T}) / (2T) are used.
-//     The table is filled on the host in long double from the all-positive
-//     series and lives in device memory (111 KB FP64, read through L1/L2),
-//     followed by exp(-k/16) for the same grid.
+//     grid point T0 of a table (spacing 1/2, 13 terms in FP64, |T - T0| <= 1/4,
+//     truncation < 3e-18 relative; 7 terms in FP32), then downward recursion
+//     F_m = (2T F_{m+1} + e^{-T}) / (2m+1), which is stable, with
+//     e^{-T} = e^{-T0} e^{T0 - T} from the same row and a Taylor series.  For
+//     T >= 36 the asymptote F_0 = sqrt(pi/T)/2 (error < e^{-T}) and the upward
+//     recursion F_{m+1} = ((2m+1) F_m - e^{-T}) / (2T) are used.  The table
+//     (73 rows x 23 columns, 13 KB in FP64) is filled on the host in long
+//     double from the all-positive series; kernels stage it in LDS so every
+//     lookup is an LDS read, not a cache miss.
 //   * compat (parity with the reference): the Numerical-Recipes incomplete
 //     gamma of libpyquante2 (cints.c:302-373) evaluated per m with its own
 //     constants (EPS = 3e-7, ITMAX = 100, x clamped to >= 1e-8), so integrals
@@ -19,12 +21,15 @@
 
 namespace rchem {
 
-constexpr int kBoysCols = 24;          // m = 0 .. 23 stored per grid point
-constexpr int kBoysTaylor = 8;         // Taylor terms
-constexpr int kBoysMaxM = kBoysCols - kBoysTaylor;  // highest m served: 16
+constexpr int kBoysMaxM = 9;                          // highest m served
+constexpr int kBoysTerms = 13;                        // Taylor terms, FP64
+constexpr int kBoysTermsF = 7;                        // Taylor terms, FP32
+constexpr int kBoysCols = kBoysMaxM + kBoysTerms;     // F_0 .. F_21 per grid point
+constexpr int kBoysStride = kBoysCols + 1;            // + exp(-T0)
 constexpr double kBoysTMax = 36.0;
-constexpr double kBoysInvH = 16.0;     // grid spacing 1/16
-constexpr int kBoysRows = 36 * 16 + 2; // T in [0, 36] rounded to the grid, plus margin
+constexpr double kBoysInvH = 2.0;                     // grid spacing 1/2
+constexpr int kBoysRows = 73;                         // T0 = 0, 0.5, ..., 36
+constexpr int kBoysTableSize = kBoysRows * kBoysStride;
 
 enum BoysMode { kBoysExact = 0, kBoysCompat = 1 };
 
@@ -44,35 +49,26 @@ __device__ __forceinline__ float fast_rsqrt(float x) {
   return y;
 }
 
-// Exact Boys F_0..F_M into F[0..M].  `table` holds the Boys grid followed by
-// exp(-k/16) for the same k; e^{-T} = e^{-T_k} e^{T_k - T} with an 8-term
-// Taylor series of the second factor (|T_k - T| <= 1/32).
+// Exact Boys F_0..F_M into F[0..M]; `table` is the grid described above
+// (normally the LDS copy).
 template <int M, typename R>
 __device__ __forceinline__ void boys_exact(R T, const R* __restrict__ table, R* __restrict__ F) {
+  static_assert(M <= kBoysMaxM, "Boys order beyond the table");
+  constexpr int K = sizeof(R) == 8 ? kBoysTerms : kBoysTermsF;
   if (T < R(kBoysTMax)) {
     const int k = static_cast<int>(T * R(kBoysInvH) + R(0.5));
-    const R d = R(k) * (R(1) / R(kBoysInvH)) - T;  // T0 - T
-    const R* __restrict__ row = table + k * kBoysCols + M;
+    const R d = R(k) * (R(1) / R(kBoysInvH)) - T;  // T0 - T, |d| <= 1/4
+    const R* __restrict__ row = table + k * kBoysStride;
     // F_M(T) = sum_j F_{M+j}(T0) (T0 - T)^j / j!
-    R f = row[7];
-    f = row[6] + d * (R(1.0 / 7.0) * f);
-    f = row[5] + d * (R(1.0 / 6.0) * f);
-    f = row[4] + d * (R(1.0 / 5.0) * f);
-    f = row[3] + d * (R(1.0 / 4.0) * f);
-    f = row[2] + d * (R(1.0 / 3.0) * f);
-    f = row[1] + d * (R(1.0 / 2.0) * f);
-    f = row[0] + d * f;
+    R f = row[M + K - 1];
+#pragma unroll
+    for (int j = K - 2; j >= 0; --j) f = row[M + j] + d * (R(1.0 / (j + 1)) * f);
     F[M] = f;
     if constexpr (M > 0) {
-      R p = R(1.0 / 5040.0);
-      p = R(1.0 / 720.0) + d * p;
-      p = R(1.0 / 120.0) + d * p;
-      p = R(1.0 / 24.0) + d * p;
-      p = R(1.0 / 6.0) + d * p;
-      p = R(0.5) + d * p;
-      p = R(1) + d * p;
-      p = R(1) + d * p;
-      const R e = table[kBoysRows * kBoysCols + k] * p;
+      R p = R(1);
+#pragma unroll
+      for (int j = K - 1; j >= 1; --j) p = R(1) + d * (R(1.0 / j) * p);
+      const R e = row[kBoysCols] * p;  // e^{-T0} e^{d}
       const R t2 = T + T;
 #pragma unroll
       for (int m = M - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + e) * R(1.0 / (2 * m + 1));
@@ -89,6 +85,34 @@ __device__ __forceinline__ void boys_exact(R T, const R* __restrict__ table, R* 
       for (int m = 0; m < M; ++m) F[m + 1] = (R(2 * m + 1) * F[m] - e) * h;
     }
   }
+}
+
+// Runtime-order version (one-electron integrals), FP64.
+__device__ __forceinline__ void boys_exact_rt(int M, double T, const double* __restrict__ table,
+                                              double* F) {
+  if (T < kBoysTMax) {
+    const int k = static_cast<int>(T * kBoysInvH + 0.5);
+    const double d = k * (1.0 / kBoysInvH) - T;
+    const double* row = table + k * kBoysStride;
+    double f = row[M + kBoysTerms - 1];
+    for (int j = kBoysTerms - 2; j >= 0; --j) f = row[M + j] + d * (f / (j + 1));
+    F[M] = f;
+    double p = 1.0;
+    for (int j = kBoysTerms - 1; j >= 1; --j) p = 1.0 + d * (p / j);
+    const double e = row[kBoysCols] * p;
+    for (int m = M - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + e) / (2 * m + 1);
+  } else {
+    F[0] = 0.88622692545275801365 / sqrt(T);
+    const double e = exp(-T);
+    for (int m = 0; m < M; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) / (2.0 * T);
+  }
+}
+
+// Copies the Boys grid into LDS (call by every thread of the workgroup,
+// followed by a barrier).
+template <typename R>
+__device__ __forceinline__ void stage_boys(R* __restrict__ dst, const R* __restrict__ src) {
+  for (int k = threadIdx.x; k < kBoysTableSize; k += blockDim.x) dst[k] = src[k];
 }
 
 // Compat: the reference's Fgamma(m, x) (cints.c:302-373), NR gser/gcf.

@@ -57,13 +57,15 @@ struct TileInfo {
 
 // Primitive loops: the ket primitive (per-lane, vector loads, fetched one
 // iteration ahead) is the outer loop, the bra primitive (wave-uniform) the inner
-// one, so each ket record is read once per quartet.  `sbra`, when non-null,
-// holds the bra's primitive pairs in LDS (the LDS-digestion kernel stages them
-// once per chunk); otherwise they come from global memory via scalar loads.
-template <int LA, int LB, int LC, int LD, int MODE, int BM, typename R>
+// one, so each ket record is read once per quartet.  With BRA_LDS the bra's
+// primitive pairs are read from `sbra` in LDS (the LDS-digestion kernel stages
+// them once per chunk), otherwise from global memory.  `boys_tab` is the Boys
+// grid in LDS.
+template <int LA, int LB, int LC, int LD, int MODE, int BM, typename R, bool BRA_LDS = false>
 __device__ __forceinline__ void eri_compute(const EriArgs& a, const Tile t, const int lane,
                                             TileInfo& ti,
                                             double (&out)[EriClass<LA, LB, LC, LD>::NOUT],
+                                            const R* __restrict__ boys_tab,
                                             const PrimRec* __restrict__ sbra = nullptr) {
   using C = EriClass<LA, LB, LC, LD>;
   constexpr int L = C::L;
@@ -80,9 +82,7 @@ __device__ __forceinline__ void eri_compute(const EriArgs& a, const Tile t, cons
 #pragma unroll
   for (int i = 0; i < C::NACC; ++i) acc[i] = R(0);
 
-  const R* __restrict__ table;
-  if constexpr (sizeof(R) == 8) table = reinterpret_cast<const R*>(a.boys_table);
-  else table = reinterpret_cast<const R*>(a.boys_table_f);
+  const R* __restrict__ table = boys_tab;
 
   const PrimRec* __restrict__ kprim = a.prims + kp.p0;
   PrimRec nxt = kprim[0];
@@ -92,7 +92,7 @@ __device__ __forceinline__ void eri_compute(const EriArgs& a, const Tile t, cons
     const R zk = R(pk.zeta), kk = R(pk.kab), oo2e = R(pk.oo2z);
     const R QCx = R(pk.px - kp.A[0]), QCy = R(pk.py - kp.A[1]), QCz = R(pk.pz - kp.A[2]);
     for (int i = 0; i < bp.np; ++i) {
-      const PrimRec pb = sbra ? sbra[i] : a.prims[bp.p0 + i * bp.ps];
+      const PrimRec pb = BRA_LDS ? sbra[i] : a.prims[bp.p0 + i * bp.ps];
       const R zb = R(pb.zeta);
       const R rs = fast_rsqrt(zb + zk);  // (zeta + eta)^(-1/2)
       const R ooze = rs * rs;
@@ -117,6 +117,12 @@ __device__ __forceinline__ void eri_compute(const EriArgs& a, const Tile t, cons
                      R(kp.AB[2]), outr);
 #pragma unroll
   for (int k = 0; k < C::NOUT; ++k) out[k] = static_cast<double>(outr[k]);
+}
+
+template <typename R>
+__device__ __forceinline__ const R* boys_src(const EriArgs& a) {
+  if constexpr (sizeof(R) == 8) return reinterpret_cast<const R*>(a.boys_table);
+  else return reinterpret_cast<const R*>(a.boys_table_f);
 }
 
 // 8-fold weight: each unique (ab|cd) stands for deg of the ordered quartets;
@@ -231,12 +237,13 @@ __device__ __forceinline__ void coulomb_sums(const double* __restrict__ D, size_
 
 // Tile-granular kernel body: diagonal, tensor, and JK with global atomics.
 template <int LA, int LB, int LC, int LD, int MODE, int BM, typename R>
-__device__ __forceinline__ void eri_tile(const EriArgs& a, const Tile t, const int lane) {
+__device__ __forceinline__ void eri_tile(const EriArgs& a, const Tile t, const int lane,
+                                         const R* __restrict__ boys_tab) {
   using C = EriClass<LA, LB, LC, LD>;
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   TileInfo ti;
   double out[C::NOUT];
-  eri_compute<LA, LB, LC, LD, MODE, BM, R>(a, t, lane, ti, out);
+  eri_compute<LA, LB, LC, LD, MODE, BM, R>(a, t, lane, ti, out, boys_tab);
   const int fa = ti.bp.fa, fb = ti.bp.fb, fc = ti.kp.fa, fd = ti.kp.fb;
   const size_t n = static_cast<size_t>(a.nbf);
 
@@ -297,6 +304,9 @@ template <int LA, int LB, int LC, int LD, int MODE, int BM, typename R>
 __global__ __launch_bounds__(256) void eri_kernel(const EriArgs a) {
   const int lane = threadIdx.x & 63;
   const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  __shared__ R boys_tab[kBoysTableSize];
+  stage_boys(boys_tab, boys_src<R>(a));
+  __syncthreads();
   // the wave index is wave-uniform; saying so lets the tile, the bra pair and
   // the bra primitives come through scalar loads
   const int w = __builtin_amdgcn_readfirstlane(wave);
@@ -305,10 +315,10 @@ __global__ __launch_bounds__(256) void eri_kernel(const EriArgs a) {
     // leaves the loop once its next index passes the end
     const int begin = a.tile_range[0], end = a.tile_range[1];
     for (int t = begin + w; t < end; t += gridDim.x * 4)
-      eri_tile<LA, LB, LC, LD, MODE, BM, R>(a, a.tiles[t], lane);
+      eri_tile<LA, LB, LC, LD, MODE, BM, R>(a, a.tiles[t], lane, boys_tab);
   } else {
     if (w >= a.ntiles) return;  // uniform per wave
-    eri_tile<LA, LB, LC, LD, MODE, BM, R>(a, a.tiles[w], lane);
+    eri_tile<LA, LB, LC, LD, MODE, BM, R>(a, a.tiles[w], lane, boys_tab);
   }
 }
 
@@ -320,6 +330,8 @@ __global__ __launch_bounds__(256) void eri_jk_lds_kernel(const EriArgs a) {
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   extern __shared__ double krows[];
   __shared__ PrimRec sbra[kMaxPrimPairs];
+  __shared__ R boys_tab[kBoysTableSize];
+  stage_boys(boys_tab, boys_src<R>(a));
   const int nbf = a.nbf;
   const size_t n = static_cast<size_t>(nbf);
   const int lane = threadIdx.x & 63;
@@ -340,7 +352,8 @@ __global__ __launch_bounds__(256) void eri_jk_lds_kernel(const EriArgs a) {
     for (int t = c.tb + wave; t < c.te; t += 4) {  // wave-uniform
       TileInfo ti;
       double out[C::NOUT];
-      eri_compute<LA, LB, LC, LD, kModeJK, BM, R>(a, a.tiles[t], lane, ti, out, sbra);
+      eri_compute<LA, LB, LC, LD, kModeJK, BM, R, true>(a, a.tiles[t], lane, ti, out, boys_tab,
+                                                        sbra);
       const int fa = ti.bp.fa, fb = ti.bp.fb, fc = ti.kp.fa, fd = ti.kp.fb;
       const double s = symmetry_weight(ti);
       double jab[NA * NB], jcd[NC * ND];

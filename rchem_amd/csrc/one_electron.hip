@@ -93,29 +93,6 @@ __device__ __forceinline__ void hermite_e(int imax, int jmax, double pa, double 
   }
 }
 
-__device__ void boys_exact_rt(int M, double T, const double* __restrict__ table, double* F) {
-  if (T < kBoysTMax) {
-    const int k = static_cast<int>(T * kBoysInvH + 0.5);
-    const double d = k * (1.0 / kBoysInvH) - T;
-    const double* row = table + k * kBoysCols + M;
-    double f = row[7];
-    f = row[6] + d * (f / 7.0);
-    f = row[5] + d * (f / 6.0);
-    f = row[4] + d * (f / 5.0);
-    f = row[3] + d * (f / 4.0);
-    f = row[2] + d * (f / 3.0);
-    f = row[1] + d * (f / 2.0);
-    f = row[0] + d * f;
-    F[M] = f;
-    const double e = exp(-T);
-    for (int m = M - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + e) / (2 * m + 1);
-  } else {
-    F[0] = 0.88622692545275801365 / sqrt(T);
-    const double e = exp(-T);
-    for (int m = 0; m < M; ++m) F[m + 1] = ((2 * m + 1) * F[m] - e) / (2.0 * T);
-  }
-}
-
 __global__ __launch_bounds__(128) void one_electron_kernel(const OneElArgs a) {
   const int pid = blockIdx.x * blockDim.x + threadIdx.x;
   if (pid >= a.npairs) return;

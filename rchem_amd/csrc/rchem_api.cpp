@@ -231,14 +231,11 @@ static std::string g_create_error;
 
 namespace {
 
-// Boys table F_m(k/16), m = 0..kBoysCols-1, from the all-positive series
-// F_m(T) = e^{-T} sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1)) in long double,
-// followed by exp(-k/16) for k = 0..kBoysRows-1.
+// Boys grid: row r holds F_m(r/2), m = 0..kBoysCols-1, from the all-positive
+// series F_m(T) = e^{-T} sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1)) in long
+// double, then exp(-r/2).
 std::vector<double> make_boys_table() {
-  std::vector<double> tab(static_cast<size_t>(kBoysRows) * (kBoysCols + 1));
-  for (int r = 0; r < kBoysRows; ++r)
-    tab[static_cast<size_t>(kBoysRows) * kBoysCols + r] =
-        static_cast<double>(expl(-static_cast<long double>(r) / kBoysInvH));
+  std::vector<double> tab(static_cast<size_t>(kBoysTableSize));
   for (int r = 0; r < kBoysRows; ++r) {
     const long double T = static_cast<long double>(r) / kBoysInvH;
     for (int m = 0; m < kBoysCols; ++m) {
@@ -248,8 +245,9 @@ std::vector<double> make_boys_table() {
         sum += term;
         if (term < sum * 1e-22L) break;
       }
-      tab[static_cast<size_t>(r) * kBoysCols + m] = static_cast<double>(expl(-T) * sum);
+      tab[static_cast<size_t>(r) * kBoysStride + m] = static_cast<double>(expl(-T) * sum);
     }
+    tab[static_cast<size_t>(r) * kBoysStride + kBoysCols] = static_cast<double>(expl(-T));
   }
   return tab;
 }
