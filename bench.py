@@ -172,6 +172,8 @@ def main():
                    "precision": args.precision},
         "quartets": st.quartets,
         "quartets_fp32": st.quartets_fp32,
+        "tiles": st.tiles,
+        "lane_fill": st.quartets / max(st.tiles * 64.0, 1.0),
         "primitive_quartets": st.primitive_quartets,
         "integrals": st.integrals,
         "fock_tflops": total_tflops,

@@ -136,6 +136,7 @@ typedef struct rchem_jk_stats {
   double integrals;          /* contracted Cartesian integrals computed */
   double flops;              /* algorithmic FP operations (FMA = 2) */
   double quartets_fp32;      /* of `quartets`, computed in FP32 */
+  double tiles;              /* wavefront tiles (64 lanes) the quartets were packed into */
 } rchem_jk_stats;
 int rchem_jk_last_stats(rchem_ctx *ctx, rchem_jk_stats *st);
 

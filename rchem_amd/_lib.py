@@ -32,7 +32,7 @@ class Opts(ctypes.Structure):
 class JKStats(ctypes.Structure):
     _fields_ = [("quartets", ctypes.c_double), ("primitive_quartets", ctypes.c_double),
                 ("integrals", ctypes.c_double), ("flops", ctypes.c_double),
-                ("quartets_fp32", ctypes.c_double)]
+                ("quartets_fp32", ctypes.c_double), ("tiles", ctypes.c_double)]
 
 
 class ClassTiming(ctypes.Structure):

@@ -7,9 +7,10 @@
 //     F_m = (2T F_{m+1} + e^{-T}) / (2m+1), which is stable, with
 //     e^{-T} = e^{-T0} e^{T0 - T} from the same row and a Taylor series.  For
 //     T >= 36 the asymptote F_0 = sqrt(pi/T)/2 (error < e^{-T}) and the upward
-//     recursion F_{m+1} = ((2m+1) F_m - e^{-T}) / (2T) are used.  The table
-//     (73 rows x 23 columns, 13 KB in FP64) is filled on the host in long
-//     double from the all-positive series; kernels stage it in LDS so every
+//     recursion F_{m+1} = ((2m+1) F_m - e^{-T}) / (2T) are used.  The Taylor
+//     coefficients F_{M+j}(T0)/j! are tabulated per order M (so the Horner
+//     steps are bare FMAs), filled on the host in long double from the
+//     all-positive series; a kernel stages its order's block in LDS, so every
 //     lookup is an LDS read, not a cache miss.
 //   * compat (parity with the reference): the Numerical-Recipes incomplete
 //     gamma of libpyquante2 (cints.c:302-373) evaluated per m with its own
@@ -24,12 +25,17 @@ namespace rchem {
 constexpr int kBoysMaxM = 9;                          // highest m served
 constexpr int kBoysTerms = 13;                        // Taylor terms, FP64
 constexpr int kBoysTermsF = 7;                        // Taylor terms, FP32
-constexpr int kBoysCols = kBoysMaxM + kBoysTerms;     // F_0 .. F_21 per grid point
-constexpr int kBoysStride = kBoysCols + 1;            // + exp(-T0)
 constexpr double kBoysTMax = 36.0;
 constexpr double kBoysInvH = 2.0;                     // grid spacing 1/2
 constexpr int kBoysRows = 73;                         // T0 = 0, 0.5, ..., 36
-constexpr int kBoysTableSize = kBoysRows * kBoysStride;
+// Device layout: for M = 0..kBoysMaxM a block of kBoysRows rows holding
+// F_{M+j}(T0) / j!, j = 0..kBoysTerms-1 (the Taylor coefficients of F_M about
+// T0 in powers of T0 - T), then one row of exp(-T0).  A kernel of order M
+// stages its own block and the exp row (8 KB in FP64).
+constexpr int kBoysBlock = kBoysRows * kBoysTerms;
+constexpr int kBoysExpOffset = (kBoysMaxM + 1) * kBoysBlock;
+constexpr int kBoysTableSize = kBoysExpOffset + kBoysRows;
+constexpr int kBoysStageSize = kBoysBlock + kBoysRows;
 
 enum BoysMode { kBoysExact = 0, kBoysCompat = 1 };
 
@@ -49,8 +55,8 @@ __device__ __forceinline__ float fast_rsqrt(float x) {
   return y;
 }
 
-// Exact Boys F_0..F_M into F[0..M]; `table` is the grid described above
-// (normally the LDS copy).
+// Exact Boys F_0..F_M into F[0..M]; `table` is the staged block of order M
+// (kBoysBlock coefficients, then the exp row), normally in LDS.
 template <int M, typename R>
 __device__ __forceinline__ void boys_exact(R T, const R* __restrict__ table, R* __restrict__ F) {
   static_assert(M <= kBoysMaxM, "Boys order beyond the table");
@@ -58,17 +64,20 @@ __device__ __forceinline__ void boys_exact(R T, const R* __restrict__ table, R* 
   if (T < R(kBoysTMax)) {
     const int k = static_cast<int>(T * R(kBoysInvH) + R(0.5));
     const R d = R(k) * (R(1) / R(kBoysInvH)) - T;  // T0 - T, |d| <= 1/4
-    const R* __restrict__ row = table + k * kBoysStride;
-    // F_M(T) = sum_j F_{M+j}(T0) (T0 - T)^j / j!
-    R f = row[M + K - 1];
+    const R* __restrict__ c = table + k * kBoysTerms;
+    R f = c[K - 1];
 #pragma unroll
-    for (int j = K - 2; j >= 0; --j) f = row[M + j] + d * (R(1.0 / (j + 1)) * f);
+    for (int j = K - 2; j >= 0; --j) f = c[j] + d * f;
     F[M] = f;
     if constexpr (M > 0) {
-      R p = R(1);
+      // e^{d} by its Taylor series with constant coefficients 1/j!
+      constexpr double inv_fact[14] = {1.0, 1.0, 1.0 / 2, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720,
+                                       1.0 / 5040, 1.0 / 40320, 1.0 / 362880, 1.0 / 3628800,
+                                       1.0 / 39916800, 1.0 / 479001600, 1.0 / 6227020800.0};
+      R p = R(inv_fact[K - 1]);
 #pragma unroll
-      for (int j = K - 1; j >= 1; --j) p = R(1) + d * (R(1.0 / j) * p);
-      const R e = row[kBoysCols] * p;  // e^{-T0} e^{d}
+      for (int j = K - 2; j >= 0; --j) p = R(inv_fact[j]) + d * p;
+      const R e = table[kBoysBlock + k] * p;  // e^{-T0} e^{d}
       const R t2 = T + T;
 #pragma unroll
       for (int m = M - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + e) * R(1.0 / (2 * m + 1));
@@ -87,19 +96,19 @@ __device__ __forceinline__ void boys_exact(R T, const R* __restrict__ table, R* 
   }
 }
 
-// Runtime-order version (one-electron integrals), FP64.
+// Runtime-order version over the full device table (one-electron integrals).
 __device__ __forceinline__ void boys_exact_rt(int M, double T, const double* __restrict__ table,
                                               double* F) {
   if (T < kBoysTMax) {
     const int k = static_cast<int>(T * kBoysInvH + 0.5);
     const double d = k * (1.0 / kBoysInvH) - T;
-    const double* row = table + k * kBoysStride;
-    double f = row[M + kBoysTerms - 1];
-    for (int j = kBoysTerms - 2; j >= 0; --j) f = row[M + j] + d * (f / (j + 1));
+    const double* c = table + M * kBoysBlock + k * kBoysTerms;
+    double f = c[kBoysTerms - 1];
+    for (int j = kBoysTerms - 2; j >= 0; --j) f = c[j] + d * f;
     F[M] = f;
     double p = 1.0;
     for (int j = kBoysTerms - 1; j >= 1; --j) p = 1.0 + d * (p / j);
-    const double e = row[kBoysCols] * p;
+    const double e = table[kBoysExpOffset + k] * p;
     for (int m = M - 1; m >= 0; --m) F[m] = (2.0 * T * F[m + 1] + e) / (2 * m + 1);
   } else {
     F[0] = 0.88622692545275801365 / sqrt(T);
@@ -108,11 +117,14 @@ __device__ __forceinline__ void boys_exact_rt(int M, double T, const double* __r
   }
 }
 
-// Copies the Boys grid into LDS (call by every thread of the workgroup,
-// followed by a barrier).
-template <typename R>
+// Copies the order-M block and the exp row into LDS (every thread of the
+// workgroup calls it; a barrier follows).
+template <int M, typename R>
 __device__ __forceinline__ void stage_boys(R* __restrict__ dst, const R* __restrict__ src) {
-  for (int k = threadIdx.x; k < kBoysTableSize; k += blockDim.x) dst[k] = src[k];
+  const int m = M <= kBoysMaxM ? M : kBoysMaxM;
+  for (int k = threadIdx.x; k < kBoysBlock; k += blockDim.x) dst[k] = src[m * kBoysBlock + k];
+  for (int k = threadIdx.x; k < kBoysRows; k += blockDim.x)
+    dst[kBoysBlock + k] = src[kBoysExpOffset + k];
 }
 
 // Compat: the reference's Fgamma(m, x) (cints.c:302-373), NR gser/gcf.

@@ -304,8 +304,8 @@ template <int LA, int LB, int LC, int LD, int MODE, int BM, typename R>
 __global__ __launch_bounds__(256) void eri_kernel(const EriArgs a) {
   const int lane = threadIdx.x & 63;
   const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-  __shared__ R boys_tab[kBoysTableSize];
-  stage_boys(boys_tab, boys_src<R>(a));
+  __shared__ R boys_tab[kBoysStageSize];
+  stage_boys<EriClass<LA, LB, LC, LD>::L>(boys_tab, boys_src<R>(a));
   __syncthreads();
   // the wave index is wave-uniform; saying so lets the tile, the bra pair and
   // the bra primitives come through scalar loads
@@ -324,14 +324,16 @@ __global__ __launch_bounds__(256) void eri_kernel(const EriArgs a) {
 
 // LDS digestion: a fixed grid of workgroups drains chunks [range[0], range[1]).
 // Dynamic LDS holds the bra's K rows: (NA + NB) x nbf doubles.
+constexpr int kLdsWaves = 4;  // waves per LDS-digestion workgroup (they share the K rows)
+
 template <int LA, int LB, int LC, int LD, int BM, typename R>
-__global__ __launch_bounds__(256) void eri_jk_lds_kernel(const EriArgs a) {
+__global__ __launch_bounds__(64 * kLdsWaves) void eri_jk_lds_kernel(const EriArgs a) {
   using C = EriClass<LA, LB, LC, LD>;
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   extern __shared__ double krows[];
   __shared__ PrimRec sbra[kMaxPrimPairs];
-  __shared__ R boys_tab[kBoysTableSize];
-  stage_boys(boys_tab, boys_src<R>(a));
+  __shared__ R boys_tab[kBoysStageSize];
+  stage_boys<C::L>(boys_tab, boys_src<R>(a));
   const int nbf = a.nbf;
   const size_t n = static_cast<size_t>(nbf);
   const int lane = threadIdx.x & 63;
@@ -349,7 +351,7 @@ __global__ __launch_bounds__(256) void eri_jk_lds_kernel(const EriArgs a) {
     double jab_acc[NA * NB];
 #pragma unroll
     for (int k = 0; k < NA * NB; ++k) jab_acc[k] = 0.0;
-    for (int t = c.tb + wave; t < c.te; t += 4) {  // wave-uniform
+    for (int t = c.tb + wave; t < c.te; t += kLdsWaves) {  // wave-uniform
       TileInfo ti;
       double out[C::NOUT];
       eri_compute<LA, LB, LC, LD, kModeJK, BM, R, true>(a, a.tiles[t], lane, ti, out, boys_tab,
@@ -413,7 +415,8 @@ template <int LA, int LB, int LC, int LD, int BM, typename R>
 hipError_t eri_jk_lds_launch(const EriArgs& a, int blocks, hipStream_t s) {
   if (blocks <= 0) return hipSuccess;
   const size_t lds = sizeof(double) * (ncart(LA) + ncart(LB)) * static_cast<size_t>(a.nbf);
-  hipLaunchKernelGGL((eri_jk_lds_kernel<LA, LB, LC, LD, BM, R>), dim3(blocks), dim3(256), lds, s,
+  hipLaunchKernelGGL((eri_jk_lds_kernel<LA, LB, LC, LD, BM, R>), dim3(blocks),
+                     dim3(64 * kLdsWaves), lds, s,
                      a);
   return hipGetLastError();
 }

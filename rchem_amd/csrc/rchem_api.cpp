@@ -231,23 +231,34 @@ static std::string g_create_error;
 
 namespace {
 
-// Boys grid: row r holds F_m(r/2), m = 0..kBoysCols-1, from the all-positive
-// series F_m(T) = e^{-T} sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1)) in long
-// double, then exp(-r/2).
+// Boys tables: for each order M, row r holds F_{M+j}(r/2) / j!,
+// j = 0..kBoysTerms-1, from the all-positive series
+// F_m(T) = e^{-T} sum_k (2T)^k / ((2m+1)(2m+3)...(2m+2k+1)) in long double;
+// then exp(-r/2) for every row.
 std::vector<double> make_boys_table() {
+  constexpr int kM = kBoysMaxM + kBoysTerms;  // orders needed
   std::vector<double> tab(static_cast<size_t>(kBoysTableSize));
+  std::vector<long double> F(kM);
   for (int r = 0; r < kBoysRows; ++r) {
     const long double T = static_cast<long double>(r) / kBoysInvH;
-    for (int m = 0; m < kBoysCols; ++m) {
+    for (int m = 0; m < kM; ++m) {
       long double term = 1.0L / (2 * m + 1), sum = term;
       for (int k = 1; k < 4000; ++k) {
         term *= 2.0L * T / (2 * m + 2 * k + 1);
         sum += term;
         if (term < sum * 1e-22L) break;
       }
-      tab[static_cast<size_t>(r) * kBoysStride + m] = static_cast<double>(expl(-T) * sum);
+      F[m] = expl(-T) * sum;
     }
-    tab[static_cast<size_t>(r) * kBoysStride + kBoysCols] = static_cast<double>(expl(-T));
+    for (int M = 0; M <= kBoysMaxM; ++M) {
+      long double fact = 1.0L;
+      for (int j = 0; j < kBoysTerms; ++j) {
+        if (j > 0) fact *= j;
+        tab[static_cast<size_t>(M) * kBoysBlock + r * kBoysTerms + j] =
+            static_cast<double>(F[M + j] / fact);
+      }
+    }
+    tab[kBoysExpOffset + r] = static_cast<double>(expl(-T));
   }
   return tab;
 }
@@ -1191,6 +1202,18 @@ int rchem_jk_class_timings(rchem_ctx* ctx, rchem_class_timing* out, int max, int
 int rchem_jk_last_stats(rchem_ctx* ctx, rchem_jk_stats* st) {
   if (!ctx || !st) return RCHEM_ERR_INVALID;
   *st = ctx->last_stats;
+  if (!ctx->pclasses.empty() && ctx->plan_tau >= 0.0) {
+    // tiles of the last build: the planner's per-class ranges
+    TRY(set_device(ctx));
+    std::vector<int> rg(8 * ctx->pclasses.size());
+    HIPC(hipMemcpyAsync(rg.data(), ctx->plan_ranges.p, sizeof(int) * rg.size(),
+                        hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    double tiles = 0.0;
+    for (size_t c = 0; c < ctx->pclasses.size(); ++c)
+      tiles += (rg[8 * c + 1] - rg[8 * c]) + (rg[8 * c + 3] - rg[8 * c + 2]);
+    st->tiles = tiles;
+  }
   if (ctx->last_mixed && !ctx->pclasses.empty()) {
     // the planner counted FP64 and FP32 quartets of the last build on the device
     TRY(set_device(ctx));
