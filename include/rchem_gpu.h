@@ -124,6 +124,14 @@ int rchem_jk_device(rchem_ctx *ctx, const double *dD, double *dJ, double *dK,
  * row-major into host memory (n^4 doubles).  No screening. */
 int rchem_eri_tensor(rchem_ctx *ctx, double *I, const rchem_opts *opts);
 
+/* Selected shell quartets: for quartet q, shells[4q..4q+3] = (A, B, C, D) of
+ * the current basis; out receives nA nB nC nD values per quartet, quartets
+ * back to back, each block row-major over (a, b, c, d) components in rchem's
+ * order.  No screening.  No rchem counterpart (rchem has no shell-level API):
+ * it samples integrals of systems too large for the full tensor. */
+int rchem_eri_shell_quartets(rchem_ctx *ctx, int nq, const int *shells, double *out,
+                             const rchem_opts *opts);
+
 /* Schwarz bounds sqrt(max_comp |(ab|ab)|) for every shell pair a >= b, as an
  * nshell x nshell lower-triangular-filled matrix (host).  No rchem
  * counterpart (rchem does not screen). */

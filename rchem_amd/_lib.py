@@ -74,6 +74,8 @@ SIGNATURES = {
                                        ctypes.c_void_p, ctypes.POINTER(Opts)]),
     "rchem_eri_tensor": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(Opts)]),
     "rchem_schwarz": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(Opts)]),
+    "rchem_eri_shell_quartets": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip, _dp,
+                                                ctypes.POINTER(Opts)]),
     "rchem_jk_last_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(JKStats)]),
     "rchem_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "rchem_reset_class_timings": (ctypes.c_int, [ctypes.c_void_p]),

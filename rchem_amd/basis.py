@@ -170,6 +170,24 @@ def build_I(basis: Basis, opts: Opts | None = None):
     return I
 
 
+def eri_shell_quartets(basis: Basis, quartets, opts: Opts | None = None):
+    """(AB|CD) blocks for rows (A, B, C, D) of shell indices; returns a list of
+    arrays shaped (nA, nB, nC, nD)."""
+    q = np.ascontiguousarray(quartets, dtype=np.int32).reshape(-1, 4)
+    ls = basis.shells()[1]
+    nc = (ls + 1) * (ls + 2) // 2
+    sizes = [int(np.prod(nc[r])) for r in q]
+    out = np.zeros(max(sum(sizes), 1))
+    o = opts or default_opts()
+    basis.ctx.check(lib().rchem_eri_shell_quartets(basis.ctx.h, len(q), _iptr(q), _dptr(out),
+                                                   ctypes.byref(o)))
+    blocks, off = [], 0
+    for r, sz in zip(q, sizes):
+        blocks.append(out[off:off + sz].reshape(tuple(nc[r])))
+        off += sz
+    return blocks
+
+
 def JK_inmem(I, D):
     """basis.rs:462 (host contraction of a tensor already in memory)."""
     return np.einsum("ijkl,kl->ij", I, D), np.einsum("ikjl,kl->ij", I, D)
@@ -204,4 +222,5 @@ def scf(basis: Basis, atomcoords, atomnos, nocc, thresh_e=1e-11, max_iter=1024,
 
 
 __all__ = ["Context", "Basis", "S", "T", "V", "JK_direct", "JK_inmem", "build_I", "scf",
+           "eri_shell_quartets",
            "one_electron", "schwarz", "jk_last_stats", "BOYS_EXACT", "BOYS_COMPAT"]

@@ -254,6 +254,11 @@ __device__ __forceinline__ void eri_tile(const EriArgs& a, const Tile t, const i
 #pragma unroll
       for (int ib = 0; ib < NB; ++ib) m = fmax(m, fabs(out[((ia * NB + ib) * NC + ia) * ND + ib]));
     if (ti.active) a.out[ti.kidx] = sqrt(m);
+  } else if constexpr (MODE == kModeQuartet) {
+    if (ti.active) {
+#pragma unroll
+      for (int k = 0; k < C::NOUT; ++k) a.out[static_cast<size_t>(t.pad) + k] = out[k];
+    }
   } else if constexpr (MODE == kModeTensor) {
     if (ti.active) {
 #pragma unroll

@@ -256,12 +256,14 @@ def tu_source(la, lb, lc, ld):
          f"    if (mode == kModeJK) return eri_launch<{A}, kModeJK, kBoysExact, double>;",
          f"    if (mode == kModeDiag) return eri_launch<{A}, kModeDiag, kBoysExact, double>;",
          f"    if (mode == kModeTensor) return eri_launch<{A}, kModeTensor, kBoysExact, double>;",
+         f"    if (mode == kModeQuartet) return eri_launch<{A}, kModeQuartet, kBoysExact, double>;",
          "    return nullptr;",
          "  }"]
     if sp:
         L += [f"  if (mode == kModeJK) return eri_launch<{A}, kModeJK, kBoysCompat, double>;",
               f"  if (mode == kModeDiag) return eri_launch<{A}, kModeDiag, kBoysCompat, double>;"]
     L += [f"  if (mode == kModeTensor) return eri_launch<{A}, kModeTensor, kBoysCompat, double>;",
+          f"  if (mode == kModeQuartet) return eri_launch<{A}, kModeQuartet, kBoysCompat, double>;",
           "  return nullptr;", "}", "",
           f"EriLaunchFn eri_jk_lds_launcher_c{key}(int bm, int fp32) {{"]
     if sp:

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -1247,6 +1248,146 @@ int rchem_eri_tensor(rchem_ctx* ctx, double* I, const rchem_opts* opts) {
   TRY(run_eri_host(ctx, kModeTensor, o.boys_mode, t.as<double>()));
   HIPC(hipMemcpyAsync(I, t.p, n4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
+  return RCHEM_OK;
+}
+
+int rchem_eri_shell_quartets(rchem_ctx* ctx, int nq, const int* shells, double* out,
+                             const rchem_opts* opts) {
+  if (!ctx) return RCHEM_ERR_INVALID;
+  if (!ctx->have_basis) return ctx->fail(RCHEM_ERR_NO_BASIS, "no basis uploaded");
+  if (nq < 0 || (nq > 0 && (!shells || !out)))
+    return ctx->fail(RCHEM_ERR_INVALID, "bad quartet list");
+  rchem_opts o;
+  rchem_opts_default(&o);
+  if (opts) o = *opts;
+  TRY(check_opts(ctx, &o));
+  TRY(set_device(ctx));
+  const int ns = static_cast<int>(ctx->shells.size());
+  for (int k = 0; k < 4 * nq; ++k)
+    if (shells[k] < 0 || shells[k] >= ns) return ctx->fail(RCHEM_ERR_INVALID, "shell index out of range");
+  if (nq == 0) return RCHEM_OK;
+  {
+    size_t total = 0;
+    for (int q = 0; q < nq; ++q) {
+      size_t blk = 1;
+      for (int k = 0; k < 4; ++k) blk *= ncart(ctx->shells[shells[4 * q + k]].l);
+      total += blk;
+    }
+    if (total > static_cast<size_t>(std::numeric_limits<int>::max()))
+      return ctx->fail(RCHEM_ERR_INVALID, "quartet list too long: output exceeds 2^31 values");
+  }
+  // Per quartet: two pair records in canonical orientation (higher l first in
+  // a pair, higher pair class as bra), their primitive pairs stored
+  // contiguously, and a one-lane tile tagged with the quartet index.
+  std::vector<PairRec> prs;
+  std::vector<PrimRec> pps, tmp;
+  struct Job { int perm[4]; int cls; size_t out_off; };
+  std::vector<Job> jobs(nq);
+  std::vector<std::vector<Tile>> by_class(kNumPairClasses * kNumPairClasses);
+  std::vector<std::pair<int, int>> cls_ls(kNumPairClasses * kNumPairClasses);
+  size_t off = 0;
+  auto add_pair = [&](int a, int b) {
+    PairRec r{};
+    const Shell& sa = ctx->shells[a];
+    const Shell& sb = ctx->shells[b];
+    for (int x = 0; x < 3; ++x) {
+      r.A[x] = sa.xyz[x];
+      r.AB[x] = sa.xyz[x] - sb.xyz[x];
+    }
+    r.fa = sa.f0;
+    r.fb = sb.f0;
+    make_prims(ctx, a, b, 0.0, &tmp);
+    r.np = static_cast<int>(tmp.size());
+    r.p0 = static_cast<int>(pps.size());
+    r.ps = 1;
+    r.flags = a != b;
+    pps.insert(pps.end(), tmp.begin(), tmp.end());
+    prs.push_back(r);
+    return static_cast<int>(prs.size()) - 1;
+  };
+  for (int q = 0; q < nq; ++q) {
+    int pos[4] = {0, 1, 2, 3};  // canonical slot k holds requested position pos[k]
+    const int* s4 = shells + 4 * q;
+    if (ctx->shells[s4[1]].l > ctx->shells[s4[0]].l) std::swap(pos[0], pos[1]);
+    if (ctx->shells[s4[3]].l > ctx->shells[s4[2]].l) std::swap(pos[2], pos[3]);
+    const int X = pair_class(ctx->shells[s4[pos[0]]].l, ctx->shells[s4[pos[1]]].l);
+    const int Y = pair_class(ctx->shells[s4[pos[2]]].l, ctx->shells[s4[pos[3]]].l);
+    if (X < Y) {
+      std::swap(pos[0], pos[2]);
+      std::swap(pos[1], pos[3]);
+    }
+    Job& j = jobs[q];
+    for (int k = 0; k < 4; ++k) j.perm[k] = pos[k];
+    const int bx = std::max(X, Y), ky = std::min(X, Y);
+    j.cls = bx * kNumPairClasses + ky;
+    j.out_off = off;
+    size_t blk = 1;
+    for (int k = 0; k < 4; ++k) blk *= ncart(ctx->shells[s4[k]].l);
+    off += blk;
+    const int bra = add_pair(s4[pos[0]], s4[pos[1]]);
+    const int ket = add_pair(s4[pos[2]], s4[pos[3]]);
+    // the tile's last field is the block's offset in the packed output
+    by_class[j.cls].push_back(Tile{bra, ket, 1, static_cast<int>(j.out_off)});
+    cls_ls[j.cls] = {bx, ky};
+  }
+  DevBuf dpr, dpp, dt, dout;
+  std::vector<Tile> tiles;
+  std::vector<std::pair<int, int>> ranges(by_class.size());
+  for (size_t c = 0; c < by_class.size(); ++c) {
+    ranges[c] = {static_cast<int>(tiles.size()), static_cast<int>(by_class[c].size())};
+    tiles.insert(tiles.end(), by_class[c].begin(), by_class[c].end());
+  }
+  HIPC(dpr.ensure(sizeof(PairRec) * prs.size()));
+  HIPC(dpp.ensure(sizeof(PrimRec) * pps.size()));
+  HIPC(dt.ensure(sizeof(Tile) * tiles.size()));
+  HIPC(dout.ensure(sizeof(double) * off));
+  HIPC(hipMemcpyAsync(dpr.p, prs.data(), sizeof(PairRec) * prs.size(), hipMemcpyHostToDevice,
+                      ctx->stream));
+  HIPC(hipMemcpyAsync(dpp.p, pps.data(), sizeof(PrimRec) * pps.size(), hipMemcpyHostToDevice,
+                      ctx->stream));
+  HIPC(hipMemcpyAsync(dt.p, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice,
+                      ctx->stream));
+  for (size_t c = 0; c < by_class.size(); ++c) {
+    if (ranges[c].second == 0) continue;
+    int la, lb, lc, ld;
+    class_ls(cls_ls[c].first, &la, &lb);
+    class_ls(cls_ls[c].second, &lc, &ld);
+    EriLaunchFn fn = eri_launcher(la, lb, lc, ld, kModeQuartet, o.boys_mode, 0);
+    if (!fn)
+      return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class (" + std::to_string(la) +
+                                                  std::to_string(lb) + "|" + std::to_string(lc) +
+                                                  std::to_string(ld) + ") not compiled into this library");
+    EriArgs a = base_args(ctx);
+    a.pairs = dpr.as<PairRec>();
+    a.prims = dpp.as<PrimRec>();
+    a.tiles = dt.as<Tile>() + ranges[c].first;
+    a.ntiles = ranges[c].second;
+    a.out = dout.as<double>();
+    HIPC(fn(a, 0, ctx->stream));
+  }
+  std::vector<double> raw(off);
+  HIPC(hipMemcpyAsync(raw.data(), dout.p, sizeof(double) * raw.size(), hipMemcpyDeviceToHost,
+                      ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  // canonical block -> requested order
+  for (int q = 0; q < nq; ++q) {
+    const int* s4 = shells + 4 * q;
+    const Job& j = jobs[q];
+    int nreq[4], ncan[4];
+    for (int k = 0; k < 4; ++k) nreq[k] = ncart(ctx->shells[s4[k]].l);
+    for (int k = 0; k < 4; ++k) ncan[k] = nreq[j.perm[k]];
+    int c[4];
+    for (c[0] = 0; c[0] < nreq[0]; ++c[0])
+      for (c[1] = 0; c[1] < nreq[1]; ++c[1])
+        for (c[2] = 0; c[2] < nreq[2]; ++c[2])
+          for (c[3] = 0; c[3] < nreq[3]; ++c[3]) {
+            const size_t src = ((static_cast<size_t>(c[j.perm[0]]) * ncan[1] + c[j.perm[1]]) *
+                                    ncan[2] + c[j.perm[2]]) * ncan[3] + c[j.perm[3]];
+            const size_t dst = ((static_cast<size_t>(c[0]) * nreq[1] + c[1]) * nreq[2] + c[2]) *
+                                   nreq[3] + c[3];
+            out[j.out_off + dst] = raw[j.out_off + src];
+          }
+  }
   return RCHEM_OK;
 }
 

@@ -51,7 +51,7 @@ struct Chunk {
   int pad;
 };
 
-enum EriMode { kModeJK = 0, kModeDiag = 1, kModeTensor = 2 };
+enum EriMode { kModeJK = 0, kModeDiag = 1, kModeTensor = 2, kModeQuartet = 3 };
 
 // Pair-indexed Coulomb accumulators: component k of pair p (pair class X) is
 // Jp[base_X + k * n_X + (p - begin_X)], so the 64 lanes of a wave, which hold
@@ -73,7 +73,8 @@ struct EriArgs {
   const double* D;   // kModeJK: density, nbf x nbf row-major, symmetric
   double* J;         // kModeJK: lower-half accumulators (J + J^T is the result)
   double* K;
-  double* out;       // kModeDiag: Schwarz bound per pair; kModeTensor: nbf^4 tensor
+  double* out;       // kModeDiag: Schwarz bound per pair; kModeTensor: nbf^4 tensor;
+                     // kModeQuartet: NOUT values per tile at out[tile.pad * NOUT]
   // LDS digestion (chunked launches): K rows of the bra in LDS, J by pair
   const Chunk* chunks;
   const int* chunk_range;  // [begin, end) in chunks, device memory

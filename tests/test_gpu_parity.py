@@ -52,34 +52,69 @@ def test_water_eri_tensor_exact_vs_oracle(gpu_ctx):
     assert np.max(np.abs(I - Io)) < 1e-12
 
 
+# (la lb|lc ld) classes compiled into the default library build
+COMPILED = {"0000", "1000", "1010", "1100", "1110", "1111", "2000", "2010", "2011", "2020",
+            "2100", "2110", "2200"}
+
+
+def _canonical_class(ls):
+    """Class key of a shell quartet as the library orders it (higher l first
+    within a pair, then the pair of higher pair-class index as bra)."""
+    a, b = sorted(ls[:2], reverse=True)
+    c, d = sorted(ls[2:], reverse=True)
+    pc = lambda x, y: x * (x + 1) // 2 + y
+    if pc(a, b) < pc(c, d):
+        a, b, c, d = c, d, a, b
+    return f"{a}{b}{c}{d}"
+
+
 def test_contracted_samples_all_classes(gpu_ctx):
-    """Every (la lb|lc ld) class the library compiles, against the reference's
-    contr_coulomb values (golden vectors)."""
+    """Every compiled (la lb|lc ld) class against the reference's contr_coulomb
+    values (golden vectors from the compiled reference C), shell quartet by
+    shell quartet, in the quartets' own (unsorted) shell order."""
     g = np.load(os.path.join(GOLD, "contracted_samples.npz"))
     ls = g["ls"]
-    keep = [s for s in range(len(ls)) if ls[s] <= 1]
-    # s/p shells only until the d classes are compiled in
-    sel = [qi for qi, q in enumerate(g["quartets"]) if all(ls[s] <= 1 for s in q)]
-    _say(f"case: contracted samples, {len(sel)} s/p quartets")
-    idx = {s: k for k, s in enumerate(keep)}
-    nprim = g["nprim"][keep]
-    ex = np.concatenate([g["exps"][3 * s:3 * s + 3] for s in keep])
-    cf = np.concatenate([g["coefs"][3 * s:3 * s + 3] for s in keep])
-    B = rb.Basis.from_shells(g["centres"][keep], ls[keep], nprim, ex, cf, ctx=gpu_ctx)
-    I = rb.build_I(B, default_opts(boys_mode=BOYS_COMPAT, schwarz_thresh=0.0))
-    f0 = np.concatenate([[0], np.cumsum([(ls[s] + 1) * (ls[s] + 2) // 2 for s in keep])])
+    sel = [qi for qi, q in enumerate(g["quartets"])
+           if _canonical_class([ls[s] for s in q]) in COMPILED]
+    classes = {_canonical_class([ls[s] for s in g["quartets"][qi]]) for qi in sel}
+    _say(f"case: contracted samples, {len(sel)} quartets over {len(classes)} classes")
+    assert len(classes) == len(COMPILED)
+    B = rb.Basis.from_shells(g["centres"], ls, g["nprim"], g["exps"], g["coefs"], ctx=gpu_ctx)
+    blocks = rb.eri_shell_quartets(B, g["quartets"][sel],
+                                   default_opts(boys_mode=BOYS_COMPAT, schwarz_thresh=0.0))
     qof = g["quartet_of"]
     worst = 0.0
-    for qi in sel:
-        q = [idx[s] for s in g["quartets"][qi]]
-        vals = g["values"][qof == qi]
-        k = 0
-        for a in range(f0[q[0]], f0[q[0] + 1]):
-            for b in range(f0[q[1]], f0[q[1] + 1]):
-                for c in range(f0[q[2]], f0[q[2] + 1]):
-                    for d in range(f0[q[3]], f0[q[3] + 1]):
-                        worst = max(worst, abs(I[a, b, c, d] - vals[k]))
-                        k += 1
+    for qi, blk in zip(sel, blocks):
+        worst = max(worst, np.max(np.abs(blk.reshape(-1) - g["values"][qof == qi])))
+    assert worst < 1e-12
+
+
+def test_sampled_quartets_h2o64_vs_oracle(gpu_ctx):
+    """Full (H2O)_64 6-31G size: 150 random shell quartets against the oracle's
+    THO integrals with the reference's Boys function."""
+    Z, xyz = water_cluster(dims=(4, 4, 4), seed=64)
+    B = rb.Basis.new(Z, xyz, "6-31G", ctx=gpu_ctx)
+    sh = B.shells()
+    ns = len(sh[1])
+    _say(f"case: (H2O)_64 sampled shell quartets, {ns} shells")
+    rng = np.random.default_rng(2026)
+    q = rng.integers(0, ns, (150, 4))
+    q[:20, 1] = q[:20, 0]          # some same-shell pairs
+    q[20:30, 2:] = q[20:30, :2]    # some (ab|ab) diagonals
+    blocks = rb.eri_shell_quartets(B, q, default_opts(boys_mode=BOYS_COMPAT, schwarz_thresh=0.0))
+    Bo = _oracle_basis(B)
+    f0 = np.concatenate([[0], np.cumsum((sh[1] + 1) * (sh[1] + 2) // 2)])
+    worst = 0.0
+    for r, blk in zip(q, blocks):
+        ref = np.zeros(blk.shape)
+        for a in range(blk.shape[0]):
+            for b in range(blk.shape[1]):
+                for c in range(blk.shape[2]):
+                    for d in range(blk.shape[3]):
+                        ref[a, b, c, d] = O.contracted_eri(Bo, f0[r[0]] + a, f0[r[1]] + b,
+                                                           f0[r[2]] + c, f0[r[3]] + d,
+                                                           boys=O.BOYS_COMPAT)
+        worst = max(worst, np.max(np.abs(blk - ref)))
     assert worst < 1e-12
 
 
