@@ -74,6 +74,17 @@ def cpu_baseline(Z, xyz, D, gpu_quartets, seconds):
     }
 
 
+def measured_traffic(kernel, precision):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, copy-calibrated)
+    from the committed rocprofv3 counter runs of this workload, or None."""
+    path = os.path.join(ROOT, "profiles", f"eri_jk_h2o64_631g_{precision}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        k = json.load(fh)["kernels"].get(kernel)
+    return None if k is None else k["fetch_bytes"] + k["write_bytes"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +160,7 @@ def main():
                         "flops": t.flops,
                         "tflops": t.flops / (avg * 1e-3) / 1e12 if avg > 0 else 0.0})
     dom = max(classes, key=lambda c: c["avg_ms"])
+    traffic = measured_traffic(dom["class"], args.precision)
     dom_fp32 = dom["class"].endswith("fp32")
     dom_peak = FP32_PEAK_TFLOPS if dom_fp32 else FP64_PEAK_TFLOPS
     ms = elapsed / args.steps * 1e3
@@ -179,7 +191,7 @@ def main():
         "fock_tflops": total_tflops,
         "roofline": {"bound": "mfma", "achieved": dom["tflops"], "peak": dom_peak,
                      "unit": "TFLOP/s", "frac": dom["tflops"] / dom_peak,
-                     "traffic": None, "kernel": dom["class"],
+                     "traffic": traffic, "kernel": dom["class"],
                      "note": ("VALU kernel; peak is MI355X's FP32 vector peak" if dom_fp32 else
                               "FP64 VALU kernel; MI355X FP64 vector peak = FP64 matrix peak")},
         "classes": classes,
