@@ -276,6 +276,38 @@ def tu_source(la, lb, lc, ld):
     return "\n".join(L)
 
 
+def write_programs(path, lmax):
+    """Operation tables of the wave-cooperative kernel for every class."""
+    from gen_eri_prog import hrr_program, vrr_program
+
+    def arr(ctype, name, vals, per_line=8):
+        body = []
+        for i in range(0, len(vals), per_line):
+            body.append("    " + ", ".join(vals[i:i + per_line]) + ",")
+        return [f"const {ctype} {name}[] = {{"] + body + ["};"]
+
+    out = ["// Generated by gen_eri.py -- do not edit.", ""]
+    entries = []
+    for (la, lb, lc, ld) in classes(lmax):
+        key = f"{la}{lb}{lc}{ld}"
+        v, h = vrr_program(la, lb, lc, ld), hrr_program(la, lb, lc, ld)
+        words = lambda rows: [f"{x}u" for r in rows for x in r]
+        out += arr("uint32_t", f"kProgV_{key}", words(v["ops"]) or ["0u"])
+        out += arr("uint32_t", f"kProgVG_{key}", words(v["groups"]) or ["0u"])
+        out += arr("uint32_t", f"kProgH_{key}", words(h["ops"]) or ["0u"])
+        out += arr("uint32_t", f"kProgHG_{key}", words(h["groups"]) or ["0u"])
+        out += arr("int", f"kProgOut_{key}", [str(sl) for sl, _ in h["out"]])
+        out += arr("double", f"kProgNorm_{key}", [repr(nm) for _, nm in h["out"]], 4)
+        entries.append(f"    {{{la}, {lb}, {lc}, {ld}, {v['L']}, {v['nacc']}, {v['nslots']}, "
+                       f"{h['nslots']}, {len(h['out'])}, {len(v['ops'])}, {len(v['groups'])}, "
+                       f"{len(h['ops'])}, {len(h['groups'])}, kProgV_{key}, kProgVG_{key}, "
+                       f"kProgH_{key}, kProgHG_{key}, kProgOut_{key}, kProgNorm_{key}}},")
+        out.append("")
+    out += ["const HostProgram kHostPrograms[] = {"] + entries + ["};", ""]
+    with open(path, "w") as fh:
+        fh.write("\n".join(out))
+
+
 def main():
     import os
     out_dir = sys.argv[1]
@@ -320,6 +352,7 @@ def main():
                 fh.write(tu)
     with open(os.path.join(out_dir, "eri_classes.inc"), "w") as fh:
         fh.write("\n".join(parts))
+    write_programs(os.path.join(out_dir, "eri_programs.inc"), lmax)
     keys = [f"{a}{b}{c}{d}" for a, b, c, d in classes(lmax)]
     if only:
         keys = [k for k in keys if k in only]

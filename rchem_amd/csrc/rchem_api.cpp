@@ -33,6 +33,22 @@ hipError_t dmax_launch(const double* D, size_t n2, unsigned long long* out, hipS
 size_t plan_scan_bytes(int ntasks);
 hipError_t plan_launch(const PlanArgs& a, const PlanBuffers& b, hipStream_t s);
 EriLaunchFn eri_jk_lds_launcher(int la, int lb, int lc, int ld, int bm, int fp32);
+hipError_t eri_generic_launch(const EriArgs& a, const GenericProgram& p, int mode, int bm,
+                              int blocks, hipStream_t s);
+size_t eri_generic_lds_bytes(const GenericProgram& p);
+
+// Operation tables of the wave-cooperative kernel, one per class (host copies).
+struct HostProgram {
+  int la, lb, lc, ld, L, nacc, nv, nh, nout, nvops, nvgroups, nhops, nhgroups;
+  const uint32_t* vops;
+  const uint32_t* vgroups;
+  const uint32_t* hops;
+  const uint32_t* hgroups;
+  const int* out_src;
+  const double* out_norm;
+};
+#include "generated/eri_programs.inc"
+constexpr int kNumHostPrograms = sizeof(kHostPrograms) / sizeof(kHostPrograms[0]);
 hipError_t jp_scatter_launch(const PairRec* pairs, const double* Jp, const JpClass* cls,
                              const int* ncomp, int ncls, int npairs, double* Jh, int nbf,
                              hipStream_t s);
@@ -198,6 +214,10 @@ struct rchem_ctx {
 
   bool profiling = false;
   bool lds_digestion = true;  // RCHEM_LDS_DIGESTION=0 selects global-atomic digestion
+  bool force_generic = false;  // RCHEM_ERI_GENERIC=1: every class on the wave-cooperative kernel
+  // wave-cooperative kernel: operation tables in device memory (uploaded on first use)
+  std::vector<GenericProgram> gprog;
+  DevBuf gprog_d;
   std::vector<ClassTimer> timers;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> event_pool;
 
@@ -402,6 +422,86 @@ int layout_pairs(rchem_ctx* ctx, const std::vector<PairHost>& sel, double prim_e
   return RCHEM_OK;
 }
 
+// The kernel that evaluates one class in one mode: the class's straight-line
+// per-lane kernel when the library carries it, else the wave-cooperative
+// kernel with the class's operation tables (always FP64; it also serves the
+// FP32 tile list of a mixed build).
+struct ClassKernel {
+  EriLaunchFn fn = nullptr;
+  const GenericProgram* gp = nullptr;
+  int mode = 0, bm = 0;
+  bool valid() const { return fn || gp; }
+  hipError_t operator()(const EriArgs& a, int blocks, hipStream_t s) const {
+    return fn ? fn(a, blocks, s) : eri_generic_launch(a, *gp, mode, bm, blocks, s);
+  }
+};
+
+int upload_programs(rchem_ctx* ctx) {
+  if (!ctx->gprog.empty()) return RCHEM_OK;
+  auto al = [](size_t b) { return (b + 15) & ~size_t(15); };
+  size_t total = 0;
+  for (int i = 0; i < kNumHostPrograms; ++i) {
+    const HostProgram& h = kHostPrograms[i];
+    total += al(16 * std::max(h.nvops, 1)) + al(16 * std::max(h.nvgroups, 1)) +
+             al(8 * std::max(h.nhops, 1)) + al(16 * std::max(h.nhgroups, 1)) + al(4 * h.nout) +
+             al(8 * h.nout);
+  }
+  std::vector<unsigned char> host(total);
+  HIPC(ctx->gprog_d.ensure(total));
+  std::vector<GenericProgram> progs(kNumHostPrograms);
+  size_t off = 0;
+  auto put = [&](const void* src, size_t bytes, size_t room) {
+    std::memcpy(host.data() + off, src, bytes);
+    const void* dev = static_cast<const unsigned char*>(ctx->gprog_d.p) + off;
+    off += al(room);
+    return dev;
+  };
+  for (int i = 0; i < kNumHostPrograms; ++i) {
+    const HostProgram& h = kHostPrograms[i];
+    GenericProgram& g = progs[i];
+    g.la = h.la; g.lb = h.lb; g.lc = h.lc; g.ld = h.ld;
+    g.L = h.L; g.nacc = h.nacc; g.nv = h.nv; g.nh = h.nh; g.nout = h.nout;
+    g.nvgroups = h.nvgroups; g.nhgroups = h.nhgroups; g.pad = 0;
+    g.vops = static_cast<const uint4*>(put(h.vops, 16 * h.nvops, 16 * std::max(h.nvops, 1)));
+    g.vgroups = static_cast<const uint4*>(put(h.vgroups, 16 * h.nvgroups, 16 * std::max(h.nvgroups, 1)));
+    g.hops = static_cast<const uint2*>(put(h.hops, 8 * h.nhops, 8 * std::max(h.nhops, 1)));
+    g.hgroups = static_cast<const uint4*>(put(h.hgroups, 16 * h.nhgroups, 16 * std::max(h.nhgroups, 1)));
+    g.out_src = static_cast<const int*>(put(h.out_src, 4 * h.nout, 4 * h.nout));
+    g.out_norm = static_cast<const double*>(put(h.out_norm, 8 * h.nout, 8 * h.nout));
+  }
+  HIPC(hipMemcpy(ctx->gprog_d.p, host.data(), total, hipMemcpyHostToDevice));
+  ctx->gprog = std::move(progs);
+  return RCHEM_OK;
+}
+
+// Kernel for class (la lb|lc ld) in `mode`; `out` stays invalid when neither
+// engine serves the class (angular momentum beyond the Boys grid's orders).
+int class_kernel(rchem_ctx* ctx, int la, int lb, int lc, int ld, int mode, int bm, int fp32,
+                 ClassKernel* out) {
+  *out = ClassKernel{};
+  out->mode = mode;
+  out->bm = bm;
+  if (!ctx->force_generic) {
+    out->fn = eri_launcher(la, lb, lc, ld, mode, bm, fp32);
+    if (out->fn) return RCHEM_OK;
+  }
+  if (la + lb + lc + ld > kBoysMaxM) return RCHEM_OK;
+  TRY(upload_programs(ctx));
+  for (int i = 0; i < kNumHostPrograms; ++i) {
+    const GenericProgram& g = ctx->gprog[i];
+    if (g.la == la && g.lb == lb && g.lc == lc && g.ld == ld) {
+      if (eri_generic_lds_bytes(g) <= 64 * 1024) out->gp = &g;
+      return RCHEM_OK;
+    }
+  }
+  return RCHEM_OK;
+}
+
+std::string class_name(int la, int lb, int lc, int ld) {
+  return "(" + std::to_string(la) + std::to_string(lb) + "|" + std::to_string(lc) +
+         std::to_string(ld) + ")";
+}
+
 int upload_tiles(rchem_ctx* ctx) {
   HIPC(ctx->tiles_d.ensure(sizeof(Tile) * std::max<size_t>(ctx->tiles.size(), 1)));
   if (!ctx->tiles.empty())
@@ -451,8 +551,11 @@ int compute_schwarz(rchem_ctx* ctx) {
     if (ranges[X].second == 0) continue;
     int l1, l2;
     class_ls(X, &l1, &l2);
-    EriLaunchFn fn = eri_launcher(l1, l2, l1, l2, kModeDiag, kBoysExact, 0);
-    if (!fn) return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class not compiled in");
+    ClassKernel fn;
+    TRY(class_kernel(ctx, l1, l2, l1, l2, kModeDiag, kBoysExact, 0, &fn));
+    if (!fn.valid())
+      return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class " + class_name(l1, l2, l1, l2) +
+                                                  " not served by this library");
     EriArgs a{};
     a.pairs = ctx->pairs_d.as<PairRec>();
     a.prims = ctx->prims_d.as<PrimRec>();
@@ -595,12 +698,13 @@ int prepare_host_tiles(rchem_ctx* ctx, double tau) {
     for (int j = lo; j < hi; j += 64) ctx->tiles.push_back(Tile{i, j, std::min(64, hi - j), 0});
   });
   flush();
-  for (const QClass& q : ctx->qclasses)
-    if (!eri_launcher(q.la, q.lb, q.lc, q.ld, kModeTensor, kBoysExact, 0))
-      return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class (" + std::to_string(q.la) +
-                                                  std::to_string(q.lb) + "|" +
-                                                  std::to_string(q.lc) + std::to_string(q.ld) +
-                                                  ") not compiled into this library");
+  for (const QClass& q : ctx->qclasses) {
+    ClassKernel fn;
+    TRY(class_kernel(ctx, q.la, q.lb, q.lc, q.ld, kModeTensor, kBoysExact, 0, &fn));
+    if (!fn.valid())
+      return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class " + class_name(q.la, q.lb, q.lc, q.ld) +
+                                                  " not served by this library");
+  }
   TRY(upload_tiles(ctx));
   ctx->tiles_tau = tau;
   return RCHEM_OK;
@@ -665,11 +769,14 @@ int prepare_plan(rchem_ctx* ctx, double tau) {
     st.primitive_quartets += qc.prim_quartets;
     st.integrals += qc.quartets * ncart(qc.la) * ncart(qc.lb) * ncart(qc.lc) * ncart(qc.ld);
     st.flops += qc.flops;
-    if (qc.quartets > 0 && !eri_launcher(qc.la, qc.lb, qc.lc, qc.ld, kModeJK, kBoysExact, 0))
-      return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class (" + std::to_string(qc.la) +
-                                                  std::to_string(qc.lb) + "|" +
-                                                  std::to_string(qc.lc) + std::to_string(qc.ld) +
-                                                  ") not compiled into this library");
+    if (qc.quartets > 0) {
+      ClassKernel fn;
+      TRY(class_kernel(ctx, qc.la, qc.lb, qc.lc, qc.ld, kModeJK, kBoysExact, 0, &fn));
+      if (!fn.valid())
+        return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class " +
+                                                    class_name(qc.la, qc.lb, qc.lc, qc.ld) +
+                                                    " not served by this library");
+    }
   }
   if (total_cap > 2.0e9) return ctx->fail(RCHEM_ERR_ALLOC, "quartet tile list too large");
   ctx->plan_cap = static_cast<int>(total_cap) + 1;
@@ -797,8 +904,11 @@ EriArgs base_args(const rchem_ctx* ctx) {
 // Host-tiled launch of every class in `mode` (diagonal / tensor modes).
 int run_eri_host(rchem_ctx* ctx, int mode, int bm, double* out) {
   for (const QClass& qc : ctx->qclasses) {
-    EriLaunchFn fn = eri_launcher(qc.la, qc.lb, qc.lc, qc.ld, mode, bm, 0);
-    if (!fn) return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class not compiled in");
+    ClassKernel fn;
+    TRY(class_kernel(ctx, qc.la, qc.lb, qc.lc, qc.ld, mode, bm, 0, &fn));
+    if (!fn.valid())
+      return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class " + class_name(qc.la, qc.lb, qc.lc, qc.ld) +
+                                                  " not served by this library");
     EriArgs a = base_args(ctx);
     a.tiles = ctx->tiles_d.as<Tile>() + qc.tile_begin;
     a.ntiles = qc.ntiles;
@@ -811,8 +921,8 @@ int run_eri_host(rchem_ctx* ctx, int mode, int bm, double* out) {
 constexpr int kPersistentBlocks = 2048;  // 256 CUs x 8 workgroups of 4 waves
 constexpr int kChunkTiles = 32;          // tiles per LDS-digestion chunk
 
-int launch_timed(rchem_ctx* ctx, const QClass& qc, int fp32, EriLaunchFn fn, const EriArgs& a,
-                 int blocks) {
+int launch_timed(rchem_ctx* ctx, const QClass& qc, int fp32, const ClassKernel& fn,
+                 const EriArgs& a, int blocks) {
   const bool timed = ctx->profiling;
   std::pair<hipEvent_t, hipEvent_t> ev{};
   if (timed) {
@@ -894,20 +1004,23 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
     a.jp_ket = ctx->jp_cls[qc.Y];
     // K rows of the bra pair in LDS when they fit in 64 KB
     const size_t lds = sizeof(double) * (ncart(qc.la) + ncart(qc.lb)) * static_cast<size_t>(ctx->nbf);
-    const bool use_lds = lds <= 65536 && ctx->lds_digestion;
+    const bool lds_ok = lds <= 65536 && ctx->lds_digestion && !ctx->force_generic;
     for (int prec = 0; prec < (mixed ? 2 : 1); ++prec) {
       const int bm = prec ? kBoysExact : o->boys_mode;
       a.tiles = prec ? ctx->plan_tiles32.as<Tile>() : ctx->plan_tiles64.as<Tile>();
-      if (use_lds) {
-        EriLaunchFn fn = eri_jk_lds_launcher(qc.la, qc.lb, qc.lc, qc.ld, bm, prec);
-        if (!fn) return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class not compiled in");
+      ClassKernel fn;
+      fn.fn = lds_ok ? eri_jk_lds_launcher(qc.la, qc.lb, qc.lc, qc.ld, bm, prec) : nullptr;
+      if (fn.valid()) {
         a.chunks = prec ? ctx->plan_chunks32.as<Chunk>() : ctx->plan_chunks64.as<Chunk>();
         a.chunk_range = ctx->plan_ranges.as<int>() + 8 * c + (prec ? 6 : 4);
         a.tile_range = nullptr;
         TRY(launch_timed(ctx, qc, prec, fn, a, std::min(kPersistentBlocks, qc.cap)));
       } else {
-        EriLaunchFn fn = eri_launcher(qc.la, qc.lb, qc.lc, qc.ld, kModeJK, bm, prec);
-        if (!fn) return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class not compiled in");
+        TRY(class_kernel(ctx, qc.la, qc.lb, qc.lc, qc.ld, kModeJK, bm, prec, &fn));
+        if (!fn.valid())
+          return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class " +
+                                                      class_name(qc.la, qc.lb, qc.lc, qc.ld) +
+                                                      " not served by this library");
         a.tile_range = ctx->plan_ranges.as<int>() + 8 * c + (prec ? 2 : 0);
         TRY(launch_timed(ctx, qc, prec, fn, a, std::min(kPersistentBlocks, (qc.cap + 3) / 4)));
       }
@@ -965,6 +1078,7 @@ int rchem_ctx_create(int device, rchem_ctx** out) {
   auto* ctx = new rchem_ctx();
   ctx->device = device;
   if (const char* e = std::getenv("RCHEM_LDS_DIGESTION")) ctx->lds_digestion = std::atoi(e) != 0;
+  if (const char* e = std::getenv("RCHEM_ERI_GENERIC")) ctx->force_generic = std::atoi(e) != 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
     g_create_error = "stream creation failed";
     delete ctx;
@@ -1352,11 +1466,11 @@ int rchem_eri_shell_quartets(rchem_ctx* ctx, int nq, const int* shells, double* 
     int la, lb, lc, ld;
     class_ls(cls_ls[c].first, &la, &lb);
     class_ls(cls_ls[c].second, &lc, &ld);
-    EriLaunchFn fn = eri_launcher(la, lb, lc, ld, kModeQuartet, o.boys_mode, 0);
-    if (!fn)
-      return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class (" + std::to_string(la) +
-                                                  std::to_string(lb) + "|" + std::to_string(lc) +
-                                                  std::to_string(ld) + ") not compiled into this library");
+    ClassKernel fn;
+    TRY(class_kernel(ctx, la, lb, lc, ld, kModeQuartet, o.boys_mode, 0, &fn));
+    if (!fn.valid())
+      return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class " + class_name(la, lb, lc, ld) +
+                                                  " not served by this library");
     EriArgs a = base_args(ctx);
     a.pairs = dpr.as<PairRec>();
     a.prims = dpp.as<PrimRec>();

@@ -9,6 +9,8 @@
 #pragma once
 #include <cstdint>
 
+#include <hip/hip_runtime.h>
+
 namespace rchem {
 
 // Primitives per shell accepted by the library (so a pair has at most 64
@@ -81,6 +83,32 @@ struct EriArgs {
   double* Jp;
   JpClass jp_bra, jp_ket;
 };
+
+// A class served by the wave-cooperative kernel: its recurrences as operation
+// tables in device memory (layouts in gen_eri_prog.py).  One wavefront
+// evaluates one contracted quartet; its LDS holds the primitive-quartet
+// records of a batch and one buffer of max(nacc + nv, nh) doubles that is the
+// VRR workspace + contracted accumulators during the primitive loop and the
+// HRR workspace afterwards.
+struct GenericProgram {
+  int la, lb, lc, ld;
+  int L;          // la + lb + lc + ld
+  int nacc;       // contracted accumulators [e0|f0]
+  int nv;         // VRR slots (slot 0 = zero, 1..L+1 = G_m)
+  int nh;         // HRR slots (the first nacc are the accumulators)
+  int nout;       // integrals of the class
+  int nvgroups, nhgroups;
+  int pad;
+  const uint4* vops;     // VRR operations
+  const uint4* vgroups;  // (op_begin, op_end, side | axis << 1 | level_end << 3, 0)
+  const uint2* hops;     // HRR operations
+  const uint4* hgroups;
+  const int* out_src;    // HRR slot of integral k (canonical component order)
+  const double* out_norm;  // per-component normalisation of integral k
+};
+
+// Primitive quartets set up per batch in the wave-cooperative kernel.
+constexpr int kGenericPqBatch = 32;
 
 // One quartet class as the device planner sees it.
 struct PlanClass {

@@ -52,9 +52,11 @@ def test_water_eri_tensor_exact_vs_oracle(gpu_ctx):
     assert np.max(np.abs(I - Io)) < 1e-12
 
 
-# (la lb|lc ld) classes compiled into the default library build
-COMPILED = {"0000", "1000", "1010", "1100", "1110", "1111", "2000", "2010", "2011", "2020",
-            "2100", "2110", "2200"}
+# every (la lb|lc ld) class with l <= 2: per-lane kernels up to L = 4, the
+# wave-cooperative kernel above
+ALL_CLASSES = {"0000", "1000", "1010", "1100", "1110", "1111", "2000", "2010", "2011", "2020",
+               "2100", "2110", "2111", "2120", "2121", "2200", "2210", "2211", "2220", "2221",
+               "2222"}
 
 
 def _canonical_class(ls):
@@ -74,11 +76,10 @@ def test_contracted_samples_all_classes(gpu_ctx):
     shell quartet, in the quartets' own (unsorted) shell order."""
     g = np.load(os.path.join(GOLD, "contracted_samples.npz"))
     ls = g["ls"]
-    sel = [qi for qi, q in enumerate(g["quartets"])
-           if _canonical_class([ls[s] for s in q]) in COMPILED]
+    sel = list(range(len(g["quartets"])))
     classes = {_canonical_class([ls[s] for s in g["quartets"][qi]]) for qi in sel}
     _say(f"case: contracted samples, {len(sel)} quartets over {len(classes)} classes")
-    assert len(classes) == len(COMPILED)
+    assert classes == ALL_CLASSES
     B = rb.Basis.from_shells(g["centres"], ls, g["nprim"], g["exps"], g["coefs"], ctx=gpu_ctx)
     blocks = rb.eri_shell_quartets(B, g["quartets"][sel],
                                    default_opts(boys_mode=BOYS_COMPAT, schwarz_thresh=0.0))
