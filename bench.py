@@ -93,6 +93,11 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--workload", choices=["h2o64", "eri2000"], default="h2o64",
+                    help="h2o64: (H2O)_64 6-31G Fock build (default); eri2000: the synthetic "
+                         "2000-shell s/p/d ERI microbenchmark, per class")
+    ap.add_argument("--shells", type=int, default=2000,
+                    help="eri2000 workload: number of shells (the cube grows to keep the density)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (for HBM counter scaling)")
     args = ap.parse_args()
@@ -105,14 +110,29 @@ def main():
 
     from rchem_amd import basis as rb
     from rchem_amd._lib import (PREC_FP64, PREC_MIXED, ClassTiming, default_opts, lib)
-    from rchem_amd.inputs import nocc_of, water_cluster
+    from rchem_amd.inputs import nocc_of, synthetic_shells, water_cluster
 
-    Z, xyz = water_cluster(dims=(4, 4, 4), seed=64)
+    t_start = time.perf_counter()
     ctx = rb.Context(0)
-    B = rb.Basis.new(Z, xyz, "6-31G", ctx=ctx)
-    n = B.nbf
-    S, T, V = rb.one_electron(B, xyz, Z)
-    D = core_guess_density(S, T + V, nocc_of(Z))
+    if args.workload == "h2o64":
+        Z, xyz = water_cluster(dims=(4, 4, 4), seed=64)
+        B = rb.Basis.new(Z, xyz, "6-31G", ctx=ctx)
+        n = B.nbf
+        S, T, V = rb.one_electron(B, xyz, Z)
+        D = core_guess_density(S, T + V, nocc_of(Z))
+        workload = ("(H2O)_64 6-31G Fock build (J and K), seed 64, 4x4x4 lattice 3.1 A, "
+                    "Schwarz 1e-12")
+    else:
+        box = 20.0 * (args.shells / 2000.0) ** (1.0 / 3.0)
+        sh = synthetic_shells(nshell=args.shells, seed=2000, box=box, nprim=3)
+        B = rb.Basis.from_shells(*sh, ctx=ctx)
+        n = B.nbf
+        rng = np.random.default_rng(2000)
+        X = rng.standard_normal((n, n)) / n
+        D = X + X.T
+        workload = (f"synthetic ERI microbenchmark: {args.shells} shells, seed 2000, "
+                    f"{box:.1f}-bohr cube, l 0/1/2 at 0.5/0.35/0.15, 3 primitives, every quartet "
+                    f"surviving Schwarz 1e-12, digested into J and K")
     prec = PREC_FP64 if args.precision == "fp64" else PREC_MIXED
     opts = default_opts(precision=prec, schwarz_thresh=1e-12)
 
@@ -134,9 +154,13 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    for _ in range(args.warmup):
+    def say(msg):
+        print(f"[bench] {msg} ({time.perf_counter() - t_start:.1f} s)", file=sys.stderr, flush=True)
+
+    for i in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        say(f"warmup step {i + 1}/{args.warmup} done")
     ctx.check(L.rchem_reset_class_timings(ctx.h))
     ctx.check(L.rchem_set_profiling(ctx.h, 1))
     torch.cuda.synchronize()
@@ -145,6 +169,7 @@ def main():
         step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    say(f"{args.steps} timed steps done")
     ctx.check(L.rchem_set_profiling(ctx.h, 0))
     st = rb.jk_last_stats(B)
     timings = (ClassTiming * 64)()
@@ -153,9 +178,9 @@ def main():
     classes = []
     for i in range(cnt.value):
         t = timings[i]
-        avg = t.ms / max(t.launches, 1)
+        avg = t.ms / args.steps   # per build (a large build launches a class once per pass)
         classes.append({"class": f"({t.la}{t.lb}|{t.lc}{t.ld})" + (" fp32" if t.fp32 else ""),
-                        "avg_ms": avg,
+                        "avg_ms": avg, "launches_per_build": t.launches / args.steps,
                         "quartets": t.quartets, "prim_quartets": t.primitive_quartets,
                         "flops": t.flops,
                         "tflops": t.flops / (avg * 1e-3) / 1e12 if avg > 0 else 0.0})
@@ -179,8 +204,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if prec == PREC_FP64 else "f64+f32",
         "data": "synthetic",
-        "config": {"workload": "(H2O)_64 6-31G Fock build (J and K), seed 64, 4x4x4 lattice "
-                               "3.1 A, Schwarz 1e-12", "nbf": n, "shells": len(B.shells()[1]),
+        "config": {"workload": workload, "nbf": n, "shells": len(B.shells()[1]),
                    "precision": args.precision},
         "quartets": st.quartets,
         "quartets_fp32": st.quartets_fp32,
@@ -196,7 +220,10 @@ def main():
                               "FP64 VALU kernel; MI355X FP64 vector peak = FP64 matrix peak")},
         "classes": classes,
     }
-    if not args.no_cpu_baseline:
+    if args.workload == "eri2000":
+        out["metric"] = "ERI shell quartets/s per (la lb|lc ld) class, synthetic 2000-shell microbench"
+        out["roofline"]["traffic"] = None
+    if not args.no_cpu_baseline and args.workload == "h2o64":
         out["cpu_baseline"] = cpu_baseline(Z, xyz, D, st.quartets, args.cpu_seconds)
     print(json.dumps(out), flush=True)
     ctx.close()

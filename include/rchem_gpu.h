@@ -174,6 +174,10 @@ typedef struct rchem_scf_opts {
   double thresh_e;     /* |dE_elec| convergence, default 1e-11 (rchem.rs:63) */
   rchem_opts jk;       /* integral options */
   int verbose;         /* 1: print rchem's iteration line (rchem.rs:83) */
+  int diis;            /* 0 (default): rchem's plain iterations; k > 0: Pulay DIIS over
+                          the last k Fock matrices with the orthogonalised commutator
+                          X^T (FDS - SDF) X as error, converged when also
+                          max |error| < 1e-8 (rchem has no DIIS, rchem.rs:72-90) */
 } rchem_scf_opts;
 
 typedef struct rchem_scf_result {
@@ -192,6 +196,21 @@ int rchem_scf_opts_default(rchem_scf_opts *o);
  * density.  Returns RCHEM_ERR_NOT_CONVERGED if max_iter is reached. */
 int rchem_scf(rchem_ctx *ctx, int natom, const double *Z, const double *xyz,
               const rchem_scf_opts *opts, rchem_scf_result *res, double *D_out);
+
+/* Analytic RHF nuclear gradient on the current basis (rchem has no derivative
+ * code; the energy it differentiates is rchem's, rchem.rs:72-90, plus the
+ * nuclear repulsion).  Runs rchem_scf's iterations until |dE_elec| <
+ * opts->thresh_e and the density changes by less than 1e-9 per iteration,
+ * then
+ *   dE/dR = 2 sum D dH + sum Gamma d(mu nu|la si) - 2 sum W dS + dE_nuc/dR,
+ *   Gamma = 2 D_mn D_ls - (D_ml D_ns + D_ms D_nl) / 2,  W = C_occ eps_occ C_occ^T,
+ * with the derivative integrals recomputed on the device (no quartet is
+ * stored).  grad: 3 x natom doubles, Eh/bohr, atom-major.  Every shell centre
+ * must coincide with one of the nuclei.  res as for rchem_scf.  Classes whose
+ * derivative recurrences exceed one workgroup's LDS ((dd|dp) and (dd|dd))
+ * return RCHEM_ERR_UNSUPPORTED. */
+int rchem_rhf_gradient(rchem_ctx *ctx, int natom, const double *Z, const double *xyz,
+                       const rchem_scf_opts *opts, rchem_scf_result *res, double *grad);
 
 #ifdef __cplusplus
 }

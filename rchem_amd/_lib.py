@@ -45,7 +45,8 @@ class ClassTiming(ctypes.Structure):
 
 class SCFOpts(ctypes.Structure):
     _fields_ = [("nocc", ctypes.c_int), ("max_iter", ctypes.c_int),
-                ("thresh_e", ctypes.c_double), ("jk", Opts), ("verbose", ctypes.c_int)]
+                ("thresh_e", ctypes.c_double), ("jk", Opts), ("verbose", ctypes.c_int),
+                ("diis", ctypes.c_int)]
 
 
 class SCFResult(ctypes.Structure):
@@ -82,6 +83,9 @@ SIGNATURES = {
     "rchem_jk_class_timings": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ClassTiming),
                                               ctypes.c_int, _ip]),
     "rchem_scf_opts_default": (ctypes.c_int, [ctypes.POINTER(SCFOpts)]),
+    "rchem_rhf_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp,
+                                          ctypes.POINTER(SCFOpts), ctypes.POINTER(SCFResult),
+                                          _dp]),
     "rchem_scf": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp,
                                  ctypes.POINTER(SCFOpts), ctypes.POINTER(SCFResult), _dp]),
 }

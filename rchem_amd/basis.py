@@ -201,10 +201,12 @@ def schwarz(basis: Basis):
 
 
 def scf(basis: Basis, atomcoords, atomnos, nocc, thresh_e=1e-11, max_iter=1024,
-        opts: Opts | None = None, verbose=False):
-    """rchem.rs:10-90 on the device: returns (SCFResult, D)."""
+        opts: Opts | None = None, verbose=False, diis=0):
+    """rchem.rs:10-90 on the device: returns (SCFResult, D).  diis = k > 0
+    replaces rchem's plain iterations by DIIS over k Fock matrices."""
     o = SCFOpts()
     lib().rchem_scf_opts_default(ctypes.byref(o))
+    o.diis = int(diis)
     o.nocc = int(nocc)
     o.max_iter = int(max_iter)
     o.thresh_e = float(thresh_e)
@@ -221,6 +223,30 @@ def scf(basis: Basis, atomcoords, atomnos, nocc, thresh_e=1e-11, max_iter=1024,
     return res, D
 
 
+def rhf_gradient(basis: Basis, atomcoords, atomnos, nocc, thresh_e=1e-11, max_iter=1024,
+                 opts: Opts | None = None, diis=0):
+    """Analytic RHF nuclear gradient (Eh/bohr) after rchem's SCF on the device
+    (DIIS over `diis` Fock matrices when > 0): returns (SCFResult, grad of
+    shape (natom, 3))."""
+    o = SCFOpts()
+    lib().rchem_scf_opts_default(ctypes.byref(o))
+    o.diis = int(diis)
+    o.nocc = int(nocc)
+    o.max_iter = int(max_iter)
+    o.thresh_e = float(thresh_e)
+    o.jk = opts or default_opts()
+    xyz = np.ascontiguousarray(atomcoords, dtype=np.float64).reshape(-1, 3)
+    Z = np.ascontiguousarray(atomnos, dtype=np.float64)
+    res = SCFResult()
+    g = np.zeros((len(Z), 3))
+    rc = lib().rchem_rhf_gradient(basis.ctx.h, len(Z), _dptr(Z), _dptr(xyz), ctypes.byref(o),
+                                  ctypes.byref(res), _dptr(g))
+    if rc not in (0, 6):
+        basis.ctx.check(rc)
+    return res, g
+
+
 __all__ = ["Context", "Basis", "S", "T", "V", "JK_direct", "JK_inmem", "build_I", "scf",
+           "rhf_gradient",
            "eri_shell_quartets",
            "one_electron", "schwarz", "jk_last_stats", "BOYS_EXACT", "BOYS_COMPAT"]

@@ -1,27 +1,30 @@
 // Wave-cooperative contracted ERIs for the classes whose integrals do not fit
 // one lane's registers (total angular momentum L >= 5: (dp|pp) ... (dd|dd)),
-// and for any class on request.
+// for any class on request, and the derivative integrals of the nuclear
+// gradient.
 //
 // One wavefront (a 64-thread workgroup) evaluates one contracted shell
-// quartet at a time:
-//   1. lanes set up a batch of up to kGenericPqBatch primitive quartets in
-//      parallel (exponent factors, PA, WP, QC, WQ and G_m = K_ab K_cd
-//      (zeta+eta)^(-1/2) F_m(T), m = 0..L), one record each, into LDS;
-//   2. per primitive quartet the Head-Gordon--Pople vertical recurrence runs
-//      as the class's operation table: operations of one dependency level are
-//      independent and spread over the 64 lanes, a level boundary is a
-//      workgroup barrier (one wave), and within a level the operations come in
-//      groups of one (side, axis), whose recurrence coefficients are
-//      wave-uniform.  Target integrals [e0|f0]^(0) add into the contracted
-//      accumulators as they are produced;
-//   3. after the primitive loop the horizontal recurrence runs the same way on
-//      the accumulators, which are the first slots of the same buffer;
-//   4. lanes take the class's integrals in canonical component order for the
-//      output mode (shell-quartet blocks, dense tensor, Schwarz diagonal, or
-//      J/K digestion with FP64 atomics and the 8-fold weights).
+// quartet at a time, P primitive quartets per batch (P a power of two, the
+// class's table value or less for short contractions):
+//   1. lanes 0..P-1 set up the batch's primitive-quartet records (exponent
+//      factors, PA, WP, QC, WQ, 2 alpha of three centres and
+//      G_m = K_ab K_cd (zeta+eta)^(-1/2) F_m(T), m = 0..L) field-major in LDS;
+//   2. the Head-Gordon--Pople vertical recurrence runs as the class's
+//      operation table on V[slot][q]: lane l always serves primitive quartet
+//      q = l mod P and takes operations l / P, l / P + 64 / P, ... of a group,
+//      a group being operations of one dependency level and one (side, axis)
+//      whose recurrence coefficients the lane loads once; a level boundary is a
+//      workgroup barrier (one wave);
+//   3. the accumulation list adds each target [e0|f0]^(0), summed over the
+//      batch with its set's weight (1 or 2 alpha of a centre), into the
+//      contracted accumulators, one lane per accumulator;
+//   4. after the primitive loop the horizontal recurrence runs the same way on
+//      the accumulators (the first slots of the same buffer);
+//   5. lanes take the outputs norm (H[plus] - n H[minus]) for the mode:
+//      shell-quartet blocks, dense tensor, Schwarz diagonal, J/K digestion
+//      with FP64 atomics and the 8-fold weights, or the gradient contraction.
 // The recurrences are the straight-line generator's (same nodes, same axis
-// choices), so both engines agree to rounding; the reference's HGP is
-// chgp.c:44-135 and chgp.c:412-586.
+// choices); the reference's HGP is chgp.c:44-135 and chgp.c:412-586.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,11 +36,17 @@ namespace rchem {
 
 namespace {
 
-// Record fields of one primitive quartet
+// Record fields of one primitive quartet (stored field-major: rec[f * P + q])
 constexpr int kPA = 0, kWP = 3, kQC = 6, kWQ = 9, kOo2z = 12, kOo2e = 13, kOo2ze = 14,
-              kRoz = 15, kRoe = 16, kG = 17;
+              kRoz = 15, kRoe = 16, kWA = 17, kWB = 18, kWC = 19, kG = 20;
 
-__device__ __forceinline__ int rec_size(int L) { return kG + L + 1; }
+__device__ __forceinline__ int rec_fields(int L) { return kG + L + 1; }
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
 
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
@@ -45,91 +54,156 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
+// pref * F_m(T), m = 0..M, into out[m * stride]: the exact Boys function of
+// boys_exact_rt (Taylor row of F_M on the T0 grid, exp row, downward
+// recursion; asymptote and upward recursion for T >= 36).
+__device__ void boys_exact_strided(int M, double T, const double* __restrict__ table, double pref,
+                                   double* __restrict__ out, int stride) {
+  if (T < kBoysTMax) {
+    const int k = static_cast<int>(T * kBoysInvH + 0.5);
+    const double d = k * (1.0 / kBoysInvH) - T;
+    const double* c = table + M * kBoysBlock + k * kBoysTerms;
+    double f = c[kBoysTerms - 1];
+    for (int j = kBoysTerms - 2; j >= 0; --j) f = c[j] + d * f;
+    double pe = 1.0;
+    for (int j = kBoysTerms - 1; j >= 1; --j) pe = 1.0 + d * (pe / j);
+    const double e = table[kBoysExpOffset + k] * pe;
+    out[M * stride] = pref * f;
+    for (int m = M - 1; m >= 0; --m) {
+      f = (2.0 * T * f + e) / (2 * m + 1);
+      out[m * stride] = pref * f;
+    }
+  } else {
+    double f = 0.88622692545275801365 / sqrt(T);
+    const double e = exp(-T);
+    out[0] = pref * f;
+    for (int m = 0; m < M; ++m) {
+      f = ((2 * m + 1) * f - e) / (2.0 * T);
+      out[(m + 1) * stride] = pref * f;
+    }
+  }
+}
+
+// Record q of the batch for primitive pairs ib of the bra and ik of the ket;
+// with valid == false every field is zero (a padding slot of the batch).
 template <int BM>
 __device__ void setup_record(const EriArgs& a, const PairRec& bp, const PairRec& kp, int ib,
-                             int ik, int L, double* __restrict__ rec) {
-  const PrimRec pb = a.prims[bp.p0 + ib * bp.ps];
-  const PrimRec pk = a.prims[kp.p0 + ik * kp.ps];
+                             int ik, bool valid, int L, int P, int q, double* __restrict__ rec) {
+  const int nf = rec_fields(L);
+  if (!valid) {
+    for (int f = 0; f < nf; ++f) rec[f * P + q] = 0.0;
+    return;
+  }
+  const int ipb = bp.p0 + ib * bp.ps, ipk = kp.p0 + ik * kp.ps;
+  const PrimRec pb = a.prims[ipb];
+  const PrimRec pk = a.prims[ipk];
   const double zb = pb.zeta, zk = pk.zeta;
   const double rs = fast_rsqrt(zb + zk);
   const double ooze = rs * rs;
   const double roz = zk * ooze, roe = zb * ooze;
   const double PQx = pb.px - pk.px, PQy = pb.py - pk.py, PQz = pb.pz - pk.pz;
   const double T = zb * roz * (PQx * PQx + PQy * PQy + PQz * PQz);
-  rec[kPA + 0] = pb.px - bp.A[0];
-  rec[kPA + 1] = pb.py - bp.A[1];
-  rec[kPA + 2] = pb.pz - bp.A[2];
-  rec[kWP + 0] = -roz * PQx;
-  rec[kWP + 1] = -roz * PQy;
-  rec[kWP + 2] = -roz * PQz;
-  rec[kQC + 0] = pk.px - kp.A[0];
-  rec[kQC + 1] = pk.py - kp.A[1];
-  rec[kQC + 2] = pk.pz - kp.A[2];
-  rec[kWQ + 0] = roe * PQx;
-  rec[kWQ + 1] = roe * PQy;
-  rec[kWQ + 2] = roe * PQz;
-  rec[kOo2z] = pb.oo2z;
-  rec[kOo2e] = pk.oo2z;
-  rec[kOo2ze] = 0.5 * ooze;
-  rec[kRoz] = roz;
-  rec[kRoe] = roe;
-  double* G = rec + kG;
-  if constexpr (BM == kBoysCompat) {
-    for (int m = 0; m <= L; ++m) G[m] = fgamma_compat(static_cast<double>(m), T);
+  rec[(kPA + 0) * P + q] = pb.px - bp.A[0];
+  rec[(kPA + 1) * P + q] = pb.py - bp.A[1];
+  rec[(kPA + 2) * P + q] = pb.pz - bp.A[2];
+  rec[(kWP + 0) * P + q] = -roz * PQx;
+  rec[(kWP + 1) * P + q] = -roz * PQy;
+  rec[(kWP + 2) * P + q] = -roz * PQz;
+  rec[(kQC + 0) * P + q] = pk.px - kp.A[0];
+  rec[(kQC + 1) * P + q] = pk.py - kp.A[1];
+  rec[(kQC + 2) * P + q] = pk.pz - kp.A[2];
+  rec[(kWQ + 0) * P + q] = roe * PQx;
+  rec[(kWQ + 1) * P + q] = roe * PQy;
+  rec[(kWQ + 2) * P + q] = roe * PQz;
+  rec[kOo2z * P + q] = pb.oo2z;
+  rec[kOo2e * P + q] = pk.oo2z;
+  rec[kOo2ze * P + q] = 0.5 * ooze;
+  rec[kRoz * P + q] = roz;
+  rec[kRoe * P + q] = roe;
+  if (a.prim_alpha) {
+    const double aa = a.prim_alpha[ipb], ac = a.prim_alpha[ipk];
+    rec[kWA * P + q] = 2.0 * aa;
+    rec[kWB * P + q] = 2.0 * (zb - aa);
+    rec[kWC * P + q] = 2.0 * ac;
   } else {
-    boys_exact_rt(L, T, a.boys_table, G);
+    rec[kWA * P + q] = 0.0;
+    rec[kWB * P + q] = 0.0;
+    rec[kWC * P + q] = 0.0;
   }
   const double pref = pb.kab * pk.kab * rs;
-  for (int m = 0; m <= L; ++m) G[m] *= pref;
+  if constexpr (BM == kBoysCompat) {
+    for (int m = 0; m <= L; ++m)
+      rec[(kG + m) * P + q] = pref * fgamma_compat(static_cast<double>(m), T);
+  } else {
+    boys_exact_strided(L, T, a.boys_table, pref, rec + kG * P + q, P);
+  }
 }
 
-// One contracted quartet (bra pair bidx, ket pair kidx) into H[out_src[k]].
+// One contracted quartet (bra pair bidx, ket pair kidx): the HRR results end in
+// H = R; the outputs are read by the epilogue.
 template <int BM>
 __device__ void quartet(const EriArgs& a, const GenericProgram& p, int bidx, int kidx,
-                        double* __restrict__ pq, double* __restrict__ R, int lane) {
+                        double* __restrict__ rec, double* __restrict__ R, int lane) {
   const PairRec bp = a.pairs[bidx];
   const PairRec kp = a.pairs[kidx];
-  const int L = p.L, RS = rec_size(L);
-  __syncthreads();  // the previous quartet's integrals have been read
+  const int L = p.L;
+  __syncthreads();  // the previous quartet's outputs have been read
   double* __restrict__ acc = R;
-  double* __restrict__ V = R + p.nacc;
   for (int k = lane; k < p.nacc; k += 64) acc[k] = 0.0;
-  if (lane == 0) V[0] = 0.0;
   const int nk = kp.np, npq = bp.np * kp.np;
-  for (int b0 = 0; b0 < npq; b0 += kGenericPqBatch) {
-    const int nb = min(kGenericPqBatch, npq - b0);
-    __syncthreads();  // the previous batch's records are consumed
-    if (lane < nb) {
+  int P = 1, LP = 0;
+  while (P < p.P && P < npq) {
+    P <<= 1;
+    ++LP;
+  }
+  double* __restrict__ V = R + p.nacc;
+  const int q = lane & (P - 1);
+  const int ostep = 64 >> LP;
+  for (int b0 = 0; b0 < npq; b0 += P) {
+    __syncthreads();  // the previous batch's records and slots are consumed
+    if (lane < P) {
       const int j = b0 + lane;
-      setup_record<BM>(a, bp, kp, j / nk, j % nk, L, pq + lane * RS);
+      const bool valid = j < npq;
+      setup_record<BM>(a, bp, kp, valid ? j / nk : 0, valid ? j % nk : 0, valid, L, P, lane, rec);
     }
     __syncthreads();
-    for (int q = 0; q < nb; ++q) {
-      const double* __restrict__ rec = pq + q * RS;
-      if (lane <= L) V[1 + lane] = rec[kG + lane];
-      __syncthreads();
-      for (int g = 0; g < p.nvgroups; ++g) {
-        const uint4 gr = p.vgroups[g];
-        const int side = gr.z & 1, ax = (gr.z >> 1) & 3;
-        const double c0 = rec[(side ? kQC : kPA) + ax];
-        const double c1 = rec[(side ? kWQ : kWP) + ax];
-        const double o = rec[side ? kOo2e : kOo2z];
-        const double r = rec[side ? kRoe : kRoz];
-        const double oz = rec[kOo2ze];
-        for (int op = static_cast<int>(gr.x) + lane; op < static_cast<int>(gr.y); op += 64) {
-          const uint4 w = p.vops[op];
-          const double n1 = static_cast<double>((w.w >> 16) & 15u);
-          const double n2 = static_cast<double>((w.w >> 20) & 15u);
-          double v = c0 * V[w.x >> 16] + c1 * V[w.y & 0xffffu];
-          v += (n1 * o) * (V[w.y >> 16] - r * V[w.z & 0xffffu]);
-          v += (n2 * oz) * V[w.z >> 16];
-          V[w.x & 0xffffu] = v;
-          const unsigned ai = w.w & 0xffffu;
-          if (ai != 0xffffu) acc[ai] += v;
-        }
-        if (gr.z & 8u) __syncthreads();
+    for (int idx = lane; idx < (L + 2) * P; idx += 64) {
+      const int s = idx >> LP, qq = idx & (P - 1);
+      V[idx] = s == 0 ? 0.0 : rec[(kG + s - 1) * P + qq];
+    }
+    __syncthreads();
+    for (int g = 0; g < p.nvgroups; ++g) {
+      const uint4 gr = p.vgroups[g];
+      const int side = gr.z & 1, ax = (gr.z >> 1) & 3;
+      const double c0 = rec[((side ? kQC : kPA) + ax) * P + q];
+      const double c1 = rec[((side ? kWQ : kWP) + ax) * P + q];
+      const double o = rec[(side ? kOo2e : kOo2z) * P + q];
+      const double r = rec[(side ? kRoe : kRoz) * P + q];
+      const double oz = rec[kOo2ze * P + q];
+      for (int op = static_cast<int>(gr.x) + (lane >> LP); op < static_cast<int>(gr.y);
+           op += ostep) {
+        const uint4 w = p.vops[op];
+        const double n1 = static_cast<double>(w.w & 15u);
+        const double n2 = static_cast<double>((w.w >> 4) & 15u);
+        double v = c0 * V[(w.x >> 16) * P + q] + c1 * V[(w.y & 0xffffu) * P + q];
+        v += (n1 * o) * (V[(w.y >> 16) * P + q] - r * V[(w.z & 0xffffu) * P + q]);
+        v += (n2 * oz) * V[(w.z >> 16) * P + q];
+        V[(w.x & 0xffffu) * P + q] = v;
       }
-      if (p.nvgroups == 0 && lane == 0) acc[0] += V[1];  // (ss|ss): the target is G_0
+      if (gr.z & 8u) __syncthreads();
+    }
+    for (int t = lane; t < p.naccs; t += 64) {
+      const uint2 e = p.accs[t];
+      const double* src = V + (e.x & 0xffffu) * P;
+      const int set = static_cast<int>(e.x >> 16);
+      double s = 0.0;
+      if (set == 0) {
+        for (int qq = 0; qq < P; ++qq) s += src[qq];
+      } else {
+        const double* wt = rec + (kWA + set - 1) * P;
+        for (int qq = 0; qq < P; ++qq) s += wt[qq] * src[qq];
+      }
+      acc[e.y] += s;
     }
   }
   __syncthreads();
@@ -152,6 +226,22 @@ __device__ void quartet(const EriArgs& a, const GenericProgram& p, int bidx, int
 
 __device__ __forceinline__ int ncart_rt(int l) { return (l + 1) * (l + 2) / 2; }
 
+__device__ __forceinline__ double output(const GenericProgram& p, const double* __restrict__ H,
+                                         int k) {
+  const int4 o = p.outs[k];
+  return p.out_norm[k] * (H[o.x] - static_cast<double>(o.z) * H[o.y]);
+}
+
+// 8-fold weight deg/8 of a unique quartet of pairs (bidx, kidx)
+__device__ __forceinline__ double sym_weight(const PairRec& bp, const PairRec& kp, int bidx,
+                                             int kidx) {
+  double s = 0.125;
+  if (bp.flags & 1) s *= 2.0;
+  if (kp.flags & 1) s *= 2.0;
+  if (bidx != kidx) s *= 2.0;
+  return s;
+}
+
 template <int MODE>
 __device__ void epilogue(const EriArgs& a, const GenericProgram& p, const Tile& t, int bidx,
                          int kidx, const double* __restrict__ H, int lane) {
@@ -162,19 +252,18 @@ __device__ void epilogue(const EriArgs& a, const GenericProgram& p, const Tile& 
   const int fa = bp.fa, fb = bp.fb, fc = kp.fa, fd = kp.fb;
   if constexpr (MODE == kModeQuartet) {
     for (int k = lane; k < p.nout; k += 64)
-      a.out[static_cast<size_t>(t.pad) + k] = p.out_norm[k] * H[p.out_src[k]];
+      a.out[static_cast<size_t>(t.pad) + k] = output(p, H, k);
   } else if constexpr (MODE == kModeDiag) {
     double m = 0.0;
     for (int k = lane; k < NA * NB; k += 64) {
       const int ia = k / NB, ib = k % NB;
-      const int o = ((ia * NB + ib) * NC + ia) * ND + ib;
-      m = fmax(m, fabs(p.out_norm[o] * H[p.out_src[o]]));
+      m = fmax(m, fabs(output(p, H, ((ia * NB + ib) * NC + ia) * ND + ib)));
     }
     m = wave_max(m);
     if (lane == 0) a.out[kidx] = sqrt(m);
   } else if constexpr (MODE == kModeTensor) {
     for (int k = lane; k < p.nout; k += 64) {
-      const double v = p.out_norm[k] * H[p.out_src[k]];
+      const double v = output(p, H, k);
       const int id = k % ND, ic = (k / ND) % NC, ib = (k / (ND * NC)) % NB, ia = k / (ND * NC * NB);
       const size_t P = fa + ia, Q = fb + ib, Rr = fc + ic, S = fd + id;
       double* I = a.out;
@@ -187,17 +276,13 @@ __device__ void epilogue(const EriArgs& a, const GenericProgram& p, const Tile& 
       I[((Rr * n + S) * n + Q) * n + P] = v;
       I[((S * n + Rr) * n + Q) * n + P] = v;
     }
-  } else {
-    // 8-fold weight: each unique (ab|cd) stands for deg ordered quartets; the
-    // lower-half accumulators get deg/8 of every permutation's contribution
-    // and J + J^T, K + K^T restore the rest
-    double s = 0.125;
-    if (bp.flags & 1) s *= 2.0;
-    if (kp.flags & 1) s *= 2.0;
-    if (bidx != kidx) s *= 2.0;
+  } else if constexpr (MODE == kModeJK) {
+    // the lower-half accumulators get deg/8 of every permutation's
+    // contribution; J + J^T, K + K^T restore the rest
+    const double s = sym_weight(bp, kp, bidx, kidx);
     const double* __restrict__ D = a.D;
     for (int k = lane; k < p.nout; k += 64) {
-      const double v = s * p.out_norm[k] * H[p.out_src[k]];
+      const double v = s * output(p, H, k);
       const int id = k % ND, ic = (k / ND) % NC, ib = (k / (ND * NC)) % NB, ia = k / (ND * NC * NB);
       const size_t A = fa + ia, B = fb + ib, C = fc + ic, Dd = fd + id;
       unsafeAtomicAdd(&a.J[A * n + B], 2.0 * v * D[C * n + Dd]);
@@ -207,6 +292,38 @@ __device__ void epilogue(const EriArgs& a, const GenericProgram& p, const Tile& 
       unsafeAtomicAdd(&a.K[B * n + C], v * D[A * n + Dd]);
       unsafeAtomicAdd(&a.K[B * n + Dd], v * D[A * n + C]);
     }
+  } else {  // kModeGrad
+    // dE/dX = sum over ordered quartets of Gamma d(mu nu|la si)/dX with
+    // Gamma = 2 D_mn D_ls - (D_ml D_ns + D_ms D_nl) / 2; a unique quartet
+    // stands for deg = 8 s ordered ones.  Outputs: (centre A, B, C) x (x, y, z)
+    // x components; centre D by translational invariance.
+    const int nc = NA * NB * NC * ND;
+    const double deg = 8.0 * sym_weight(bp, kp, bidx, kidx);
+    const double* __restrict__ D = a.D;
+    double g[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) g[c] = 0.0;
+    for (int k = lane; k < nc; k += 64) {
+      const int id = k % ND, ic = (k / ND) % NC, ib = (k / (ND * NC)) % NB, ia = k / (ND * NC * NB);
+      const size_t A = fa + ia, B = fb + ib, C = fc + ic, Dd = fd + id;
+      const double gam = 2.0 * D[A * n + B] * D[C * n + Dd] -
+                         0.5 * (D[A * n + C] * D[B * n + Dd] + D[A * n + Dd] * D[B * n + C]);
+#pragma unroll
+      for (int c = 0; c < 9; ++c) g[c] += gam * output(p, H, c * nc + k);
+    }
+    const int2 ab = a.pair_atoms[bidx], cd = a.pair_atoms[kidx];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double gA = deg * wave_sum(g[i]);
+      const double gB = deg * wave_sum(g[3 + i]);
+      const double gC = deg * wave_sum(g[6 + i]);
+      if (lane == 0) {
+        unsafeAtomicAdd(&a.grad[3 * ab.x + i], gA);
+        unsafeAtomicAdd(&a.grad[3 * ab.y + i], gB);
+        unsafeAtomicAdd(&a.grad[3 * cd.x + i], gC);
+        unsafeAtomicAdd(&a.grad[3 * cd.y + i], -(gA + gB + gC));
+      }
+    }
   }
 }
 
@@ -214,8 +331,8 @@ template <int MODE, int BM>
 __global__ __launch_bounds__(64) void eri_generic_kernel(const EriArgs a, const GenericProgram p) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x;
-  double* pq = lds;
-  double* R = lds + kGenericPqBatch * rec_size(p.L);
+  double* rec = lds;
+  double* R = lds + p.P * rec_fields(p.L);
   int begin, end;
   if (a.tile_range) {
     begin = a.tile_range[0];
@@ -230,7 +347,7 @@ __global__ __launch_bounds__(64) void eri_generic_kernel(const EriArgs a, const 
     for (int i = 0; i < t.count; ++i) {
       const int kidx = t.ket0 + i;
       const int bidx = (MODE == kModeDiag) ? kidx : t.bra;
-      quartet<BM>(a, p, bidx, kidx, pq, R, lane);
+      quartet<BM>(a, p, bidx, kidx, rec, R, lane);
       epilogue<MODE>(a, p, t, bidx, kidx, R, lane);
     }
   }
@@ -247,9 +364,10 @@ hipError_t launch_mode(const EriArgs& a, const GenericProgram& p, unsigned grid,
 
 // LDS bytes of one workgroup for program p.
 size_t eri_generic_lds_bytes(const GenericProgram& p) {
-  const size_t rs = 17 + p.L + 1;
-  const size_t buf = std::max(static_cast<size_t>(p.nacc + p.nv), static_cast<size_t>(p.nh));
-  return sizeof(double) * (kGenericPqBatch * rs + buf);
+  const size_t nf = 20 + p.L + 1;
+  const size_t buf = std::max(static_cast<size_t>(p.nacc) + static_cast<size_t>(p.nv) * p.P,
+                              static_cast<size_t>(p.nh));
+  return sizeof(double) * (static_cast<size_t>(p.P) * nf + buf);
 }
 
 // blocks: with a device tile range, the number of 4-wave blocks the per-lane
@@ -265,8 +383,9 @@ hipError_t eri_generic_launch(const EriArgs& a, const GenericProgram& p, int mod
     if (a.ntiles <= 0) return hipSuccess;
     grid = static_cast<unsigned>(a.ntiles);
   }
+  if (p.P < 1 || p.P > 64 || (p.P & (p.P - 1)) != 0 || p.L > kBoysMaxM) return hipErrorInvalidValue;
   const size_t lds = eri_generic_lds_bytes(p);
-  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  if (lds > 64 * 1024) return hipErrorInvalidValue;
   const bool compat = bm == kBoysCompat;
   switch (mode) {
     case kModeJK:
@@ -281,6 +400,8 @@ hipError_t eri_generic_launch(const EriArgs& a, const GenericProgram& p, int mod
     case kModeQuartet:
       return compat ? launch_mode<kModeQuartet, kBoysCompat>(a, p, grid, lds, s)
                     : launch_mode<kModeQuartet, kBoysExact>(a, p, grid, lds, s);
+    case kModeGrad:
+      return launch_mode<kModeGrad, kBoysExact>(a, p, grid, lds, s);
     default:
       return hipErrorInvalidValue;
   }

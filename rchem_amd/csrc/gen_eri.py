@@ -277,32 +277,42 @@ def tu_source(la, lb, lc, ld):
 
 
 def write_programs(path, lmax):
-    """Operation tables of the wave-cooperative kernel for every class."""
-    from gen_eri_prog import hrr_program, vrr_program
+    """Operation tables of the wave-cooperative kernel: the integrals of every
+    class, and the gradient's derivative integrals where they fit its LDS."""
+    from gen_eri_prog import eri_program, grad_program
 
     def arr(ctype, name, vals, per_line=8):
+        vals = vals or ["0"]
         body = []
         for i in range(0, len(vals), per_line):
             body.append("    " + ", ".join(vals[i:i + per_line]) + ",")
         return [f"const {ctype} {name}[] = {{"] + body + ["};"]
 
+    def words(rows):
+        return [f"{x}u" for r in rows for x in r]
+
     out = ["// Generated by gen_eri.py -- do not edit.", ""]
     entries = []
     for (la, lb, lc, ld) in classes(lmax):
-        key = f"{la}{lb}{lc}{ld}"
-        v, h = vrr_program(la, lb, lc, ld), hrr_program(la, lb, lc, ld)
-        words = lambda rows: [f"{x}u" for r in rows for x in r]
-        out += arr("uint32_t", f"kProgV_{key}", words(v["ops"]) or ["0u"])
-        out += arr("uint32_t", f"kProgVG_{key}", words(v["groups"]) or ["0u"])
-        out += arr("uint32_t", f"kProgH_{key}", words(h["ops"]) or ["0u"])
-        out += arr("uint32_t", f"kProgHG_{key}", words(h["groups"]) or ["0u"])
-        out += arr("int", f"kProgOut_{key}", [str(sl) for sl, _ in h["out"]])
-        out += arr("double", f"kProgNorm_{key}", [repr(nm) for _, nm in h["out"]], 4)
-        entries.append(f"    {{{la}, {lb}, {lc}, {ld}, {v['L']}, {v['nacc']}, {v['nslots']}, "
-                       f"{h['nslots']}, {len(h['out'])}, {len(v['ops'])}, {len(v['groups'])}, "
-                       f"{len(h['ops'])}, {len(h['groups'])}, kProgV_{key}, kProgVG_{key}, "
-                       f"kProgH_{key}, kProgHG_{key}, kProgOut_{key}, kProgNorm_{key}}},")
-        out.append("")
+        for kind, fn in (("E", eri_program), ("G", grad_program)):
+            pr = fn(la, lb, lc, ld)
+            if pr["P"] == 0:
+                continue  # does not fit a workgroup's LDS
+            key = f"{kind}{la}{lb}{lc}{ld}"
+            out += arr("uint32_t", f"kProgV_{key}", words(pr["vops"]))
+            out += arr("uint32_t", f"kProgVG_{key}", words(pr["vgroups"]))
+            out += arr("uint32_t", f"kProgA_{key}", words(pr["accs"]))
+            out += arr("uint32_t", f"kProgH_{key}", words(pr["hops"]))
+            out += arr("uint32_t", f"kProgHG_{key}", words(pr["hgroups"]))
+            out += arr("int", f"kProgOut_{key}", [str(x) for o in pr["outs"] for x in (o[0], o[1], o[2], 0)])
+            out += arr("double", f"kProgNorm_{key}", [repr(o[3]) for o in pr["outs"]], 4)
+            entries.append(
+                f"    {{{la}, {lb}, {lc}, {ld}, {int(kind == 'G')}, {pr['L']}, {pr['nacc']}, "
+                f"{pr['nv']}, {pr['nh']}, {len(pr['outs'])}, {len(pr['vops'])}, "
+                f"{len(pr['vgroups'])}, {len(pr['accs'])}, {len(pr['hops'])}, "
+                f"{len(pr['hgroups'])}, {pr['P']}, kProgV_{key}, kProgVG_{key}, kProgA_{key}, "
+                f"kProgH_{key}, kProgHG_{key}, kProgOut_{key}, kProgNorm_{key}}},")
+            out.append("")
     out += ["const HostProgram kHostPrograms[] = {"] + entries + ["};", ""]
     with open(path, "w") as fh:
         fh.write("\n".join(out))

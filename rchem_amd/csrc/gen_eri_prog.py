@@ -1,35 +1,45 @@
-"""Operation tables for the wave-cooperative ERI kernel (classes with L >= 5).
+"""Operation tables for the wave-cooperative ERI kernel.
 
 For one (la lb|lc ld) class this builds the same Head-Gordon--Pople
-recurrences as the straight-line generator (gen_eri.py), but as data: the
-vertical recurrence for one primitive quartet becomes a list of operations
-over slots of an LDS buffer, grouped into dependency levels and, inside a
-level, into groups of equal (side, axis) so that every recurrence
-coefficient is wave-uniform; the horizontal recurrence on the contracted
-accumulators becomes a second list of the same shape.
+recurrences as the straight-line generator (gen_eri.py), but as data that the
+kernel (eri_generic.hip) interprets:
 
-VRR op (four 32-bit words):
-  V[dst] = c0 V[s0] + c1 V[s1] + n1 o (V[s2] - r V[s3]) + n2 oo2ze V[s4]
+* VRR: for a batch of P primitive quartets, the vertical recurrence as a list
+  of operations over slots of an LDS buffer V[slot][q], grouped into
+  dependency levels and, inside a level, into groups of one (side, axis), so
+  that a lane -- which always serves the same primitive quartet q of the batch
+  -- loads the group's recurrence coefficients once:
+    V[dst] = c0 V[s0] + c1 V[s1] + n1 o (V[s2] - r V[s3]) + n2 oo2ze V[s4]
   with (c0, c1, o, r) = (PA_i, WP_i, 1/2zeta, rho/zeta) on the bra side and
-  (QC_i, WQ_i, 1/2eta, rho/eta) on the ket side; a target node [e0|f0]^(0)
-  also adds its value into accumulator `acc` (0xffff: none).
-  w0 = dst | s0 << 16, w1 = s1 | s2 << 16, w2 = s3 | s4 << 16,
-  w3 = acc | n1 << 16 | n2 << 20.
-HRR op (two 32-bit words): H[dst] = H[s0] + X_i H[s1], X = AB (bra) or CD (ket);
+  (QC_i, WQ_i, 1/2eta, rho/eta) on the ket side.
+    w0 = dst | s0 << 16, w1 = s1 | s2 << 16, w2 = s3 | s4 << 16, w3 = n1 | n2 << 4.
+  Slot 0 holds zero, slots 1..L+1 hold G_0..G_L; target slots [e0|f0]^(0) stay
+  live until the end of the batch, other slots are reused once their last
+  reader's level has passed.
+* Accumulation: (slot | set << 16, acc) -- acc[acc] += sum_q w_set(q) V[slot][q],
+  with w_0 = 1 and, for the gradient, w_1 = 2 alpha_a, w_2 = 2 alpha_b,
+  w_3 = 2 alpha_c of each primitive quartet.
+* HRR on the contracted accumulators (the first slots of a buffer H), the same
+  level/group layout: H[dst] = H[s0] + X_i H[s1], X = AB (bra) or CD (ket);
   w0 = dst | s0 << 16, w1 = s1.
-Group record (four words): op_begin, op_end, side | axis << 1 | level_end << 3, 0.
-The same recurrences as the reference's chgp.c:44-135 and chgp.c:412-586.
+* Outputs: value_k = norm_k (H[plus_k] - n_k H[minus_k]) -- the integrals of the
+  class (n_k = 0), or for the gradient the derivative integrals
+  d/dA_i (ab|cd) = (a+1_i b|cd)_{2 alpha_a} - a_i (a-1_i b|cd), and likewise for
+  B and C, in the order (centre A, B, C) x (x, y, z) x components.
+Group record: (op_begin, op_end, side | axis << 1 | level_end << 3, 0).
+The recurrences are those of the reference's chgp.c:44-135 and chgp.c:412-586.
 """
 from gen_eri import cart, comp_norm, dec, inc
 
 NONE = 0xFFFF
+LDS_BUDGET = 48 * 1024   # bytes of LDS per one-wave workgroup
+PMAX = 32                # primitive quartets per batch at most
+NREC_FIXED = 20          # record fields before G_0 (see eri_generic.hip)
 
 
-def _vrr_graph(la, lb, lc, ld):
-    """Nodes (e, f, m) -> (side, axis, srcs[5], n1, n2), built with the same
-    axis choices as the straight-line generator."""
-    E = [t for l in range(la, la + lb + 1) for t in cart(l)]
-    F = [t for l in range(lc, lc + ld + 1) for t in cart(l)]
+def _build_vrr_nodes(targets):
+    """Nodes (e, f, m) -> (side, axis, srcs[5], n1, n2) or None for G_m, with
+    the straight-line generator's axis choices."""
     nodes = {}
 
     def node(e, f, m):
@@ -38,7 +48,7 @@ def _vrr_graph(la, lb, lc, ld):
             return key
         if sum(f) == 0:
             if sum(e) == 0:
-                nodes[key] = None  # G_m
+                nodes[key] = None
                 return key
             i = min((k for k in range(3) if e[k] > 0), key=lambda k: (e[k] >= 2, k))
             e1 = dec(e, i)
@@ -63,24 +73,18 @@ def _vrr_graph(la, lb, lc, ld):
             nodes[key] = (1, i, s, n1, n2)
         return key
 
-    acc_of = {}
-    for ie, e in enumerate(E):
-        for jf, f in enumerate(F):
-            acc_of[node(e, f, 0)] = ie * len(F) + jf
-    return E, F, nodes, acc_of
+    for (e, f) in targets:
+        node(e, f, 0)
+    return nodes
 
 
-def _levels(nodes):
+def _levels(nodes, base):
     lev = {}
 
     def level(k):
-        if k in lev:
-            return lev[k]
-        d = nodes[k]
-        if d is None:
-            lev[k] = 0
-        else:
-            lev[k] = 1 + max(level(s) for s in d[2] if s is not None)
+        if k not in lev:
+            d = base(k)
+            lev[k] = 0 if d is None else 1 + max(level(s) for s in d if s is not None)
         return lev[k]
 
     for k in nodes:
@@ -88,140 +92,127 @@ def _levels(nodes):
     return lev
 
 
-def vrr_program(la, lb, lc, ld):
-    L = la + lb + lc + ld
-    E, F, nodes, acc_of = _vrr_graph(la, lb, lc, ld)
-    lev = _levels(nodes)
+def vrr_program(L, targets):
+    """VRR ops for one primitive quartet building [e0|f0]^(0) for every (e, f)
+    in `targets` (total angular momentum up to L)."""
+    nodes = _build_vrr_nodes(targets)
+    lev = _levels(nodes, lambda k: None if nodes[k] is None else nodes[k][2])
     nlev = max(lev.values())
+    INF = 1 << 30
     last_use = {}
     for k, d in nodes.items():
-        if d is None:
-            continue
-        for s in d[2]:
-            if s is not None:
-                last_use[s] = max(last_use.get(s, 0), lev[k])
-    # slots: 0 holds zero, 1..L+1 hold G_0..G_L (loaded per primitive quartet)
-    ZERO = 0
-    slot = {}
-    for k, d in nodes.items():
-        if d is None:
-            slot[k] = 1 + k[2]
-    nfixed = L + 2
-    free = []
-    nslots = nfixed
-    live = {}  # slot -> node
-    ops_by_level = [[] for _ in range(nlev + 1)]
-    for k in sorted((k for k in nodes if nodes[k] is not None), key=lambda k: lev[k]):
-        ops_by_level[lev[k]].append(k)
-    groups, ops = [], []
+        if d is not None:
+            for s in d[2]:
+                if s is not None:
+                    last_use[s] = max(last_use.get(s, 0), lev[k])
+    for (e, f) in targets:
+        last_use[(e, f, 0)] = INF
+    slot = {k: 1 + k[2] for k, d in nodes.items() if d is None}
+    assert max(slot.values()) <= L + 1
+    nslots = L + 2
+    free, live = [], {}
+    ops, groups = [], []
+    by_level = {}
+    for k in nodes:
+        if nodes[k] is not None:
+            by_level.setdefault(lev[k], []).append(k)
     for lv in range(1, nlev + 1):
-        # release slots whose last reader is an earlier level
         for s_, k in list(live.items()):
             if last_use.get(k, 0) < lv:
                 free.append(s_)
                 del live[s_]
         free.sort(reverse=True)
-        lvl_ops = ops_by_level[lv]
-        for k in lvl_ops:
+        lvl = sorted(by_level.get(lv, []), key=lambda k: (nodes[k][0], nodes[k][1], k))
+        for k in lvl:
             if free:
-                s_ = free.pop()
+                slot[k] = free.pop()
             else:
-                s_ = nslots
+                slot[k] = nslots
                 nslots += 1
-            slot[k] = s_
-        # a slot written at this level must not be one read at this level:
-        # released slots had their last reader at an earlier level, so this holds.
-        for k in lvl_ops:
             if k in last_use:
                 live[slot[k]] = k
-        keyed = sorted(lvl_ops, key=lambda k: (nodes[k][0], nodes[k][1]))
         gi = 0
-        while gi < len(keyed):
-            side, ax = nodes[keyed[gi]][0], nodes[keyed[gi]][1]
+        while gi < len(lvl):
+            side, ax = nodes[lvl[gi]][0], nodes[lvl[gi]][1]
             gb = len(ops)
-            while gi < len(keyed) and nodes[keyed[gi]][:2] == (side, ax):
-                k = keyed[gi]
+            while gi < len(lvl) and nodes[lvl[gi]][:2] == (side, ax):
+                k = lvl[gi]
                 _, _, s, n1, n2 = nodes[k]
-                idx = [slot[x] if x is not None else ZERO for x in s]
-                acc = acc_of.get(k, NONE)
+                idx = [slot[x] if x is not None else 0 for x in s]
                 ops.append((slot[k] | idx[0] << 16, idx[1] | idx[2] << 16,
-                            idx[3] | idx[4] << 16, acc | n1 << 16 | n2 << 20))
+                            idx[3] | idx[4] << 16, n1 | n2 << 4))
                 gi += 1
             groups.append([gb, len(ops), side | ax << 1, 0])
-        groups[-1][2] |= 1 << 3  # level end
-    assert nslots < NONE and len(E) * len(F) < NONE
-    # G slots that are themselves targets (the (ss|ss) class only) never occur for L >= 1
-    return dict(L=L, nacc=len(E) * len(F), nslots=nslots, ops=ops, groups=groups,
-                nlevels=nlev, E=E, F=F)
+        if groups:
+            groups[-1][2] |= 1 << 3
+    assert nslots < NONE
+    return dict(ops=ops, groups=groups, nslots=nslots,
+                target_slot={(e, f): slot[(e, f, 0)] for (e, f) in targets})
 
 
-def hrr_program(la, lb, lc, ld):
-    E = [t for l in range(la, la + lb + 1) for t in cart(l)]
-    F = [t for l in range(lc, lc + ld + 1) for t in cart(l)]
-    acc_index = {(e, f): ie * len(F) + jf for ie, e in enumerate(E) for jf, f in enumerate(F)}
-    nodes = {}  # key -> (side, axis, s0, s1) or ("acc", index)
+def _hrr(accsets, outs_spec):
+    """HRR nodes over accumulator sets.  accsets[s] = (E, F, base) with the
+    contracted [e0|f0] of set s at accumulator base + ie * len(F) + jf.
+    outs_spec: list of (plus, minus, n, norm), plus/minus = (set, a, b, c, d)
+    or None.  Returns ops, groups, nslots, outs (slot_plus, slot_minus, n, norm)."""
+    acc_index = {}
+    for s, (E, F, base) in enumerate(accsets):
+        for ie, e in enumerate(E):
+            for jf, f in enumerate(F):
+                acc_index[(s, e, f)] = base + ie * len(F) + jf
+    nacc = sum(len(E) * len(F) for E, F, _ in accsets)
+    nodes = {}
 
-    def bra(a, b, f):
-        key = ("b", a, b, f)
+    def bra(s, a, b, f):
+        key = ("b", s, a, b, f)
         if key not in nodes:
             if sum(b) == 0:
-                nodes[key] = ("acc", acc_index[(a, f)])
+                nodes[key] = ("acc", acc_index[(s, a, f)])
             else:
                 i = next(k for k in range(3) if b[k] > 0)
                 b1 = dec(b, i)
-                nodes[key] = (0, i, bra(inc(a, i), b1, f), bra(a, b1, f))
+                nodes[key] = (0, i, bra(s, inc(a, i), b1, f), bra(s, a, b1, f))
         return key
 
-    def ket(a, b, c, d):
+    def ket(s, a, b, c, d):
         if sum(d) == 0:
-            return bra(a, b, c)
-        key = ("k", a, b, c, d)
+            return bra(s, a, b, c)
+        key = ("k", s, a, b, c, d)
         if key not in nodes:
             i = next(k for k in range(3) if d[k] > 0)
             d1 = dec(d, i)
-            nodes[key] = (1, i, ket(a, b, inc(c, i), d1), ket(a, b, c, d1))
+            nodes[key] = (1, i, ket(s, a, b, inc(c, i), d1), ket(s, a, b, c, d1))
         return key
 
-    outs = []
-    for a in cart(la):
-        for b in cart(lb):
-            for c in cart(lc):
-                for d in cart(ld):
-                    k = ket(a, b, c, d)
-                    outs.append((k, comp_norm(a) * comp_norm(b) * comp_norm(c) * comp_norm(d)))
-    lev = {}
-
-    def level(k):
-        if k not in lev:
-            d = nodes[k]
-            lev[k] = 0 if d[0] == "acc" else 1 + max(level(d[2]), level(d[3]))
-        return lev[k]
-
-    for k in nodes:
-        level(k)
-    nacc = len(E) * len(F)
+    out_nodes = []
+    for plus, minus, n, nrm in outs_spec:
+        kp = ket(*plus)
+        km = ket(*minus) if minus is not None else None
+        out_nodes.append((kp, km, n, nrm))
+    lev = _levels(nodes, lambda k: None if nodes[k][0] == "acc" else (nodes[k][2], nodes[k][3]))
     INF = 1 << 30
     last_use = {}
     for k, d in nodes.items():
         if d[0] != "acc":
             for s_ in (d[2], d[3]):
                 last_use[s_] = max(last_use.get(s_, 0), lev[k])
-    for k, _ in outs:
-        last_use[k] = INF
-    # accumulators occupy slots [0, nacc); everything else reuses released slots
+    for kp, km, _, _ in out_nodes:
+        last_use[kp] = INF
+        if km is not None:
+            last_use[km] = INF
     slot = {k: d[1] for k, d in nodes.items() if d[0] == "acc"}
     live = {slot[k]: k for k in slot if k in last_use}
     free = sorted(set(range(nacc)) - set(live), reverse=True)
     nslots = nacc
     ops, groups = [], []
-    nlev = max(lev.values())
+    nlev = max(lev.values()) if lev else 0
     for lv in range(1, nlev + 1):
         for s_, k in list(live.items()):
             if last_use[k] < lv:
                 free.append(s_)
                 del live[s_]
         free.sort(reverse=True)
-        lvl = sorted((k for k in nodes if lev[k] == lv), key=lambda k: (nodes[k][0], nodes[k][1]))
+        lvl = sorted((k for k in nodes if lev[k] == lv), key=lambda k: (nodes[k][0], nodes[k][1], str(k)))
         for k in lvl:
             if free:
                 slot[k] = free.pop()
@@ -240,16 +231,101 @@ def hrr_program(la, lb, lc, ld):
             groups.append([gb, len(ops), side | ax << 1, 0])
         groups[-1][2] |= 1 << 3
     assert nslots < NONE
-    return dict(nslots=nslots, ops=ops, groups=groups,
-                out=[(slot[k], nrm) for k, nrm in outs], nlevels=nlev)
+    outs = [(slot[kp], slot[km] if km is not None else slot[kp], n, nrm)
+            for kp, km, n, nrm in out_nodes]
+    return dict(ops=ops, groups=groups, nslots=nslots, outs=outs, nacc=nacc)
+
+
+def _range(l0, l1):
+    return [t for l in range(max(l0, 0), l1 + 1) for t in cart(l)]
+
+
+def lds_bytes(L, nacc, nv, nh, p):
+    return 8 * ((NREC_FIXED + L + 1) * p + max(nacc + nv * p, nh))
+
+
+def _choose_p(L, nacc, nv, nh):
+    """Largest batch within the LDS budget; 0 when not even one primitive
+    quartet fits a workgroup's 64 KB."""
+    p = PMAX
+    while p > 1 and lds_bytes(L, nacc, nv, nh, p) > LDS_BUDGET:
+        p //= 2
+    return p if lds_bytes(L, nacc, nv, nh, p) <= 64 * 1024 else 0
+
+
+def _assemble(L, accsets, vrr_targets, outs_spec):
+    v = vrr_program(L, vrr_targets)
+    acc_list = []
+    for s, (E, F, base) in enumerate(accsets):
+        for ie, e in enumerate(E):
+            for jf, f in enumerate(F):
+                acc_list.append((v["target_slot"][(e, f)] | s << 16, base + ie * len(F) + jf))
+    h = _hrr(accsets, outs_spec)
+    nacc = h["nacc"]
+    P = _choose_p(L, nacc, v["nslots"], h["nslots"])
+    return dict(L=L, nacc=nacc, nv=v["nslots"], nh=h["nslots"], P=P, vops=v["ops"],
+                vgroups=v["groups"], accs=acc_list, hops=h["ops"], hgroups=h["groups"],
+                outs=h["outs"])
+
+
+def eri_program(la, lb, lc, ld):
+    """The integrals (ab|cd) of the class, canonical component order."""
+    E, F = _range(la, la + lb), _range(lc, lc + ld)
+    outs = []
+    for a in cart(la):
+        for b in cart(lb):
+            for c in cart(lc):
+                for d in cart(ld):
+                    nrm = comp_norm(a) * comp_norm(b) * comp_norm(c) * comp_norm(d)
+                    outs.append(((0, a, b, c, d), None, 0, nrm))
+    return _assemble(la + lb + lc + ld, [(E, F, 0)], [(e, f) for e in E for f in F], outs)
+
+
+def grad_program(la, lb, lc, ld):
+    """Derivatives of (ab|cd) with respect to the centres A, B and C (D follows
+    from translational invariance), outputs ordered (A, B, C) x (x, y, z) x
+    canonical components.  Sets: 0 unweighted, 1 by 2 alpha_a, 2 by 2 alpha_b,
+    3 by 2 alpha_c."""
+    sets = [
+        (_range(la - 1, la + lb), _range(lc - 1, lc + ld)),   # minus terms
+        (_range(la + 1, la + lb + 1), _range(lc, lc + ld)),   # A plus
+        (_range(la, la + lb + 1), _range(lc, lc + ld)),       # B plus
+        (_range(la, la + lb), _range(lc + 1, lc + ld + 1)),   # C plus
+    ]
+    accsets, base = [], 0
+    for E, F in sets:
+        accsets.append((E, F, base))
+        base += len(E) * len(F)
+    targets = sorted({(e, f) for E, F in sets for e in E for f in F})
+    outs = []
+    for centre in range(3):
+        for i in range(3):
+            for a in cart(la):
+                for b in cart(lb):
+                    for c in cart(lc):
+                        for d in cart(ld):
+                            nrm = comp_norm(a) * comp_norm(b) * comp_norm(c) * comp_norm(d)
+                            t = [a, b, c, d]
+                            plus = list(t)
+                            plus[centre] = inc(t[centre], i)
+                            n = t[centre][i]
+                            minus = None
+                            if n > 0:
+                                minus = list(t)
+                                minus[centre] = dec(t[centre], i)
+                                minus = (0, *minus)
+                            outs.append(((1 + centre, *plus), minus, n, nrm))
+    return _assemble(la + lb + lc + ld + 1, accsets, targets, outs)
 
 
 if __name__ == "__main__":
     import sys
-    for key in sys.argv[1:] or ["2111", "2120", "2121", "2210", "2211", "2220", "2221", "2222"]:
+    keys = sys.argv[1:] or ["0000", "1111", "2111", "2121", "2221", "2222"]
+    for key in keys:
         la, lb, lc, ld = (int(x) for x in key)
-        v = vrr_program(la, lb, lc, ld)
-        h = hrr_program(la, lb, lc, ld)
-        print(key, "vrr ops", len(v["ops"]), "slots", v["nslots"], "groups", len(v["groups"]),
-              "levels", v["nlevels"], "nacc", v["nacc"], "| hrr ops", len(h["ops"]), "slots",
-              h["nslots"], "groups", len(h["groups"]), "nout", len(h["out"]))
+        for name, fn in (("eri", eri_program), ("grad", grad_program)):
+            p = fn(la, lb, lc, ld)
+            lds = lds_bytes(p["L"], p["nacc"], p["nv"], p["nh"], p["P"])
+            print(f"{key} {name}: L {p['L']} vops {len(p['vops'])} vgroups {len(p['vgroups'])} "
+                  f"nv {p['nv']} nacc {p['nacc']} accs {len(p['accs'])} hops {len(p['hops'])} "
+                  f"nh {p['nh']} outs {len(p['outs'])} P {p['P']} lds {lds}")

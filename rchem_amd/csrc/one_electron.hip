@@ -206,6 +206,217 @@ __global__ __launch_bounds__(128) void one_electron_kernel(const OneElArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Nuclear gradient of the one-electron energy terms
+//   sum_{mu nu} [2 D (dT + dV) - 2 W dS]
+// with W the energy-weighted density.  A basis function's derivative with
+// respect to its centre is 2 alpha |a + 1_k> - a_k |a - 1_k> (same
+// normalisation as the undifferentiated function).  S and T are invariant to
+// translating both centres (d/dB = -d/dA); V_C adds the operator term
+// d/dC = -(d/dA + d/dB).  One thread per (shell pair a >= b, nucleus C),
+// nucleus fastest; the C = 0 thread also adds the S and T terms.
+struct OneElGradArgs {
+  const ShellDev* shells;
+  const double* exps;
+  const double* ccoef;
+  const int2* pairs;
+  int npairs;
+  int nbf;
+  int natom;
+  const double* atom_xyz;
+  const double* atom_z;
+  const double* boys_table;
+  const int* shell_atom;
+  const double* D;
+  const double* W;
+  double* grad;
+};
+
+constexpr int kIg = kLmax1 + 2;      // i range for a + 1
+constexpr int kTg = 2 * kLmax1 + 4;  // Hermite t range for (a + 1, b + 2)
+constexpr int kRg = 2 * kLmax1 + 2;  // R_tuv index range for L = la + lb + 1
+
+__device__ __forceinline__ void hermite_eg(int imax, int jmax, double pa, double pb, double oo2p,
+                                           double E[kIg][kE][kTg]) {
+  for (int i = 0; i <= imax; ++i)
+    for (int j = 0; j <= jmax; ++j)
+      for (int t = 0; t < kTg; ++t) E[i][j][t] = 0.0;
+  E[0][0][0] = 1.0;
+  for (int i = 0; i <= imax; ++i) {
+    if (i > 0) {
+      for (int t = 0; t <= i; ++t) {
+        double v = pa * E[i - 1][0][t];
+        if (t > 0) v += oo2p * E[i - 1][0][t - 1];
+        if (t + 1 <= i - 1) v += (t + 1) * E[i - 1][0][t + 1];
+        E[i][0][t] = v;
+      }
+    }
+    for (int j = 1; j <= jmax; ++j)
+      for (int t = 0; t <= i + j; ++t) {
+        double v = pb * E[i][j - 1][t];
+        if (t > 0) v += oo2p * E[i][j - 1][t - 1];
+        if (t + 1 <= i + j - 1) v += (t + 1) * E[i][j - 1][t + 1];
+        E[i][j][t] = v;
+      }
+  }
+}
+
+__device__ __forceinline__ double ovl_g(const double (&E)[3][kIg][kE][kTg], const int* A,
+                                        const int* B) {
+  return E[0][A[0]][B[0]][0] * E[1][A[1]][B[1]][0] * E[2][A[2]][B[2]][0];
+}
+
+// kinetic energy integral over the Hermite overlaps (cints.c:152-177 form)
+__device__ __forceinline__ double kin_g(const double (&E)[3][kIg][kE][kTg], const int* A,
+                                        const int* B, double be) {
+  const double s0 = ovl_g(E, A, B);
+  double t1 = 0.0, t2 = 0.0;
+  for (int x = 0; x < 3; ++x) {
+    int Bp[3] = {B[0], B[1], B[2]};
+    Bp[x] += 2;
+    t1 += ovl_g(E, A, Bp);
+    if (B[x] >= 2) {
+      int Bm[3] = {B[0], B[1], B[2]};
+      Bm[x] -= 2;
+      t2 += B[x] * (B[x] - 1) * ovl_g(E, A, Bm);
+    }
+  }
+  const int lb = B[0] + B[1] + B[2];
+  return be * (2 * lb + 3) * s0 - 2.0 * be * be * t1 - 0.5 * t2;
+}
+
+__device__ __forceinline__ double nuc_g(const double (&E)[3][kIg][kE][kTg],
+                                        const double (&R)[kRg][kRg][kRg], const int* A,
+                                        const int* B) {
+  double sum = 0.0;
+  for (int t = 0; t <= A[0] + B[0]; ++t)
+    for (int u = 0; u <= A[1] + B[1]; ++u)
+      for (int v = 0; v <= A[2] + B[2]; ++v)
+        sum += E[0][A[0]][B[0]][t] * E[1][A[1]][B[1]][u] * E[2][A[2]][B[2]][v] * R[t][u][v];
+  return sum;
+}
+
+__global__ __launch_bounds__(128) void one_electron_grad_kernel(const OneElGradArgs a) {
+  const long tid = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (tid >= static_cast<long>(a.npairs) * a.natom) return;
+  const int pid = static_cast<int>(tid / a.natom), c = static_cast<int>(tid % a.natom);
+  const int2 pr = a.pairs[pid];
+  const ShellDev sa = a.shells[pr.x], sb = a.shells[pr.y];
+  const int la = sa.l, lb = sb.l;
+  const int na = ncart1(la), nb = ncart1(lb);
+  const int atA = a.shell_atom[pr.x], atB = a.shell_atom[pr.y];
+  const double f = pr.x != pr.y ? 2.0 : 1.0;  // the (b, a) block of the sum
+  int ca[6][3], cb[6][3];
+  for (int k = 0; k < na; ++k) cart_comp(la, k, ca[k]);
+  for (int k = 0; k < nb; ++k) cart_comp(lb, k, cb[k]);
+  const double AB2 = (sa.xyz[0] - sb.xyz[0]) * (sa.xyz[0] - sb.xyz[0]) +
+                     (sa.xyz[1] - sb.xyz[1]) * (sa.xyz[1] - sb.xyz[1]) +
+                     (sa.xyz[2] - sb.xyz[2]) * (sa.xyz[2] - sb.xyz[2]);
+  const double Cx = a.atom_xyz[3 * c], Cy = a.atom_xyz[3 * c + 1], Cz = a.atom_xyz[3 * c + 2];
+  const size_t n = static_cast<size_t>(a.nbf);
+  const int L = la + lb + 1;
+  double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0}, gC[3] = {0.0, 0.0, 0.0};
+  double E[3][kIg][kE][kTg];
+  double R[kRg][kRg][kRg], Rn[kRg][kRg][kRg];
+  double F[kRg];
+  for (int ip = 0; ip < sa.nprim; ++ip)
+    for (int jp = 0; jp < sb.nprim; ++jp) {
+      const double al = a.exps[sa.p0 + ip], be = a.exps[sb.p0 + jp];
+      const double p = al + be, oop = 1.0 / p;
+      const double cc = a.ccoef[sa.p0 + ip] * a.ccoef[sb.p0 + jp];
+      const double kab = exp(-al * be * oop * AB2);
+      double P[3];
+      for (int x = 0; x < 3; ++x) {
+        P[x] = (al * sa.xyz[x] + be * sb.xyz[x]) * oop;
+        hermite_eg(la + 1, lb + 2, P[x] - sa.xyz[x], P[x] - sb.xyz[x], 0.5 * oop, E[x]);
+      }
+      // Hermite Coulomb integrals R_tuv(p, P - C), level by level (R_{n} from R_{n+1})
+      const double PC[3] = {P[0] - Cx, P[1] - Cy, P[2] - Cz};
+      const double T = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
+      boys_exact_rt(L, T, a.boys_table, F);
+      double m2p = 1.0;
+      for (int k = 0; k < L; ++k) m2p *= -2.0 * p;
+      Rn[0][0][0] = m2p * F[L];
+      for (int lev = L - 1; lev >= 0; --lev) {
+        m2p /= -2.0 * p;
+        const int top = L - lev;
+        for (int tt = 0; tt <= top; ++tt)
+          for (int uu = 0; uu <= top - tt; ++uu)
+            for (int vv = 0; vv <= top - tt - uu; ++vv) {
+              double r;
+              if (tt + uu + vv == 0) {
+                r = m2p * F[lev];
+              } else if (tt > 0) {
+                r = PC[0] * Rn[tt - 1][uu][vv];
+                if (tt > 1) r += (tt - 1) * Rn[tt - 2][uu][vv];
+              } else if (uu > 0) {
+                r = PC[1] * Rn[tt][uu - 1][vv];
+                if (uu > 1) r += (uu - 1) * Rn[tt][uu - 2][vv];
+              } else {
+                r = PC[2] * Rn[tt][uu][vv - 1];
+                if (vv > 1) r += (vv - 1) * Rn[tt][uu][vv - 2];
+              }
+              R[tt][uu][vv] = r;
+            }
+        for (int tt = 0; tt <= top; ++tt)
+          for (int uu = 0; uu <= top - tt; ++uu)
+            for (int vv = 0; vv <= top - tt - uu; ++vv) Rn[tt][uu][vv] = R[tt][uu][vv];
+      }
+      if (L == 0) R[0][0][0] = Rn[0][0][0];
+      const double sq = sqrt(3.14159265358979323846 * oop);
+      const double pre_s = cc * kab * sq * sq * sq;
+      const double pre_v = -a.atom_z[c] * cc * kab * 2.0 * 3.14159265358979323846 * oop;
+      for (int i = 0; i < na; ++i)
+        for (int j = 0; j < nb; ++j) {
+          const int* A = ca[i];
+          const int* B = cb[j];
+          const double w = f * comp_factor(A) * comp_factor(B);
+          const size_t mu = sa.f0 + i, nu = sb.f0 + j;
+          const double d2 = 2.0 * a.D[mu * n + nu], w2 = 2.0 * a.W[mu * n + nu];
+          for (int k = 0; k < 3; ++k) {
+            int Ap[3] = {A[0], A[1], A[2]}, Am[3] = {A[0], A[1], A[2]};
+            int Bp[3] = {B[0], B[1], B[2]}, Bm[3] = {B[0], B[1], B[2]};
+            Ap[k] += 1;
+            Am[k] -= 1;
+            Bp[k] += 1;
+            Bm[k] -= 1;
+            double dVA = 2.0 * al * nuc_g(E, R, Ap, B);
+            if (A[k] > 0) dVA -= A[k] * nuc_g(E, R, Am, B);
+            double dVB = 2.0 * be * nuc_g(E, R, A, Bp);
+            if (B[k] > 0) dVB -= B[k] * nuc_g(E, R, A, Bm);
+            const double vA = w * d2 * pre_v * dVA, vB = w * d2 * pre_v * dVB;
+            gA[k] += vA;
+            gB[k] += vB;
+            gC[k] -= vA + vB;
+            if (c == 0) {
+              double dS = 2.0 * al * ovl_g(E, Ap, B);
+              double dT = 2.0 * al * kin_g(E, Ap, B, be);
+              if (A[k] > 0) {
+                dS -= A[k] * ovl_g(E, Am, B);
+                dT -= A[k] * kin_g(E, Am, B, be);
+              }
+              const double st = w * pre_s * (d2 * dT - w2 * dS);
+              gA[k] += st;
+              gB[k] -= st;
+            }
+          }
+        }
+    }
+  for (int k = 0; k < 3; ++k) {
+    unsafeAtomicAdd(&a.grad[3 * atA + k], gA[k]);
+    unsafeAtomicAdd(&a.grad[3 * atB + k], gB[k]);
+    unsafeAtomicAdd(&a.grad[3 * c + k], gC[k]);
+  }
+}
+
+hipError_t one_electron_grad_launch(const OneElGradArgs& a, hipStream_t s) {
+  const long total = static_cast<long>(a.npairs) * a.natom;
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(one_electron_grad_kernel, dim3(static_cast<unsigned>((total + 127) / 128)),
+                     dim3(128), 0, s, a);
+  return hipGetLastError();
+}
+
 hipError_t one_electron_launch(const OneElArgs& a, hipStream_t s) {
   if (a.npairs <= 0) return hipSuccess;
   hipLaunchKernelGGL(one_electron_kernel, dim3((a.npairs + 127) / 128), dim3(128), 0, s, a);

@@ -35,16 +35,19 @@ hipError_t plan_launch(const PlanArgs& a, const PlanBuffers& b, hipStream_t s);
 EriLaunchFn eri_jk_lds_launcher(int la, int lb, int lc, int ld, int bm, int fp32);
 hipError_t eri_generic_launch(const EriArgs& a, const GenericProgram& p, int mode, int bm,
                               int blocks, hipStream_t s);
+
 size_t eri_generic_lds_bytes(const GenericProgram& p);
 
-// Operation tables of the wave-cooperative kernel, one per class (host copies).
+// Operation tables of the wave-cooperative kernel (host copies): the integrals
+// of every class, and the derivative integrals of the gradient (grad = 1).
 struct HostProgram {
-  int la, lb, lc, ld, L, nacc, nv, nh, nout, nvops, nvgroups, nhops, nhgroups;
+  int la, lb, lc, ld, grad, L, nacc, nv, nh, nout, nvops, nvgroups, naccs, nhops, nhgroups, P;
   const uint32_t* vops;
   const uint32_t* vgroups;
+  const uint32_t* accs;
   const uint32_t* hops;
   const uint32_t* hgroups;
-  const int* out_src;
+  const int* outs;
   const double* out_norm;
 };
 #include "generated/eri_programs.inc"
@@ -59,6 +62,7 @@ hipError_t fock_launch(const double* H, const double* J, const double* K, double
                        hipStream_t s);
 hipError_t energy_launch(const double* D, const double* H, const double* F, double* out,
                          size_t n2, hipStream_t s);
+hipError_t antisym_launch(const double* A, double* E, int n, hipStream_t s);
 hipError_t scale_columns_launch(const double* U, const double* w, double* out, int n,
                                 hipStream_t s);
 
@@ -81,6 +85,23 @@ struct OneElArgs {
   double* T;
   double* V;
 };
+struct OneElGradArgs {
+  const ShellDev* shells;
+  const double* exps;
+  const double* ccoef;
+  const int2* pairs;
+  int npairs;
+  int nbf;
+  int natom;
+  const double* atom_xyz;
+  const double* atom_z;
+  const double* boys_table;
+  const int* shell_atom;
+  const double* D;
+  const double* W;
+  double* grad;
+};
+hipError_t one_electron_grad_launch(const OneElGradArgs& a, hipStream_t s);
 hipError_t one_electron_launch(const OneElArgs& a, hipStream_t s);
 }  // namespace rchem
 
@@ -183,17 +204,29 @@ struct rchem_ctx {
   std::vector<Bucket> buckets[kNumPairClasses];
   int class_begin[kNumPairClasses + 1] = {0};
   DevBuf pairs_d, prims_d;
+  DevBuf alpha_d;  // exponent of the first shell of every primitive pair (prims layout)
 
   // tiles for a threshold
   double tiles_tau = -1.0;
   std::vector<Tile> tiles;
-  std::vector<QClass> qclasses;
+  std::vector<QClass> qclasses;   // device-planner classes
+  std::vector<QClass> hclasses;   // classes of the host-built tile list (tensor, gradient)
   DevBuf tiles_d;
   rchem_jk_stats planned{};  // statistics of the planned tile list
 
   // device planner
   double plan_tau = -1.0;
+  // The plan is cut into passes of at most kPassTiles tiles: a pass is a list
+  // of segments (a class restricted to a window of bra pairs), planned and
+  // launched together; every build runs the passes in order into the same
+  // J, K and Coulomb accumulators.  pclasses holds the segments of all passes.
+  struct Pass {
+    int seg_begin, nseg, tasks, units, cap;
+  };
   std::vector<PlanClass> pclasses;
+  std::vector<int> seg_class;  // segment -> qclasses index
+  std::vector<int> seg_cap;    // tiles reserved for the segment
+  std::vector<Pass> passes;
   int plan_tasks = 0, plan_cap = 0;
   size_t plan_scan_bytes = 0;
   int plan_units = 0;
@@ -207,6 +240,9 @@ struct rchem_ctx {
   DevBuf jp_d, jp_cls_d, jp_ncomp_d;
 
   DevBuf boys_d, boys_f_d;
+  // state of the last SCF: density, MO coefficients and orbital energies
+  // (column-major, as rocSOLVER leaves them), kept for the gradient
+  DevBuf scf_D, scf_C, scf_w, scf_Dold;
   DevBuf work_Jh, work_Kh, work_D, work_J, work_K;
   rchem_jk_stats last_stats{};
   bool last_mixed = false;
@@ -325,12 +361,14 @@ int finish_basis(rchem_ctx* ctx) {
 
 // Primitive pairs of shell pair (a, b) whose |K_ab| reaches eps (all of them
 // when eps <= 0), in (i of a, j of b) order.
-void make_prims(const rchem_ctx* ctx, int a, int b, double eps, std::vector<PrimRec>* out) {
+void make_prims(const rchem_ctx* ctx, int a, int b, double eps, std::vector<PrimRec>* out,
+                std::vector<double>* alpha = nullptr) {
   const Shell& sa = ctx->shells[a];
   const Shell& sb = ctx->shells[b];
   double AB2 = 0.0;
   for (int x = 0; x < 3; ++x) AB2 += (sa.xyz[x] - sb.xyz[x]) * (sa.xyz[x] - sb.xyz[x]);
   out->clear();
+  if (alpha) alpha->clear();
   for (int ip = 0; ip < sa.nprim; ++ip)
     for (int jp = 0; jp < sb.nprim; ++jp) {
       const double al = ctx->exps[sa.p0 + ip], be = ctx->exps[sb.p0 + jp];
@@ -345,6 +383,7 @@ void make_prims(const rchem_ctx* ctx, int a, int b, double eps, std::vector<Prim
       pr.oo2z = 0.5 / z;
       if (eps > 0.0 && std::fabs(pr.kab) < eps) continue;
       out->push_back(pr);
+      if (alpha) alpha->push_back(al);
     }
 }
 
@@ -373,6 +412,7 @@ int layout_pairs(rchem_ctx* ctx, const std::vector<PairHost>& sel, double prim_e
   ctx->pairQ.resize(v.size());
   ctx->pair_shells.resize(v.size());
   std::vector<PrimRec> prims;
+  std::vector<double> alphas, tmp_alpha;
   for (int c = 0; c < kNumPairClasses; ++c) ctx->buckets[c].clear();
   size_t i = 0;
   for (int c = 0; c <= kNumPairClasses; ++c) {
@@ -387,6 +427,7 @@ int layout_pairs(rchem_ctx* ctx, const std::vector<PairHost>& sel, double prim_e
     ctx->buckets[v[pos].cls].push_back(Bucket{static_cast<int>(pos), static_cast<int>(end), np});
     const size_t base = prims.size();
     prims.resize(base + static_cast<size_t>(n) * np);
+    alphas.resize(base + static_cast<size_t>(n) * np);
     for (int q = 0; q < n; ++q) {
       const PairHost& ph = v[pos + q];
       const Shell& sa = ctx->shells[ph.a];
@@ -404,12 +445,19 @@ int layout_pairs(rchem_ctx* ctx, const std::vector<PairHost>& sel, double prim_e
       r.flags = (ph.a != ph.b) ? 1 : 0;
       ctx->pairQ[pos + q] = ph.Q;
       ctx->pair_shells[pos + q] = std::make_pair(ph.a, ph.b);
-      make_prims(ctx, ph.a, ph.b, prim_eps, &tmp);
-      for (int k = 0; k < np; ++k) prims[base + static_cast<size_t>(k) * n + q] = tmp[k];
+      make_prims(ctx, ph.a, ph.b, prim_eps, &tmp, &tmp_alpha);
+      for (int k = 0; k < np; ++k) {
+        prims[base + static_cast<size_t>(k) * n + q] = tmp[k];
+        alphas[base + static_cast<size_t>(k) * n + q] = tmp_alpha[k];
+      }
     }
     pos = end;
   }
   ctx->nprims = prims.size();
+  HIPC(ctx->alpha_d.ensure(sizeof(double) * std::max<size_t>(alphas.size(), 1)));
+  if (!alphas.empty())
+    HIPC(hipMemcpyAsync(ctx->alpha_d.p, alphas.data(), sizeof(double) * alphas.size(),
+                        hipMemcpyHostToDevice, ctx->stream));
   HIPC(ctx->pairs_d.ensure(sizeof(PairRec) * std::max<size_t>(ctx->pairs.size(), 1)));
   HIPC(ctx->prims_d.ensure(sizeof(PrimRec) * std::max<size_t>(prims.size(), 1)));
   if (!ctx->pairs.empty())
@@ -439,21 +487,22 @@ struct ClassKernel {
 int upload_programs(rchem_ctx* ctx) {
   if (!ctx->gprog.empty()) return RCHEM_OK;
   auto al = [](size_t b) { return (b + 15) & ~size_t(15); };
+  auto room = [](int n, size_t w) { return w * static_cast<size_t>(std::max(n, 1)); };
   size_t total = 0;
   for (int i = 0; i < kNumHostPrograms; ++i) {
     const HostProgram& h = kHostPrograms[i];
-    total += al(16 * std::max(h.nvops, 1)) + al(16 * std::max(h.nvgroups, 1)) +
-             al(8 * std::max(h.nhops, 1)) + al(16 * std::max(h.nhgroups, 1)) + al(4 * h.nout) +
-             al(8 * h.nout);
+    total += al(room(h.nvops, 16)) + al(room(h.nvgroups, 16)) + al(room(h.naccs, 8)) +
+             al(room(h.nhops, 8)) + al(room(h.nhgroups, 16)) + al(room(h.nout, 16)) +
+             al(room(h.nout, 8));
   }
-  std::vector<unsigned char> host(total);
+  std::vector<unsigned char> host(total, 0);
   HIPC(ctx->gprog_d.ensure(total));
   std::vector<GenericProgram> progs(kNumHostPrograms);
   size_t off = 0;
-  auto put = [&](const void* src, size_t bytes, size_t room) {
-    std::memcpy(host.data() + off, src, bytes);
+  auto put = [&](const void* src, int n, size_t w) {
+    std::memcpy(host.data() + off, src, w * static_cast<size_t>(n));
     const void* dev = static_cast<const unsigned char*>(ctx->gprog_d.p) + off;
-    off += al(room);
+    off += al(room(n, w));
     return dev;
   };
   for (int i = 0; i < kNumHostPrograms; ++i) {
@@ -461,16 +510,33 @@ int upload_programs(rchem_ctx* ctx) {
     GenericProgram& g = progs[i];
     g.la = h.la; g.lb = h.lb; g.lc = h.lc; g.ld = h.ld;
     g.L = h.L; g.nacc = h.nacc; g.nv = h.nv; g.nh = h.nh; g.nout = h.nout;
-    g.nvgroups = h.nvgroups; g.nhgroups = h.nhgroups; g.pad = 0;
-    g.vops = static_cast<const uint4*>(put(h.vops, 16 * h.nvops, 16 * std::max(h.nvops, 1)));
-    g.vgroups = static_cast<const uint4*>(put(h.vgroups, 16 * h.nvgroups, 16 * std::max(h.nvgroups, 1)));
-    g.hops = static_cast<const uint2*>(put(h.hops, 8 * h.nhops, 8 * std::max(h.nhops, 1)));
-    g.hgroups = static_cast<const uint4*>(put(h.hgroups, 16 * h.nhgroups, 16 * std::max(h.nhgroups, 1)));
-    g.out_src = static_cast<const int*>(put(h.out_src, 4 * h.nout, 4 * h.nout));
-    g.out_norm = static_cast<const double*>(put(h.out_norm, 8 * h.nout, 8 * h.nout));
+    g.nvgroups = h.nvgroups; g.nhgroups = h.nhgroups; g.naccs = h.naccs; g.P = h.P;
+    g.vops = static_cast<const uint4*>(put(h.vops, h.nvops, 16));
+    g.vgroups = static_cast<const uint4*>(put(h.vgroups, h.nvgroups, 16));
+    g.accs = static_cast<const uint2*>(put(h.accs, h.naccs, 8));
+    g.hops = static_cast<const uint2*>(put(h.hops, h.nhops, 8));
+    g.hgroups = static_cast<const uint4*>(put(h.hgroups, h.nhgroups, 16));
+    g.outs = static_cast<const int4*>(put(h.outs, h.nout, 16));
+    g.out_norm = static_cast<const double*>(put(h.out_norm, h.nout, 8));
   }
   HIPC(hipMemcpy(ctx->gprog_d.p, host.data(), total, hipMemcpyHostToDevice));
   ctx->gprog = std::move(progs);
+  return RCHEM_OK;
+}
+
+// The wave-cooperative program of class (la lb|lc ld): integrals (grad = 0)
+// or derivative integrals (grad = 1); nullptr when the library has none.
+int find_program(rchem_ctx* ctx, int la, int lb, int lc, int ld, int grad,
+                 const GenericProgram** out) {
+  *out = nullptr;
+  TRY(upload_programs(ctx));
+  for (int i = 0; i < kNumHostPrograms; ++i) {
+    const HostProgram& h = kHostPrograms[i];
+    if (h.la == la && h.lb == lb && h.lc == lc && h.ld == ld && h.grad == grad) {
+      if (eri_generic_lds_bytes(ctx->gprog[i]) <= 64 * 1024) *out = &ctx->gprog[i];
+      return RCHEM_OK;
+    }
+  }
   return RCHEM_OK;
 }
 
@@ -486,15 +552,7 @@ int class_kernel(rchem_ctx* ctx, int la, int lb, int lc, int ld, int mode, int b
     if (out->fn) return RCHEM_OK;
   }
   if (la + lb + lc + ld > kBoysMaxM) return RCHEM_OK;
-  TRY(upload_programs(ctx));
-  for (int i = 0; i < kNumHostPrograms; ++i) {
-    const GenericProgram& g = ctx->gprog[i];
-    if (g.la == la && g.lb == lb && g.lc == lc && g.ld == ld) {
-      if (eri_generic_lds_bytes(g) <= 64 * 1024) out->gp = &g;
-      return RCHEM_OK;
-    }
-  }
-  return RCHEM_OK;
+  return find_program(ctx, la, lb, lc, ld, 0, &out->gp);
 }
 
 std::string class_name(int la, int lb, int lc, int ld) {
@@ -680,12 +738,12 @@ int prepare_host_tiles(rchem_ctx* ctx, double tau) {
   TRY(prepare_pairs(ctx, tau));
   if (ctx->tiles_tau == tau) return RCHEM_OK;
   ctx->tiles.clear();
-  ctx->qclasses.clear();
+  ctx->hclasses.clear();
   int curX = -1, curY = -1;
   QClass qc{};
   auto flush = [&]() {
     qc.ntiles = static_cast<int>(ctx->tiles.size()) - qc.tile_begin;
-    if (curX >= 0 && qc.ntiles > 0) ctx->qclasses.push_back(qc);
+    if (curX >= 0 && qc.ntiles > 0) ctx->hclasses.push_back(qc);
   };
   for_each_task(ctx, tau, [&](int X, int Y, int i, int, int lo, int hi) {
     if (X != curX || Y != curY) {
@@ -698,7 +756,7 @@ int prepare_host_tiles(rchem_ctx* ctx, double tau) {
     for (int j = lo; j < hi; j += 64) ctx->tiles.push_back(Tile{i, j, std::min(64, hi - j), 0});
   });
   flush();
-  for (const QClass& q : ctx->qclasses) {
+  for (const QClass& q : ctx->hclasses) {
     ClassKernel fn;
     TRY(class_kernel(ctx, q.la, q.lb, q.lc, q.ld, kModeTensor, kBoysExact, 0, &fn));
     if (!fn.valid())
@@ -712,59 +770,55 @@ int prepare_host_tiles(rchem_ctx* ctx, double tau) {
 
 // Device-planner tables for threshold tau: classes, buckets, capacities and
 // the FP64 statistics (exact for all-FP64 builds).
+constexpr double kPassTiles = 67108864.0;  // 2^26 tiles (1 GiB per tile list) per pass
+
 int prepare_plan(rchem_ctx* ctx, double tau) {
   TRY(prepare_pairs(ctx, tau));
   if (ctx->plan_tau == tau) return RCHEM_OK;
   ctx->pclasses.clear();
+  ctx->seg_class.clear();
+  ctx->seg_cap.clear();
+  ctx->passes.clear();
   ctx->qclasses.clear();
   std::vector<int4> bks;
-  int tasks = 0, units = 0;
+  std::vector<int> cls_bk_off;
   for (int X = 0; X < kNumPairClasses; ++X)
     for (int Y = 0; Y <= X; ++Y) {
       const int nbra = ctx->class_begin[X + 1] - ctx->class_begin[X];
       const int nbk = static_cast<int>(ctx->buckets[Y].size());
       if (nbra == 0 || nbk == 0) continue;
-      PlanClass pc{};
-      pc.bra_begin = ctx->class_begin[X];
-      pc.bra_end = ctx->class_begin[X + 1];
-      pc.bk_off = static_cast<int>(bks.size());
-      pc.nbk = nbk;
-      pc.task_begin = tasks;
-      pc.same = X == Y;
-      pc.unit_begin = units;
+      cls_bk_off.push_back(static_cast<int>(bks.size()));
       for (const Bucket& bk : ctx->buckets[Y]) bks.push_back(make_int4(bk.begin, bk.end, bk.np, 0));
-      tasks += nbra * nbk;
-      units += nbra;
-      ctx->pclasses.push_back(pc);
-      QClass qc = make_qclass(X, Y);
-      ctx->qclasses.push_back(qc);
+      ctx->qclasses.push_back(make_qclass(X, Y));
     }
-  // capacities (one partial tile more per task covers the FP64/FP32 split) and stats
-  std::vector<double> cap(ctx->qclasses.size(), 0.0);
+  // tiles reserved per bra pair (one partial tile more per task covers the
+  // FP64/FP32 split) and the FP64 statistics
+  const size_t nq = ctx->qclasses.size();
+  std::vector<std::vector<double>> bra_cap(nq);
   rchem_jk_stats st{};
   std::vector<int> cls_index(kNumPairClasses * kNumPairClasses, -1);
-  for (size_t c = 0; c < ctx->qclasses.size(); ++c)
-    cls_index[ctx->qclasses[c].X * kNumPairClasses + ctx->qclasses[c].Y] = static_cast<int>(c);
+  for (size_t c = 0; c < nq; ++c) {
+    const QClass& qc = ctx->qclasses[c];
+    cls_index[qc.X * kNumPairClasses + qc.Y] = static_cast<int>(c);
+    bra_cap[c].assign(ctx->class_begin[qc.X + 1] - ctx->class_begin[qc.X], 0.0);
+  }
   for (auto& qc : ctx->qclasses) qc.quartets = qc.prim_quartets = qc.flops = 0.0;
   for_each_task(ctx, tau, [&](int X, int Y, int i, int b, int lo, int hi) {
     const int c = cls_index[X * kNumPairClasses + Y];
     if (c < 0) return;
     const int cnt = hi - lo;
-    cap[c] += (cnt + 63) / 64 + 1;
+    bra_cap[c][i - ctx->class_begin[X]] += (cnt + 63) / 64 + 1;
     if (cnt <= 0) return;
     QClass& qc = ctx->qclasses[c];
     const double pq = static_cast<double>(cnt) * ctx->pairs[i].np * ctx->buckets[Y][b].np;
     qc.quartets += cnt;
     qc.prim_quartets += pq;
   });
-  double total_cap = 0.0;
-  for (size_t c = 0; c < ctx->qclasses.size(); ++c) {
+  for (size_t c = 0; c < nq; ++c) {
     QClass& qc = ctx->qclasses[c];
     int fh = 0;
     const double fp = per_prim_flops(qc, &fh);
     qc.flops = qc.prim_quartets * fp + qc.quartets * fh;
-    qc.cap = static_cast<int>(std::min(cap[c], 2.0e9));
-    total_cap += cap[c];
     st.quartets += qc.quartets;
     st.primitive_quartets += qc.prim_quartets;
     st.integrals += qc.quartets * ncart(qc.la) * ncart(qc.lb) * ncart(qc.lc) * ncart(qc.ld);
@@ -778,8 +832,60 @@ int prepare_plan(rchem_ctx* ctx, double tau) {
                                                     " not served by this library");
     }
   }
-  if (total_cap > 2.0e9) return ctx->fail(RCHEM_ERR_ALLOC, "quartet tile list too large");
-  ctx->plan_cap = static_cast<int>(total_cap) + 1;
+  // segments and passes
+  rchem_ctx::Pass cur{0, 0, 0, 0, 0};
+  double pass_cap = 0.0;
+  auto close_pass = [&]() {
+    if (cur.nseg == 0) return;
+    cur.cap = static_cast<int>(pass_cap) + 1;
+    ctx->passes.push_back(cur);
+    cur = rchem_ctx::Pass{static_cast<int>(ctx->pclasses.size()), 0, 0, 0, 0};
+    pass_cap = 0.0;
+  };
+  auto add_segment = [&](int c, int lo, int hi, double cap) {
+    const QClass& qc = ctx->qclasses[c];
+    PlanClass pc{};
+    pc.bra_begin = lo;
+    pc.bra_end = hi;
+    pc.bk_off = cls_bk_off[c];
+    pc.nbk = static_cast<int>(ctx->buckets[qc.Y].size());
+    pc.task_begin = cur.tasks;
+    pc.same = qc.X == qc.Y;
+    pc.unit_begin = cur.units;
+    ctx->pclasses.push_back(pc);
+    ctx->seg_class.push_back(c);
+    ctx->seg_cap.push_back(static_cast<int>(cap));
+    cur.tasks += (hi - lo) * pc.nbk;
+    cur.units += hi - lo;
+    cur.nseg += 1;
+    pass_cap += cap;
+  };
+  for (size_t c = 0; c < nq; ++c) {
+    const int b0 = ctx->class_begin[ctx->qclasses[c].X];
+    const int nb = static_cast<int>(bra_cap[c].size());
+    int lo = 0;
+    double acc = 0.0;
+    for (int i = 0; i < nb; ++i) {
+      const double bc = bra_cap[c][i];
+      if (bc > 2.0e9) return ctx->fail(RCHEM_ERR_ALLOC, "quartet tile list of one bra pair too large");
+      if (pass_cap + acc + bc > kPassTiles && (acc > 0.0 || pass_cap > 0.0)) {
+        if (acc > 0.0) add_segment(static_cast<int>(c), b0 + lo, b0 + i, acc);
+        close_pass();
+        lo = i;
+        acc = 0.0;
+      }
+      acc += bc;
+    }
+    if (nb > lo) add_segment(static_cast<int>(c), b0 + lo, b0 + nb, acc);
+  }
+  close_pass();
+  int tasks = 0, units = 0;
+  ctx->plan_cap = 1;
+  for (const auto& ps : ctx->passes) {
+    tasks = std::max(tasks, ps.tasks);
+    units = std::max(units, ps.units);
+    ctx->plan_cap = std::max(ctx->plan_cap, ps.cap);
+  }
   ctx->plan_tasks = tasks;
   ctx->plan_units = units;
   const size_t nt = static_cast<size_t>(tasks) + 1;
@@ -903,7 +1009,7 @@ EriArgs base_args(const rchem_ctx* ctx) {
 
 // Host-tiled launch of every class in `mode` (diagonal / tensor modes).
 int run_eri_host(rchem_ctx* ctx, int mode, int bm, double* out) {
-  for (const QClass& qc : ctx->qclasses) {
+  for (const QClass& qc : ctx->hclasses) {
     ClassKernel fn;
     TRY(class_kernel(ctx, qc.la, qc.lb, qc.lc, qc.ld, mode, bm, 0, &fn));
     if (!fn.valid())
@@ -957,72 +1063,78 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
     HIPC(fill_launch(ctx->plan_dmax.as<double>(), 1, 0.0, ctx->stream));
     HIPC(dmax_launch(dD, n2, ctx->plan_dmax.as<unsigned long long>(), ctx->stream));
   }
-  PlanArgs pa{};
-  pa.classes = ctx->plan_classes_d.as<PlanClass>();
-  pa.nclasses = nc;
-  pa.ntasks = ctx->plan_tasks;
-  pa.nunits = ctx->plan_units;
-  pa.chunk_tiles = kChunkTiles;
-  pa.buckets = ctx->plan_buckets_d.as<int4>();
-  pa.Q = ctx->plan_q_d.as<double>();
-  pa.pairs = ctx->pairs_d.as<PairRec>();
-  pa.tau = o->schwarz_thresh;
-  pa.mixed_thresh = mixed ? o->mixed_thresh : 0.0;
-  pa.dmax = ctx->plan_dmax.as<double>();
-  pa.cnt64 = ctx->plan_cnt64.as<int>();
-  pa.cnt32 = ctx->plan_cnt32.as<int>();
-  pa.lo = ctx->plan_lo.as<int>();
-  pa.mid = ctx->plan_mid.as<int>();
-  pa.hi = ctx->plan_hi.as<int>();
-  pa.stats = ctx->plan_stats.as<double>();
-  PlanBuffers pb{};
-  pb.off64 = ctx->plan_off64.as<int>();
-  pb.off32 = ctx->plan_off32.as<int>();
-  pb.tiles64 = ctx->plan_tiles64.as<Tile>();
-  pb.tiles32 = ctx->plan_tiles32.as<Tile>();
-  pb.chc64 = ctx->plan_chc64.as<int>();
-  pb.chc32 = ctx->plan_chc32.as<int>();
-  pb.choff64 = ctx->plan_choff64.as<int>();
-  pb.choff32 = ctx->plan_choff32.as<int>();
-  pb.chunks64 = ctx->plan_chunks64.as<Chunk>();
-  pb.chunks32 = ctx->plan_chunks32.as<Chunk>();
-  pb.cap = ctx->plan_cap;
-  pb.ranges = ctx->plan_ranges.as<int>();
-  pb.scan_tmp = ctx->plan_scan_tmp.p;
-  pb.scan_bytes = ctx->plan_scan_bytes;
-  if (nc > 0) HIPC(plan_launch(pa, pb, ctx->stream));
   HIPC(fill_launch(ctx->jp_d.as<double>(), std::max<size_t>(ctx->jp_size, 1), 0.0, ctx->stream));
-  for (int c = 0; c < nc; ++c) {
-    const QClass& qc = ctx->qclasses[c];
-    if (qc.quartets <= 0) continue;
-    EriArgs a = base_args(ctx);
-    a.D = dD;
-    a.J = ctx->work_Jh.as<double>();
-    a.K = ctx->work_Kh.as<double>();
-    a.Jp = ctx->jp_d.as<double>();
-    a.jp_bra = ctx->jp_cls[qc.X];
-    a.jp_ket = ctx->jp_cls[qc.Y];
-    // K rows of the bra pair in LDS when they fit in 64 KB
-    const size_t lds = sizeof(double) * (ncart(qc.la) + ncart(qc.lb)) * static_cast<size_t>(ctx->nbf);
-    const bool lds_ok = lds <= 65536 && ctx->lds_digestion && !ctx->force_generic;
-    for (int prec = 0; prec < (mixed ? 2 : 1); ++prec) {
-      const int bm = prec ? kBoysExact : o->boys_mode;
-      a.tiles = prec ? ctx->plan_tiles32.as<Tile>() : ctx->plan_tiles64.as<Tile>();
-      ClassKernel fn;
-      fn.fn = lds_ok ? eri_jk_lds_launcher(qc.la, qc.lb, qc.lc, qc.ld, bm, prec) : nullptr;
-      if (fn.valid()) {
-        a.chunks = prec ? ctx->plan_chunks32.as<Chunk>() : ctx->plan_chunks64.as<Chunk>();
-        a.chunk_range = ctx->plan_ranges.as<int>() + 8 * c + (prec ? 6 : 4);
-        a.tile_range = nullptr;
-        TRY(launch_timed(ctx, qc, prec, fn, a, std::min(kPersistentBlocks, qc.cap)));
-      } else {
-        TRY(class_kernel(ctx, qc.la, qc.lb, qc.lc, qc.ld, kModeJK, bm, prec, &fn));
-        if (!fn.valid())
-          return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class " +
-                                                      class_name(qc.la, qc.lb, qc.lc, qc.ld) +
-                                                      " not served by this library");
-        a.tile_range = ctx->plan_ranges.as<int>() + 8 * c + (prec ? 2 : 0);
-        TRY(launch_timed(ctx, qc, prec, fn, a, std::min(kPersistentBlocks, (qc.cap + 3) / 4)));
+  for (const auto& ps : ctx->passes) {
+    PlanArgs pa{};
+    pa.classes = ctx->plan_classes_d.as<PlanClass>() + ps.seg_begin;
+    pa.nclasses = ps.nseg;
+    pa.ntasks = ps.tasks;
+    pa.nunits = ps.units;
+    pa.chunk_tiles = kChunkTiles;
+    pa.buckets = ctx->plan_buckets_d.as<int4>();
+    pa.Q = ctx->plan_q_d.as<double>();
+    pa.pairs = ctx->pairs_d.as<PairRec>();
+    pa.tau = o->schwarz_thresh;
+    pa.mixed_thresh = mixed ? o->mixed_thresh : 0.0;
+    pa.dmax = ctx->plan_dmax.as<double>();
+    pa.cnt64 = ctx->plan_cnt64.as<int>();
+    pa.cnt32 = ctx->plan_cnt32.as<int>();
+    pa.lo = ctx->plan_lo.as<int>();
+    pa.mid = ctx->plan_mid.as<int>();
+    pa.hi = ctx->plan_hi.as<int>();
+    pa.stats = ctx->plan_stats.as<double>() + 4 * static_cast<size_t>(ps.seg_begin);
+    PlanBuffers pb{};
+    pb.off64 = ctx->plan_off64.as<int>();
+    pb.off32 = ctx->plan_off32.as<int>();
+    pb.tiles64 = ctx->plan_tiles64.as<Tile>();
+    pb.tiles32 = ctx->plan_tiles32.as<Tile>();
+    pb.chc64 = ctx->plan_chc64.as<int>();
+    pb.chc32 = ctx->plan_chc32.as<int>();
+    pb.choff64 = ctx->plan_choff64.as<int>();
+    pb.choff32 = ctx->plan_choff32.as<int>();
+    pb.chunks64 = ctx->plan_chunks64.as<Chunk>();
+    pb.chunks32 = ctx->plan_chunks32.as<Chunk>();
+    pb.cap = ps.cap;
+    pb.ranges = ctx->plan_ranges.as<int>() + 8 * static_cast<size_t>(ps.seg_begin);
+    pb.scan_tmp = ctx->plan_scan_tmp.p;
+    pb.scan_bytes = ctx->plan_scan_bytes;
+    HIPC(plan_launch(pa, pb, ctx->stream));
+    for (int sg = ps.seg_begin; sg < ps.seg_begin + ps.nseg; ++sg) {
+      const int c = ctx->seg_class[sg];
+      const QClass& qc = ctx->qclasses[c];
+      if (qc.quartets <= 0) continue;
+      const int cap = ctx->seg_cap[sg];
+      EriArgs a = base_args(ctx);
+      a.D = dD;
+      a.J = ctx->work_Jh.as<double>();
+      a.K = ctx->work_Kh.as<double>();
+      a.Jp = ctx->jp_d.as<double>();
+      a.jp_bra = ctx->jp_cls[qc.X];
+      a.jp_ket = ctx->jp_cls[qc.Y];
+      // K rows of the bra pair in LDS when they fit in 64 KB
+      const size_t lds =
+          sizeof(double) * (ncart(qc.la) + ncart(qc.lb)) * static_cast<size_t>(ctx->nbf);
+      const bool lds_ok = lds <= 65536 && ctx->lds_digestion && !ctx->force_generic;
+      int* ranges = pb.ranges + 8 * (sg - ps.seg_begin);
+      for (int prec = 0; prec < (mixed ? 2 : 1); ++prec) {
+        const int bm = prec ? kBoysExact : o->boys_mode;
+        a.tiles = prec ? ctx->plan_tiles32.as<Tile>() : ctx->plan_tiles64.as<Tile>();
+        ClassKernel fn;
+        fn.fn = lds_ok ? eri_jk_lds_launcher(qc.la, qc.lb, qc.lc, qc.ld, bm, prec) : nullptr;
+        if (fn.valid()) {
+          a.chunks = prec ? ctx->plan_chunks32.as<Chunk>() : ctx->plan_chunks64.as<Chunk>();
+          a.chunk_range = ranges + (prec ? 6 : 4);
+          a.tile_range = nullptr;
+          TRY(launch_timed(ctx, qc, prec, fn, a, std::min(kPersistentBlocks, cap)));
+        } else {
+          TRY(class_kernel(ctx, qc.la, qc.lb, qc.lc, qc.ld, kModeJK, bm, prec, &fn));
+          if (!fn.valid())
+            return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class " +
+                                                        class_name(qc.la, qc.lb, qc.lc, qc.ld) +
+                                                        " not served by this library");
+          a.tile_range = ranges + (prec ? 2 : 0);
+          TRY(launch_timed(ctx, qc, prec, fn, a, std::min(kPersistentBlocks, (cap + 3) / 4)));
+        }
       }
     }
   }
@@ -1300,8 +1412,12 @@ int rchem_jk_class_timings(rchem_ctx* ctx, rchem_class_timing* out, int max, int
       const QClass& qc = ctx->qclasses[t.cls];
       int fh = 0;
       const double fp = per_prim_flops(qc, &fh);
-      t.quartets = dv[4 * t.cls + (t.fp32 ? 1 : 0)];
-      t.prim_quartets = dv[4 * t.cls + (t.fp32 ? 3 : 2)];
+      t.quartets = t.prim_quartets = 0.0;
+      for (size_t sg = 0; sg < ctx->seg_class.size(); ++sg)
+        if (ctx->seg_class[sg] == t.cls) {
+          t.quartets += dv[4 * sg + (t.fp32 ? 1 : 0)];
+          t.prim_quartets += dv[4 * sg + (t.fp32 ? 3 : 2)];
+        }
       t.flops = t.prim_quartets * fp + t.quartets * fh;
     }
   }
@@ -1591,11 +1707,16 @@ int rchem_scf_opts_default(rchem_scf_opts* o) {
   o->thresh_e = 1e-11;
   rchem_opts_default(&o->jk);
   o->verbose = 0;
+  o->diis = 0;
   return RCHEM_OK;
 }
 
-int rchem_scf(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
-              const rchem_scf_opts* opts, rchem_scf_result* res, double* D_out) {
+}  // extern "C"
+
+namespace {
+
+int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
+            const rchem_scf_opts* opts, rchem_scf_result* res, double dconv) {
   if (!ctx) return RCHEM_ERR_INVALID;
   if (!ctx->have_basis) return ctx->fail(RCHEM_ERR_NO_BASIS, "no basis uploaded");
   if (!opts || !res) return ctx->fail(RCHEM_ERR_INVALID, "null opts or result");
@@ -1609,8 +1730,12 @@ int rchem_scf(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
   std::vector<double> S(n2), T(n2), V(n2), H(n2);
   TRY(rchem_one_electron(ctx, natom, Z, xyz, S.data(), T.data(), V.data()));
   for (size_t i = 0; i < n2; ++i) H[i] = T[i] + V[i];
-  DevBuf dS, dH, dX, dW, dF, dFp, dC, dD, dJ, dK, dtmp, dw, de, dinfo, dE;
-  for (DevBuf* b : {&dS, &dH, &dX, &dW, &dF, &dFp, &dC, &dD, &dJ, &dK, &dtmp})
+  DevBuf dS, dH, dX, dW, dF, dFp, dJ, dK, dtmp, de, dinfo, dE;
+  DevBuf& dC = ctx->scf_C;
+  DevBuf& dD = ctx->scf_D;
+  DevBuf& dw = ctx->scf_w;
+  DevBuf& dDo = ctx->scf_Dold;
+  for (DevBuf* b : {&dS, &dH, &dX, &dW, &dF, &dFp, &dC, &dD, &dJ, &dK, &dtmp, &dDo})
     HIPC(b->ensure(n2 * sizeof(double)));
   HIPC(dw.ensure(n * sizeof(double)));
   HIPC(de.ensure(n * sizeof(double)));
@@ -1655,17 +1780,132 @@ int rchem_scf(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
   res->converged = 0;
   res->iterations = 0;
   int it = 0;
-  for (; it < opts->max_iter; ++it) {
+  if (opts->diis > 0) {
+    // Pulay DIIS: extrapolate F from the last k Fock matrices so that the
+    // orthogonalised commutators X^T (FDS - SDF) X combine to the smallest norm.
+    const int k = std::min(opts->diis, 32);
+    const double errtol = dconv > 0.0 ? dconv : 1e-8;
+    DevBuf dFs, dEs, dE1, dFx;
+    HIPC(dFs.ensure(n2 * sizeof(double) * k));
+    HIPC(dEs.ensure(n2 * sizeof(double) * k));
+    HIPC(dE1.ensure(n2 * sizeof(double)));
+    HIPC(dFx.ensure(n2 * sizeof(double)));
+    std::vector<double> B(static_cast<size_t>(k) * k, 0.0);
+    int count = 0;
+    double e_prev = 0.0;
+    const rocblas_int nn = static_cast<rocblas_int>(n2);
+    for (; it < opts->max_iter; ++it) {
+      TRY(jk_device(ctx, dD.as<double>(), dJ.as<double>(), dK.as<double>(), &opts->jk));
+      HIPC(fock_launch(dH.as<double>(), dJ.as<double>(), dK.as<double>(), dF.as<double>(), n2,
+                       ctx->stream));
+      double e_cur = 0.0;
+      TRY(energy(dF.as<double>(), &e_cur));  // sum D o (H + F(D)) of the current density
+      if (!std::isfinite(e_cur)) return ctx->fail(RCHEM_ERR_NONFINITE, "non-finite SCF energy");
+      // commutator FDS - SDF, then X^T (.) X
+      BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
+                          dF.as<double>(), n, dD.as<double>(), n, &zero, dtmp.as<double>(), n));
+      BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
+                          dtmp.as<double>(), n, dS.as<double>(), n, &zero, dFp.as<double>(), n));
+      HIPC(antisym_launch(dFp.as<double>(), dE1.as<double>(), n, ctx->stream));
+      const int slot = count % k;
+      double* Fslot = dFs.as<double>() + static_cast<size_t>(slot) * n2;
+      double* Eslot = dEs.as<double>() + static_cast<size_t>(slot) * n2;
+      BLASC(rocblas_dgemm(hb, rocblas_operation_transpose, rocblas_operation_none, n, n, n, &one,
+                          dX.as<double>(), n, dE1.as<double>(), n, &zero, dtmp.as<double>(), n));
+      BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
+                          dtmp.as<double>(), n, dX.as<double>(), n, &zero, Eslot, n));
+      HIPC(hipMemcpyAsync(Fslot, dF.p, n2 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      ++count;
+      const int m = std::min(count, k);
+      for (int i = 0; i < m; ++i) {
+        double dot = 0.0;
+        BLASC(rocblas_ddot(hb, nn, Eslot, 1, dEs.as<double>() + static_cast<size_t>(i) * n2, 1,
+                           &dot));
+        B[static_cast<size_t>(slot) * k + i] = B[static_cast<size_t>(i) * k + slot] = dot;
+      }
+      rocblas_int imax = 0;
+      BLASC(rocblas_idamax(hb, nn, Eslot, 1, &imax));
+      double errmax = 0.0;
+      HIPC(hipMemcpy(&errmax, Eslot + (imax - 1), sizeof(double), hipMemcpyDeviceToHost));
+      errmax = std::fabs(errmax);
+      const double de_ = e_cur - e_prev;
+      if (opts->verbose) std::printf("%4d %20.12f %20.12f %12.3e\n", it, e_cur, de_, errmax);
+      e_new = e_cur;
+      if (it > 0 && std::fabs(de_) < opts->thresh_e && errmax < errtol) {
+        if (opts->verbose) std::printf("Convergence achieved!\n");
+        res->converged = 1;
+        ++it;
+        TRY(density(dF.as<double>()));  // C, eps and D of the converged Fock matrix
+        break;
+      }
+      e_prev = e_cur;
+      // coefficients: [B 1; 1 0] (c, -lambda) = (0, 1), Gaussian elimination
+      const int sz = m + 1;
+      std::vector<double> A(static_cast<size_t>(sz) * sz, 0.0), rhs(sz, 0.0);
+      for (int i = 0; i < m; ++i) {
+        for (int j = 0; j < m; ++j) A[i * sz + j] = B[static_cast<size_t>(i) * k + j];
+        A[i * sz + m] = A[m * sz + i] = 1.0;
+      }
+      rhs[m] = 1.0;
+      bool ok = true;
+      for (int c = 0; c < sz && ok; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < sz; ++r)
+          if (std::fabs(A[r * sz + c]) > std::fabs(A[piv * sz + c])) piv = r;
+        if (std::fabs(A[piv * sz + c]) < 1e-300) {
+          ok = false;
+          break;
+        }
+        if (piv != c) {
+          for (int j = 0; j < sz; ++j) std::swap(A[c * sz + j], A[piv * sz + j]);
+          std::swap(rhs[c], rhs[piv]);
+        }
+        for (int r = 0; r < sz; ++r) {
+          if (r == c) continue;
+          const double f = A[r * sz + c] / A[c * sz + c];
+          for (int j = c; j < sz; ++j) A[r * sz + j] -= f * A[c * sz + j];
+          rhs[r] -= f * rhs[c];
+        }
+      }
+      if (ok) {
+        HIPC(fill_launch(dFx.as<double>(), n2, 0.0, ctx->stream));
+        for (int i = 0; i < m; ++i) {
+          const double ci = rhs[i] / A[i * sz + i];
+          BLASC(rocblas_daxpy(hb, nn, &ci, dFs.as<double>() + static_cast<size_t>(i) * n2, 1,
+                              dFx.as<double>(), 1));
+        }
+        TRY(density(dFx.as<double>()));
+      } else {
+        count = 0;  // singular subspace: restart from the current Fock matrix
+        TRY(density(dF.as<double>()));
+      }
+    }
+  }
+  for (; opts->diis <= 0 && it < opts->max_iter; ++it) {
     TRY(jk_device(ctx, dD.as<double>(), dJ.as<double>(), dK.as<double>(), &opts->jk));
     HIPC(fock_launch(dH.as<double>(), dJ.as<double>(), dK.as<double>(), dF.as<double>(), n2,
                      ctx->stream));
+    if (dconv > 0.0)
+      HIPC(hipMemcpyAsync(dDo.p, dD.p, n2 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     TRY(density(dF.as<double>()));
     const double e_old = e_new;
     TRY(energy(dF.as<double>(), &e_new));
     if (!std::isfinite(e_new)) return ctx->fail(RCHEM_ERR_NONFINITE, "non-finite SCF energy");
     const double de_ = e_new - e_old;
     if (opts->verbose) std::printf("%4d %20.12f %20.12f\n", it, e_new, de_);
-    if (std::fabs(de_) < opts->thresh_e) {
+    double ddmax = 0.0;
+    if (dconv > 0.0) {
+      // max |D_new - D_old| (the gradient wants a self-consistent density)
+      const double m1 = -1.0;
+      rocblas_int imax = 0;
+      BLASC(rocblas_daxpy(hb, static_cast<rocblas_int>(n2), &m1, dD.as<double>(), 1,
+                          dDo.as<double>(), 1));
+      BLASC(rocblas_idamax(hb, static_cast<rocblas_int>(n2), dDo.as<double>(), 1, &imax));
+      HIPC(hipMemcpy(&ddmax, dDo.as<double>() + (imax - 1), sizeof(double),
+                     hipMemcpyDeviceToHost));
+      ddmax = std::fabs(ddmax);
+    }
+    if (std::fabs(de_) < opts->thresh_e && ddmax <= dconv) {
       if (opts->verbose) std::printf("Convergence achieved!\n");
       res->converged = 1;
       ++it;
@@ -1683,11 +1923,148 @@ int rchem_scf(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
     }
   res->e_nuc = enuc;
   res->e_total = e_new + enuc;
-  if (D_out)
-    HIPC(hipMemcpy(D_out, dD.p, n2 * sizeof(double), hipMemcpyDeviceToHost));
+  HIPC(hipStreamSynchronize(ctx->stream));
   if (!res->converged)
     return ctx->fail(RCHEM_ERR_NOT_CONVERGED, "SCF did not converge in max_iter iterations");
   return RCHEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rchem_rhf_gradient(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
+                       const rchem_scf_opts* opts, rchem_scf_result* res, double* grad) {
+  if (!ctx) return RCHEM_ERR_INVALID;
+  if (!grad) return ctx->fail(RCHEM_ERR_INVALID, "null gradient");
+  if (natom <= 0 || !Z || !xyz) return ctx->fail(RCHEM_ERR_INVALID, "bad nuclei");
+  const int rc = scf_run(ctx, natom, Z, xyz, opts, res, 1e-9);
+  if (rc != RCHEM_OK && rc != RCHEM_ERR_NOT_CONVERGED) return rc;
+  const int n = ctx->nbf, ns = static_cast<int>(ctx->shells.size());
+  const size_t n2 = static_cast<size_t>(n) * n;
+  // energy-weighted density W = C_occ diag(eps_occ) C_occ^T (column-major C)
+  DevBuf dCe, dW, dgrad, dpa, dsat, dsh, dex, dcc, dpr, dax, daz;
+  HIPC(dCe.ensure(n2 * sizeof(double)));
+  HIPC(dW.ensure(n2 * sizeof(double)));
+  const double one = 1.0, zero = 0.0;
+  BLASC(rocblas_ddgmm(ctx->blas, rocblas_side_right, n, opts->nocc, ctx->scf_C.as<double>(), n,
+                      ctx->scf_w.as<double>(), 1, dCe.as<double>(), n));
+  BLASC(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose, n, n,
+                      opts->nocc, &one, dCe.as<double>(), n, ctx->scf_C.as<double>(), n, &zero,
+                      dW.as<double>(), n));
+  // shell -> nucleus
+  std::vector<int> satom(ns, -1);
+  for (int s = 0; s < ns; ++s)
+    for (int A = 0; A < natom; ++A) {
+      const double dx = ctx->shells[s].xyz[0] - xyz[3 * A], dy = ctx->shells[s].xyz[1] - xyz[3 * A + 1],
+                   dz = ctx->shells[s].xyz[2] - xyz[3 * A + 2];
+      if (dx * dx + dy * dy + dz * dz < 1e-16) {
+        satom[s] = A;
+        break;
+      }
+    }
+  for (int s = 0; s < ns; ++s)
+    if (satom[s] < 0) return ctx->fail(RCHEM_ERR_INVALID, "a shell centre is not at a nucleus");
+  HIPC(dgrad.ensure(sizeof(double) * 3 * natom));
+  HIPC(fill_launch(dgrad.as<double>(), 3 * static_cast<size_t>(natom), 0.0, ctx->stream));
+  // two-electron part: derivative quartets contracted with Gamma on the fly
+  const double tau = std::min(opts->jk.schwarz_thresh > 0.0 ? opts->jk.schwarz_thresh : 1e-12,
+                              1e-12) * 1e-2;
+  TRY(prepare_host_tiles(ctx, tau));
+  std::vector<int2> pat(ctx->pairs.size());
+  for (size_t i = 0; i < pat.size(); ++i)
+    pat[i] = make_int2(satom[ctx->pair_shells[i].first], satom[ctx->pair_shells[i].second]);
+  HIPC(dpa.ensure(sizeof(int2) * std::max<size_t>(pat.size(), 1)));
+  if (!pat.empty())
+    HIPC(hipMemcpyAsync(dpa.p, pat.data(), sizeof(int2) * pat.size(), hipMemcpyHostToDevice,
+                        ctx->stream));
+  for (const QClass& hc : ctx->hclasses) {
+    const GenericProgram* gp = nullptr;
+    TRY(find_program(ctx, hc.la, hc.lb, hc.lc, hc.ld, 1, &gp));
+    if (!gp)
+      return ctx->fail(RCHEM_ERR_UNSUPPORTED, "gradient of ERI class " +
+                                                  class_name(hc.la, hc.lb, hc.lc, hc.ld) +
+                                                  " not served by this library");
+    EriArgs a = base_args(ctx);
+    a.tiles = ctx->tiles_d.as<Tile>() + hc.tile_begin;
+    a.ntiles = hc.ntiles;
+    a.D = ctx->scf_D.as<double>();
+    a.prim_alpha = ctx->alpha_d.as<double>();
+    a.pair_atoms = dpa.as<int2>();
+    a.grad = dgrad.as<double>();
+    HIPC(eri_generic_launch(a, *gp, kModeGrad, kBoysExact, 0, ctx->stream));
+  }
+  // one-electron part
+  std::vector<ShellDev> sh(ns);
+  for (int s = 0; s < ns; ++s) {
+    for (int x = 0; x < 3; ++x) sh[s].xyz[x] = ctx->shells[s].xyz[x];
+    sh[s].l = ctx->shells[s].l;
+    sh[s].nprim = ctx->shells[s].nprim;
+    sh[s].p0 = ctx->shells[s].p0;
+    sh[s].f0 = ctx->shells[s].f0;
+  }
+  std::vector<int2> pr;
+  for (int a = 0; a < ns; ++a)
+    for (int b = 0; b <= a; ++b) pr.push_back(make_int2(a, b));
+  HIPC(dsh.ensure(sizeof(ShellDev) * ns));
+  HIPC(dex.ensure(sizeof(double) * ctx->exps.size()));
+  HIPC(dcc.ensure(sizeof(double) * ctx->ccoef.size()));
+  HIPC(dpr.ensure(sizeof(int2) * pr.size()));
+  HIPC(dax.ensure(sizeof(double) * 3 * natom));
+  HIPC(daz.ensure(sizeof(double) * natom));
+  HIPC(dsat.ensure(sizeof(int) * ns));
+  HIPC(hipMemcpyAsync(dsh.p, sh.data(), sizeof(ShellDev) * ns, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(dex.p, ctx->exps.data(), sizeof(double) * ctx->exps.size(),
+                      hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(dcc.p, ctx->ccoef.data(), sizeof(double) * ctx->ccoef.size(),
+                      hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(dpr.p, pr.data(), sizeof(int2) * pr.size(), hipMemcpyHostToDevice,
+                      ctx->stream));
+  HIPC(hipMemcpyAsync(dax.p, xyz, sizeof(double) * 3 * natom, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(daz.p, Z, sizeof(double) * natom, hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemcpyAsync(dsat.p, satom.data(), sizeof(int) * ns, hipMemcpyHostToDevice, ctx->stream));
+  OneElGradArgs g{};
+  g.shells = dsh.as<ShellDev>();
+  g.exps = dex.as<double>();
+  g.ccoef = dcc.as<double>();
+  g.pairs = dpr.as<int2>();
+  g.npairs = static_cast<int>(pr.size());
+  g.nbf = n;
+  g.natom = natom;
+  g.atom_xyz = dax.as<double>();
+  g.atom_z = daz.as<double>();
+  g.boys_table = ctx->boys_d.as<double>();
+  g.shell_atom = dsat.as<int>();
+  g.D = ctx->scf_D.as<double>();
+  g.W = dW.as<double>();
+  g.grad = dgrad.as<double>();
+  HIPC(one_electron_grad_launch(g, ctx->stream));
+  HIPC(hipMemcpyAsync(grad, dgrad.p, sizeof(double) * 3 * natom, hipMemcpyDeviceToHost,
+                      ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  // nuclear repulsion: d/dR_A sum_{B<C} Z_B Z_C / R_BC
+  for (int A = 0; A < natom; ++A)
+    for (int B = 0; B < natom; ++B) {
+      if (B == A) continue;
+      double d[3], r2 = 0.0;
+      for (int x = 0; x < 3; ++x) {
+        d[x] = xyz[3 * A + x] - xyz[3 * B + x];
+        r2 += d[x] * d[x];
+      }
+      const double r3 = r2 * std::sqrt(r2);
+      for (int x = 0; x < 3; ++x) grad[3 * A + x] -= Z[A] * Z[B] * d[x] / r3;
+    }
+  return rc;
+}
+
+int rchem_scf(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
+              const rchem_scf_opts* opts, rchem_scf_result* res, double* D_out) {
+  const int rc = scf_run(ctx, natom, Z, xyz, opts, res, 0.0);
+  if ((rc == RCHEM_OK || rc == RCHEM_ERR_NOT_CONVERGED) && D_out) {
+    const size_t n2 = static_cast<size_t>(ctx->nbf) * ctx->nbf;
+    HIPC(hipMemcpy(D_out, ctx->scf_D.p, n2 * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return rc;
 }
 
 }  // extern "C"

@@ -3,7 +3,8 @@
 //   fock:          F = H + 2J - K (rchem.rs:110-127);
 //   energy:        E = sum D o (H + F) (rchem.rs:106-108), one workgroup,
 //                  fixed reduction order, so it is deterministic;
-//   scale_columns: U diag(lambda^-1/2), the middle of Loewdin X (rchem.rs:44-51).
+//   scale_columns: U diag(lambda^-1/2), the middle of Loewdin X (rchem.rs:44-51);
+//   antisym:       E = A - A^T, the DIIS commutator FDS - SDF from A = FDS.
 #include <hip/hip_runtime.h>
 
 namespace rchem {
@@ -46,6 +47,13 @@ __global__ void scale_columns_kernel(const double* __restrict__ U, const double*
   out[k] = U[k] / sqrt(w[j]);
 }
 
+__global__ void antisym_kernel(const double* __restrict__ A, double* __restrict__ E, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= n) return;
+  E[static_cast<size_t>(j) * n + i] = A[static_cast<size_t>(j) * n + i] - A[static_cast<size_t>(i) * n + j];
+}
+
 static unsigned grid_for(size_t n) {
   size_t b = (n + 255) / 256;
   if (b > 65536) b = 65536;
@@ -78,4 +86,11 @@ hipError_t scale_columns_launch(const double* U, const double* w, double* out, i
   return hipGetLastError();
 }
 
+}  // namespace rchem
+
+namespace rchem {
+hipError_t antisym_launch(const double* A, double* E, int n, hipStream_t s) {
+  hipLaunchKernelGGL(antisym_kernel, dim3((n + 255) / 256, n), dim3(256), 0, s, A, E, n);
+  return hipGetLastError();
+}
 }  // namespace rchem

@@ -53,7 +53,7 @@ struct Chunk {
   int pad;
 };
 
-enum EriMode { kModeJK = 0, kModeDiag = 1, kModeTensor = 2, kModeQuartet = 3 };
+enum EriMode { kModeJK = 0, kModeDiag = 1, kModeTensor = 2, kModeQuartet = 3, kModeGrad = 4 };
 
 // Pair-indexed Coulomb accumulators: component k of pair p (pair class X) is
 // Jp[base_X + k * n_X + (p - begin_X)], so the 64 lanes of a wave, which hold
@@ -82,33 +82,37 @@ struct EriArgs {
   const int* chunk_range;  // [begin, end) in chunks, device memory
   double* Jp;
   JpClass jp_bra, jp_ket;
+  // kModeGrad (wave-cooperative kernel): exponent of the first shell of each
+  // primitive pair (same layout as prims), atoms of each pair's two shells,
+  // and the 3 x natom gradient accumulated into
+  const double* prim_alpha;
+  const int2* pair_atoms;
+  double* grad;
 };
 
 // A class served by the wave-cooperative kernel: its recurrences as operation
 // tables in device memory (layouts in gen_eri_prog.py).  One wavefront
-// evaluates one contracted quartet; its LDS holds the primitive-quartet
-// records of a batch and one buffer of max(nacc + nv, nh) doubles that is the
-// VRR workspace + contracted accumulators during the primitive loop and the
+// evaluates one contracted quartet; its LDS holds the records of a batch of P
+// primitive quartets and one buffer of max(nacc + nv P, nh) doubles that is
+// the contracted accumulators + VRR slots during the primitive loop and the
 // HRR workspace afterwards.
 struct GenericProgram {
   int la, lb, lc, ld;
-  int L;          // la + lb + lc + ld
-  int nacc;       // contracted accumulators [e0|f0]
-  int nv;         // VRR slots (slot 0 = zero, 1..L+1 = G_m)
+  int L;          // highest Boys order (la + lb + lc + ld, + 1 for the gradient)
+  int nacc;       // contracted accumulators
+  int nv;         // VRR slots per primitive quartet (slot 0 = zero, 1..L+1 = G_m)
   int nh;         // HRR slots (the first nacc are the accumulators)
-  int nout;       // integrals of the class
-  int nvgroups, nhgroups;
-  int pad;
-  const uint4* vops;     // VRR operations
-  const uint4* vgroups;  // (op_begin, op_end, side | axis << 1 | level_end << 3, 0)
-  const uint2* hops;     // HRR operations
+  int nout;       // outputs (integrals, or 9 x integrals for the gradient)
+  int nvgroups, nhgroups, naccs;
+  int P;          // primitive quartets per batch (power of two)
+  const uint4* vops;      // VRR operations
+  const uint4* vgroups;   // (op_begin, op_end, side | axis << 1 | level_end << 3, 0)
+  const uint2* accs;      // (slot | set << 16, accumulator)
+  const uint2* hops;      // HRR operations
   const uint4* hgroups;
-  const int* out_src;    // HRR slot of integral k (canonical component order)
-  const double* out_norm;  // per-component normalisation of integral k
+  const int4* outs;       // (plus slot, minus slot, n, 0): norm (H[plus] - n H[minus])
+  const double* out_norm;
 };
-
-// Primitive quartets set up per batch in the wave-cooperative kernel.
-constexpr int kGenericPqBatch = 32;
 
 // One quartet class as the device planner sees it.
 struct PlanClass {

@@ -224,3 +224,47 @@ def test_mixed_precision_h2o64(gpu_ctx):
     _say(f"  E2(fp64)={e64:.10f} E2(mixed)={e32:.10f} diff={e32 - e64:.3e}")
     assert abs(e32 - e64) < 1e-6
     assert np.max(np.abs(Jm - J)) < 1e-6 and np.max(np.abs(Km - K)) < 1e-6
+
+
+def test_microbench_basis_sampled_quartets_every_class(gpu_ctx):
+    """The synthetic 2000-shell s/p/d microbenchmark basis: per (la lb|lc ld)
+    class, shell quartets drawn at random among the Schwarz-surviving ones
+    (Q_ab Q_cd >= 1e-12), against the oracle's THO integrals with the
+    reference's Boys function (bar 1e-12 absolute)."""
+    from rchem_amd.inputs import synthetic_shells
+    sh = synthetic_shells(nshell=2000, seed=2000, box=20.0, nprim=3)
+    B = rb.Basis.from_shells(*sh, ctx=gpu_ctx)
+    Q = rb.schwarz(B)
+    Q = np.maximum(Q, Q.T)
+    ls = sh[1]
+    rng = np.random.default_rng(99)
+    picked = {}
+    tries = 0
+    while len(picked) < 21 * 2 and tries < 200000:
+        tries += 1
+        q = rng.integers(0, len(ls), 4)
+        if Q[q[0], q[1]] * Q[q[2], q[3]] < 1e-12:
+            continue
+        key = _canonical_class([ls[s] for s in q])
+        picked.setdefault(key, [])
+        if len(picked[key]) < 2:
+            picked[key].append(q)
+    quartets = np.array([q for v in picked.values() for q in v])
+    _say(f"case: microbench basis, {len(quartets)} quartets over {len(picked)} classes")
+    assert set(picked) == ALL_CLASSES
+    blocks = rb.eri_shell_quartets(B, quartets, default_opts(boys_mode=BOYS_COMPAT,
+                                                             schwarz_thresh=0.0))
+    Bo = O.cgtos_from_shells(*sh)
+    f0 = np.concatenate([[0], np.cumsum((ls + 1) * (ls + 2) // 2)])
+    worst = 0.0
+    for r, blk in zip(quartets, blocks):
+        ref = np.zeros(blk.shape)
+        for a in range(blk.shape[0]):
+            for b in range(blk.shape[1]):
+                for c in range(blk.shape[2]):
+                    for d in range(blk.shape[3]):
+                        ref[a, b, c, d] = O.contracted_eri(Bo, f0[r[0]] + a, f0[r[1]] + b,
+                                                           f0[r[2]] + c, f0[r[3]] + d,
+                                                           boys=O.BOYS_COMPAT)
+        worst = max(worst, np.max(np.abs(blk - ref)))
+    assert worst < 1e-12
