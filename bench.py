@@ -2,21 +2,29 @@
 """Fock-build benchmark: (H2O)_64, 6-31G (832 functions), one GPU.
 
 A step is one Fock build (J and K for a fixed density) through
-``rchem_jk_device`` with D, J, K resident in HBM.  Prints one JSON line:
+``rchem_jk_device`` with D, J, K resident in HBM.  The default run times the
+mixed-precision build (FP32 for quartets whose Schwarz bound times max|D| is
+below 1e-7, FP64 for the rest) and the all-FP64 build on the same density,
+then runs the SCF to convergence both ways (DIIS, all-FP64 final iterations
+after the convergence-triggered switch) and reports the energy difference.
+Prints one JSON line:
 
   value        unique contracted shell quartets per second (after Schwarz
-               screening at 1e-12 and 8-fold symmetry) of the whole build;
-  ms_per_step  Fock-build milliseconds;
-  roofline     the dominant (la lb|lc ld) class kernel: algorithmic FP64 flops
-               per launch (generator operation counts, FMA = 2) over its
-               average launch time from HIP events on the library's stream;
+               screening at 1e-12 and 8-fold symmetry) of the mixed build;
+  ms_per_step  mixed Fock-build milliseconds; "fp64" holds the all-FP64 build,
+               "speedup_mixed_vs_fp64" their ratio, "scf" the two SCF energies;
+  roofline     the dominant (la lb|lc ld) class kernel of the mixed build:
+               algorithmic flops per launch (generator operation counts,
+               FMA = 2) over its launch time from HIP events on the library's
+               stream, HBM traffic from the committed rocprofv3 counter runs;
   cpu_baseline the reference algorithm (rchem's JK_direct loop over the
                reference's own cints coulomb_repulsion, oracle/_ref) on one
                host core over a bounded random sample of function quartets,
                expressed as the same Fock build's quartets per second.
 
-Options: --steps K --warmup W --gpus N (N must be 1) --precision fp64|mixed
-         --no-cpu-baseline --cpu-seconds S
+Options: --steps K --warmup W --gpus N (N must be 1) --no-cpu-baseline
+         --cpu-seconds S --no-scf --precision both|fp64|mixed
+         --workload h2o64|eri2000 [--shells N]
 """
 import argparse
 import ctypes
@@ -90,7 +98,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--no-scf", action="store_true", help="skip the two full SCF runs")
+    ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both",
+                    help="Fock builds to time (profiling runs take one)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", choices=["h2o64", "eri2000"], default="h2o64",
@@ -133,19 +143,14 @@ def main():
         workload = (f"synthetic ERI microbenchmark: {args.shells} shells, seed 2000, "
                     f"{box:.1f}-bohr cube, l 0/1/2 at 0.5/0.35/0.15, 3 primitives, every quartet "
                     f"surviving Schwarz 1e-12, digested into J and K")
-    prec = PREC_FP64 if args.precision == "fp64" else PREC_MIXED
-    opts = default_opts(precision=prec, schwarz_thresh=1e-12)
-
     dev = torch.device("cuda:0")
     dD = torch.from_numpy(D).to(dev)
     dJ = torch.zeros((n, n), dtype=torch.float64, device=dev)
     dK = torch.zeros((n, n), dtype=torch.float64, device=dev)
     L = lib()
 
-    def step():
-        ctx.check(L.rchem_jk_device(ctx.h, ctypes.c_void_p(dD.data_ptr()),
-                                    ctypes.c_void_p(dJ.data_ptr()),
-                                    ctypes.c_void_p(dK.data_ptr()), ctypes.byref(opts)))
+    def say(msg):
+        print(f"[bench] {msg} ({time.perf_counter() - t_start:.1f} s)", file=sys.stderr, flush=True)
 
     if args.copy_calibration_mb > 0:
         nel = args.copy_calibration_mb * (1 << 20) // 8
@@ -154,46 +159,63 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    def say(msg):
-        print(f"[bench] {msg} ({time.perf_counter() - t_start:.1f} s)", file=sys.stderr, flush=True)
 
-    for i in range(args.warmup):
-        step()
+    def measure(prec_name):
+        prec = PREC_FP64 if prec_name == "fp64" else PREC_MIXED
+        opts = default_opts(precision=prec, schwarz_thresh=1e-12)
+
+        def step():
+            ctx.check(L.rchem_jk_device(ctx.h, ctypes.c_void_p(dD.data_ptr()),
+                                        ctypes.c_void_p(dJ.data_ptr()),
+                                        ctypes.c_void_p(dK.data_ptr()), ctypes.byref(opts)))
+
+        for i in range(args.warmup):
+            step()
+            torch.cuda.synchronize()
+            say(f"{prec_name} warmup step {i + 1}/{args.warmup} done")
+        ctx.check(L.rchem_reset_class_timings(ctx.h))
+        ctx.check(L.rchem_set_profiling(ctx.h, 1))
         torch.cuda.synchronize()
-        say(f"warmup step {i + 1}/{args.warmup} done")
-    ctx.check(L.rchem_reset_class_timings(ctx.h))
-    ctx.check(L.rchem_set_profiling(ctx.h, 1))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    say(f"{args.steps} timed steps done")
-    ctx.check(L.rchem_set_profiling(ctx.h, 0))
-    st = rb.jk_last_stats(B)
-    timings = (ClassTiming * 64)()
-    cnt = ctypes.c_int()
-    ctx.check(L.rchem_jk_class_timings(ctx.h, timings, 64, ctypes.byref(cnt)))
-    classes = []
-    for i in range(cnt.value):
-        t = timings[i]
-        avg = t.ms / args.steps   # per build (a large build launches a class once per pass)
-        classes.append({"class": f"({t.la}{t.lb}|{t.lc}{t.ld})" + (" fp32" if t.fp32 else ""),
-                        "avg_ms": avg, "launches_per_build": t.launches / args.steps,
-                        "quartets": t.quartets, "prim_quartets": t.primitive_quartets,
-                        "flops": t.flops,
-                        "tflops": t.flops / (avg * 1e-3) / 1e12 if avg > 0 else 0.0})
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        say(f"{prec_name}: {args.steps} timed steps done")
+        ctx.check(L.rchem_set_profiling(ctx.h, 0))
+        st = rb.jk_last_stats(B)
+        timings = (ClassTiming * 64)()
+        cnt = ctypes.c_int()
+        ctx.check(L.rchem_jk_class_timings(ctx.h, timings, 64, ctypes.byref(cnt)))
+        classes = []
+        for i in range(cnt.value):
+            t = timings[i]
+            avg = t.ms / args.steps   # per build (a large build launches a class once per pass)
+            classes.append({"class": f"({t.la}{t.lb}|{t.lc}{t.ld})" + (" fp32" if t.fp32 else ""),
+                            "avg_ms": avg, "launches_per_build": t.launches / args.steps,
+                            "quartets": t.quartets, "prim_quartets": t.primitive_quartets,
+                            "flops": t.flops,
+                            "tflops": t.flops / (avg * 1e-3) / 1e12 if avg > 0 else 0.0})
+        ms = elapsed / args.steps * 1e3
+        return {"ms": ms, "st": st, "classes": classes,
+                "quartets_per_s": st.quartets / (ms * 1e-3),
+                "tflops": st.flops / (ms * 1e-3) / 1e12}
+
+    precisions = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    if args.workload == "eri2000":
+        precisions = ["fp64"]
+    runs = {p: measure(p) for p in precisions}
+    main_run = runs["mixed"] if "mixed" in runs else runs["fp64"]
+    st = main_run["st"]
+    classes = main_run["classes"]
+    ms = main_run["ms"]
     dom = max(classes, key=lambda c: c["avg_ms"])
-    traffic = measured_traffic(dom["class"], args.precision)
     dom_fp32 = dom["class"].endswith("fp32")
     dom_peak = FP32_PEAK_TFLOPS if dom_fp32 else FP64_PEAK_TFLOPS
-    ms = elapsed / args.steps * 1e3
-    value = st.quartets / (ms * 1e-3)
-    total_tflops = st.flops / (ms * 1e-3) / 1e12
+    traffic = measured_traffic(dom["class"], "mixed" if "mixed" in runs else "fp64")
     out = {
         "metric": "Fock-build ms & ERI quartets/s, mixed vs FP64 on GPU at |dE|<1e-6 Eh",
-        "value": value,
+        "value": main_run["quartets_per_s"],
         "unit": "ERI shell quartets/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -202,17 +224,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == PREC_FP64 else "f64+f32",
+        "dtype": "f64+f32" if "mixed" in runs else "f64",
         "data": "synthetic",
         "config": {"workload": workload, "nbf": n, "shells": len(B.shells()[1]),
-                   "precision": args.precision},
+                   "precision": "mixed" if "mixed" in runs else "fp64"},
         "quartets": st.quartets,
         "quartets_fp32": st.quartets_fp32,
         "tiles": st.tiles,
         "lane_fill": st.quartets / max(st.tiles * 64.0, 1.0),
         "primitive_quartets": st.primitive_quartets,
         "integrals": st.integrals,
-        "fock_tflops": total_tflops,
+        "fock_tflops": main_run["tflops"],
         "roofline": {"bound": "mfma", "achieved": dom["tflops"], "peak": dom_peak,
                      "unit": "TFLOP/s", "frac": dom["tflops"] / dom_peak,
                      "traffic": traffic, "kernel": dom["class"],
@@ -220,9 +242,42 @@ def main():
                               "FP64 VALU kernel; MI355X FP64 vector peak = FP64 matrix peak")},
         "classes": classes,
     }
+    if "mixed" in runs and "fp64" in runs:
+        f = runs["fp64"]
+        out["fp64"] = {"ms_per_step": f["ms"], "value": f["quartets_per_s"],
+                       "fock_tflops": f["tflops"], "classes": f["classes"]}
+        out["speedup_mixed_vs_fp64"] = f["ms"] / ms
     if args.workload == "eri2000":
         out["metric"] = "ERI shell quartets/s per (la lb|lc ld) class, synthetic 2000-shell microbench"
         out["roofline"]["traffic"] = None
+    if args.workload == "h2o64" and not args.no_scf:
+        scf = {}
+        for p in ("fp64", "mixed"):
+            o = default_opts(precision=PREC_FP64 if p == "fp64" else PREC_MIXED,
+                             schwarz_thresh=1e-12)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            res, Dc = rb.scf(B, xyz, Z, nocc=nocc_of(Z), thresh_e=1e-10, diis=8, opts=o)
+            scf[p] = {"e_total": res.e_total, "iterations": res.iterations,
+                      "converged": bool(res.converged), "fp64_from": res.fp64_from,
+                      "seconds": time.perf_counter() - t0}
+            say(f"SCF {p}: {res.iterations} iterations, E = {res.e_total:.10f}")
+            if p == "fp64":
+                D_conv = Dc
+        scf["delta_e"] = abs(scf["mixed"]["e_total"] - scf["fp64"]["e_total"])
+        # the mixed Fock build itself at the converged density:
+        # E = sum D o (H + F), F = H + 2J - K, mixed vs all-FP64 J and K
+        Hc = T + V
+        e_build = {}
+        for p in ("fp64", "mixed"):
+            Jm, Km = np.zeros((n, n)), np.zeros((n, n))
+            rb.JK_direct(Jm, Km, B, D_conv, default_opts(
+                precision=PREC_FP64 if p == "fp64" else PREC_MIXED, schwarz_thresh=1e-12))
+            e_build[p] = float(np.sum(D_conv * (2.0 * Hc + 2.0 * Jm - Km)))
+        scf["delta_e_fock_build"] = abs(e_build["mixed"] - e_build["fp64"])
+        scf["note"] = ("DIIS(8); mixed switches to all-FP64 J/K once |dE| < 1e-7 Eh; "
+                       "convergence |dE| < 1e-10 Eh and max|FDS-SDF| < 1e-8")
+        out["scf"] = scf
     if not args.no_cpu_baseline and args.workload == "h2o64":
         out["cpu_baseline"] = cpu_baseline(Z, xyz, D, st.quartets, args.cpu_seconds)
     print(json.dumps(out), flush=True)

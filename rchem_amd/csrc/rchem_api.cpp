@@ -1780,6 +1780,20 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
   res->converged = 0;
   res->iterations = 0;
   int it = 0;
+  // Mixed precision: FP32 quartets until the energy changes by less than
+  // kSwitch per iteration, FP64 for every quartet from then on, so the last
+  // iterations (and the converged energy) see the all-FP64 Fock matrix.
+  constexpr double kSwitch = 1e-7;
+  rchem_opts jk = opts->jk;
+  res->fp64_from = jk.precision == RCHEM_PREC_MIXED ? -1 : 0;
+  auto maybe_switch = [&](double de_) {
+    if (jk.precision == RCHEM_PREC_MIXED && std::fabs(de_) < kSwitch) {
+      jk.precision = RCHEM_PREC_FP64;
+      res->fp64_from = it + 1;
+      return true;
+    }
+    return false;
+  };
   if (opts->diis > 0) {
     // Pulay DIIS: extrapolate F from the last k Fock matrices so that the
     // orthogonalised commutators X^T (FDS - SDF) X combine to the smallest norm.
@@ -1795,7 +1809,7 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
     double e_prev = 0.0;
     const rocblas_int nn = static_cast<rocblas_int>(n2);
     for (; it < opts->max_iter; ++it) {
-      TRY(jk_device(ctx, dD.as<double>(), dJ.as<double>(), dK.as<double>(), &opts->jk));
+      TRY(jk_device(ctx, dD.as<double>(), dJ.as<double>(), dK.as<double>(), &jk));
       HIPC(fock_launch(dH.as<double>(), dJ.as<double>(), dK.as<double>(), dF.as<double>(), n2,
                        ctx->stream));
       double e_cur = 0.0;
@@ -1831,7 +1845,12 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
       const double de_ = e_cur - e_prev;
       if (opts->verbose) std::printf("%4d %20.12f %20.12f %12.3e\n", it, e_cur, de_, errmax);
       e_new = e_cur;
-      if (it > 0 && std::fabs(de_) < opts->thresh_e && errmax < errtol) {
+      if (it > 0 && maybe_switch(de_)) {
+        e_prev = e_cur;
+        continue;  // the next Fock matrix is all-FP64; keep the DIIS history
+      }
+      if (it > 0 && jk.precision == RCHEM_PREC_FP64 && std::fabs(de_) < opts->thresh_e &&
+          errmax < errtol) {
         if (opts->verbose) std::printf("Convergence achieved!\n");
         res->converged = 1;
         ++it;
@@ -1882,7 +1901,7 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
     }
   }
   for (; opts->diis <= 0 && it < opts->max_iter; ++it) {
-    TRY(jk_device(ctx, dD.as<double>(), dJ.as<double>(), dK.as<double>(), &opts->jk));
+    TRY(jk_device(ctx, dD.as<double>(), dJ.as<double>(), dK.as<double>(), &jk));
     HIPC(fock_launch(dH.as<double>(), dJ.as<double>(), dK.as<double>(), dF.as<double>(), n2,
                      ctx->stream));
     if (dconv > 0.0)
@@ -1905,6 +1924,7 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
                      hipMemcpyDeviceToHost));
       ddmax = std::fabs(ddmax);
     }
+    if (maybe_switch(de_)) continue;
     if (std::fabs(de_) < opts->thresh_e && ddmax <= dconv) {
       if (opts->verbose) std::printf("Convergence achieved!\n");
       res->converged = 1;
