@@ -93,6 +93,81 @@ def measured_traffic(kernel, precision):
     return None if k is None else k["fetch_bytes"] + k["write_bytes"]
 
 
+def gradient_workload(args, ctx, t_start):
+    """(H2O)_32 6-31G (416 functions, 288 coordinates): forward SCF time, the
+    analytic gradient's time, and central differences (h = 1e-4 bohr) of the
+    GPU energy on sampled coordinates."""
+    from rchem_amd import basis as rb
+    from rchem_amd._lib import default_opts
+    from rchem_amd.inputs import nocc_of, water_cluster
+    import torch
+
+    def say(msg):
+        print(f"[bench] {msg} ({time.perf_counter() - t_start:.1f} s)", file=sys.stderr, flush=True)
+
+    Z, xyz = water_cluster(dims=(2, 4, 4), seed=32)
+    nocc = nocc_of(Z)
+    o = default_opts(schwarz_thresh=1e-12)
+
+    def energy(p):
+        B = rb.Basis.new(Z, p, "6-31G", ctx=ctx)
+        res, _ = rb.scf(B, p, Z, nocc=nocc, thresh_e=1e-11, diis=8, opts=o)
+        assert res.converged
+        return res.e_total
+
+    B = rb.Basis.new(Z, xyz, "6-31G", ctx=ctx)
+    energy(xyz)  # warm-up (pair layout, programs, kernels)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res, _ = rb.scf(B, xyz, Z, nocc=nocc, thresh_e=1e-11, diis=8, opts=o)
+    t_scf = time.perf_counter() - t0
+    say(f"forward SCF {res.iterations} iterations, {t_scf:.3f} s")
+    t0 = time.perf_counter()
+    resg, g = rb.rhf_gradient(B, xyz, Z, nocc=nocc, thresh_e=1e-11, diis=8, opts=o)
+    t_grad = time.perf_counter() - t0
+    say(f"gradient call {t_grad:.3f} s (includes its own SCF, {resg.iterations} iterations)")
+    ncoord = 3 * len(Z)
+    if args.fd_coords == "all":
+        coords = list(range(ncoord))
+    else:
+        coords = sorted(np.random.default_rng(32).choice(ncoord, int(args.fd_coords),
+                                                         replace=False).tolist())
+    h = 1e-4
+    worst = 0.0
+    for c in coords:
+        a, x = divmod(c, 3)
+        xp, xm = xyz.copy(), xyz.copy()
+        xp[a, x] += h
+        xm[a, x] -= h
+        fd = (energy(xp) - energy(xm)) / (2 * h)
+        worst = max(worst, abs(fd - g[a, x]))
+        say(f"coordinate {c}: analytic {g[a, x]:+.9f} FD {fd:+.9f}")
+    out = {
+        "metric": "RHF nuclear gradient, (H2O)_32 6-31G: analytic adjoint time vs forward SCF, "
+                  "max |analytic - central FD| (Eh/bohr)",
+        "value": t_grad - t_scf,
+        "unit": "s (gradient beyond its SCF)",
+        "higher_is_better": False,
+        "n_gpus": 1,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "(H2O)_32 6-31G, seed 32, 2x4x4 lattice 3.1 A, DIIS(8), "
+                               "Schwarz 1e-12 (gradient quartets 1e-14)", "nbf": B.nbf,
+                   "atoms": len(Z)},
+        "forward_scf_s": t_scf,
+        "scf_iterations": res.iterations,
+        "gradient_call_s": t_grad,
+        "gradient_over_scf": (t_grad - t_scf) / t_scf,
+        "e_total": res.e_total,
+        "max_abs_gradient": float(np.max(np.abs(g))),
+        "fd_coordinates": len(coords),
+        "fd_step_bohr": h,
+        "max_abs_fd_error": worst,
+        "net_force": float(np.max(np.abs(g.sum(axis=0)))),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,9 +178,13 @@ def main():
                     help="Fock builds to time (profiling runs take one)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", choices=["h2o64", "eri2000"], default="h2o64",
+    ap.add_argument("--fd-coords", default="6",
+                    help="grad32 workload: coordinates checked by central differences "
+                         "(a count sampled with seed 32, or 'all')")
+    ap.add_argument("--workload", choices=["h2o64", "eri2000", "grad32"], default="h2o64",
                     help="h2o64: (H2O)_64 6-31G Fock build (default); eri2000: the synthetic "
-                         "2000-shell s/p/d ERI microbenchmark, per class")
+                         "2000-shell s/p/d ERI microbenchmark, per class; grad32: analytic "
+                         "RHF gradient of (H2O)_32 6-31G against the forward SCF")
     ap.add_argument("--shells", type=int, default=2000,
                     help="eri2000 workload: number of shells (the cube grows to keep the density)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -124,6 +203,10 @@ def main():
 
     t_start = time.perf_counter()
     ctx = rb.Context(0)
+    if args.workload == "grad32":
+        gradient_workload(args, ctx, t_start)
+        ctx.close()
+        return
     if args.workload == "h2o64":
         Z, xyz = water_cluster(dims=(4, 4, 4), seed=64)
         B = rb.Basis.new(Z, xyz, "6-31G", ctx=ctx)
