@@ -83,6 +83,21 @@ SIGNATURES = {
     "rchem_jk_class_timings": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ClassTiming),
                                               ctypes.c_int, _ip]),
     "rchem_scf_opts_default": (ctypes.c_int, [ctypes.POINTER(SCFOpts)]),
+    # cints.h compatibility exports (include/rchem_cints_compat.h)
+    "coulomb_repulsion": (ctypes.c_double, ([ctypes.c_double] * 4 + [ctypes.c_int] * 3 +
+                                            [ctypes.c_double]) * 4),
+    "contr_coulomb": (ctypes.c_double, ([ctypes.c_int, _dp, _dp, _dp] + [ctypes.c_double] * 3 +
+                                        [ctypes.c_int] * 3) * 4),
+    "overlap": (ctypes.c_double, ([ctypes.c_double] + [ctypes.c_int] * 3 +
+                                  [ctypes.c_double] * 3) * 2),
+    "kinetic": (ctypes.c_double, ([ctypes.c_double] + [ctypes.c_int] * 3 +
+                                  [ctypes.c_double] * 3) * 2),
+    "nuclear_attraction": (ctypes.c_double, ([ctypes.c_double] * 4 + [ctypes.c_int] * 3 +
+                                             [ctypes.c_double]) * 2 + [ctypes.c_double] * 3),
+    "Fgamma": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
+    "rchem_batch_scf": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip, _ip, _dp,
+                                       ctypes.c_char_p, ctypes.POINTER(SCFOpts),
+                                       ctypes.POINTER(SCFResult)]),
     "rchem_rhf_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp,
                                           ctypes.POINTER(SCFOpts), ctypes.POINTER(SCFResult),
                                           _dp]),

@@ -223,6 +223,29 @@ def scf(basis: Basis, atomcoords, atomnos, nocc, thresh_e=1e-11, max_iter=1024,
     return res, D
 
 
+def batch_scf(molecules, basis_set_name, thresh_e=1e-11, max_iter=1024, diis=0,
+              opts: Opts | None = None, ctx=None):
+    """Many independent molecules [(atomnos, atomcoords), ...] solved together
+    on the device; returns a list of SCFResult (the context's basis is cleared)."""
+    ctx = ctx or default_context()
+    natom = np.array([len(z) for z, _ in molecules], dtype=np.int32)
+    Z = np.ascontiguousarray(np.concatenate([np.asarray(z) for z, _ in molecules]), dtype=np.int32)
+    xyz = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1, 3)
+                                               for _, x in molecules]))
+    o = SCFOpts()
+    lib().rchem_scf_opts_default(ctypes.byref(o))
+    o.max_iter = int(max_iter)
+    o.thresh_e = float(thresh_e)
+    o.diis = int(diis)
+    o.jk = opts or default_opts()
+    res = (SCFResult * len(molecules))()
+    rc = lib().rchem_batch_scf(ctx.h, len(molecules), _iptr(natom), _iptr(Z), _dptr(xyz),
+                               basis_set_name.encode(), ctypes.byref(o), res)
+    if rc not in (0, 6):
+        ctx.check(rc)
+    return list(res)
+
+
 def rhf_gradient(basis: Basis, atomcoords, atomnos, nocc, thresh_e=1e-11, max_iter=1024,
                  opts: Opts | None = None, diis=0):
     """Analytic RHF nuclear gradient (Eh/bohr) after rchem's SCF on the device
@@ -247,6 +270,6 @@ def rhf_gradient(basis: Basis, atomcoords, atomnos, nocc, thresh_e=1e-11, max_it
 
 
 __all__ = ["Context", "Basis", "S", "T", "V", "JK_direct", "JK_inmem", "build_I", "scf",
-           "rhf_gradient",
+           "rhf_gradient", "batch_scf",
            "eri_shell_quartets",
            "one_electron", "schwarz", "jk_last_stats", "BOYS_EXACT", "BOYS_COMPAT"]

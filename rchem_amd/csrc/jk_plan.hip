@@ -77,6 +77,11 @@ __global__ void plan_kernel(const PlanArgs a) {
     const int i = pc.bra_begin + local / pc.nbk;
     const int4 bk = a.buckets[pc.bk_off + local % pc.nbk];
     int lo = bk.x, hi = bk.y;
+    if (a.segs) {  // the bra's molecule only
+      const int2 sg = a.segs[static_cast<size_t>(pc.bk_off + local % pc.nbk) * a.nmol + a.pair_mol[i]];
+      lo = sg.x;
+      hi = sg.y;
+    }
     if (pc.same) hi = (lo > i) ? lo : min(hi, i + 1);
     const double qi = a.Q[i];
     if (a.tau > 0.0) hi = (qi > 0.0) ? first_below(a.Q, lo, hi, a.tau / qi) : lo;

@@ -36,6 +36,8 @@ struct OneElArgs {
   double* S;
   double* T;
   double* V;
+  const int* shell_mol;   // several molecules: the molecule of each shell, else null
+  const int2* mol_atoms;  // nuclei [begin, end) of each molecule
 };
 
 constexpr int kLmax1 = 2;              // highest l served
@@ -147,7 +149,13 @@ __global__ __launch_bounds__(128) void one_electron_kernel(const OneElArgs a) {
         }
       // nuclear attraction: R_tuv via the Hermite recursion, one centre at a time
       const int L = la + lb;
-      for (int c = 0; c < a.natom; ++c) {
+      int c0 = 0, c1 = a.natom;
+      if (a.shell_mol) {  // the pair's own molecule's nuclei
+        const int2 r = a.mol_atoms[a.shell_mol[pr.x]];
+        c0 = r.x;
+        c1 = r.y;
+      }
+      for (int c = c0; c < c1; ++c) {
         const double PC[3] = {P[0] - a.atom_xyz[3 * c], P[1] - a.atom_xyz[3 * c + 1],
                               P[2] - a.atom_xyz[3 * c + 2]};
         const double T = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);

@@ -63,6 +63,21 @@ hipError_t fock_launch(const double* H, const double* J, const double* K, double
 hipError_t energy_launch(const double* D, const double* H, const double* F, double* out,
                          size_t n2, hipStream_t s);
 hipError_t antisym_launch(const double* A, double* E, int n, hipStream_t s);
+hipError_t batch_pack(const double* A, size_t n, const int* f0, const int* nb, int nmax, int nbatch,
+                      double pad, double* out, hipStream_t s);
+hipError_t batch_unpack(const double* in, int nmax, const int* f0, const int* nb, int nbatch,
+                        size_t n, double* A, hipStream_t s);
+hipError_t batch_mask_cols(double* C, int nmax, const int* nocc, int nbatch, hipStream_t s);
+hipError_t batch_inv_sqrt_cols(const double* U, const double* w, int nmax, int nbatch, double* out,
+                               hipStream_t s);
+hipError_t batch_energy(const double* D, const double* H, const double* F, size_t n, const int* f0,
+                        const int* nb, int nbatch, double* out, hipStream_t s);
+hipError_t batch_antisym(const double* A, double* E, int nmax, int nbatch, hipStream_t s);
+hipError_t batch_dots(const double* base, const long long* pairs, int npairs, size_t len,
+                      double* out, hipStream_t s);
+hipError_t batch_maxabs(const double* E, int nmax, int nbatch, double* out, hipStream_t s);
+hipError_t batch_combine(const double* Fs, size_t batch_stride, int m, int k, const double* c,
+                         int nmax, int nbatch, double* out, hipStream_t s);
 hipError_t scale_columns_launch(const double* U, const double* w, double* out, int n,
                                 hipStream_t s);
 
@@ -84,6 +99,8 @@ struct OneElArgs {
   double* S;
   double* T;
   double* V;
+  const int* shell_mol;   // several molecules: the molecule of each shell, else null
+  const int2* mol_atoms;  // nuclei [begin, end) of each molecule
 };
 struct OneElGradArgs {
   const ShellDev* shells;
@@ -191,6 +208,14 @@ struct rchem_ctx {
   int nbf = 0;
   bool have_basis = false;
 
+  // Several independent molecules in one basis (batched SCF): molecule of
+  // every shell; only pairs and quartets within one molecule exist.
+  int nmol = 1;
+  std::vector<int> shell_mol;                   // empty: one molecule
+  std::vector<int> pair_mol;                    // layout order
+  std::vector<std::vector<int2>> bucket_segs;   // [pair class][bucket * nmol + molecule]
+  DevBuf pair_mol_d, plan_segs_d;
+
   // Schwarz bounds of every shell pair (a >= b), computed once per basis
   std::vector<PairHost> all_pairs;
   bool have_schwarz = false;
@@ -264,6 +289,12 @@ struct rchem_ctx {
 };
 
 static std::string g_create_error;
+
+namespace rchem {
+// error text reported by rchem_last_error(NULL) (context creation and the
+// context-free compatibility functions)
+void set_global_error(const std::string& msg) { g_create_error = msg; }
+}  // namespace rchem
 
 #define HIPC(expr)                                                                        \
   do {                                                                                    \
@@ -403,11 +434,16 @@ int layout_pairs(rchem_ctx* ctx, const std::vector<PairHost>& sel, double prim_e
     q.np = static_cast<int>(tmp.size());
     v.push_back(q);
   }
-  std::stable_sort(v.begin(), v.end(), [](const PairHost& x, const PairHost& y) {
+  const bool multi = !ctx->shell_mol.empty();
+  auto mol_of = [&](const PairHost& p) { return multi ? ctx->shell_mol[p.a] : 0; };
+  std::stable_sort(v.begin(), v.end(), [&](const PairHost& x, const PairHost& y) {
     if (x.cls != y.cls) return x.cls < y.cls;
     if (x.np != y.np) return x.np > y.np;
+    if (mol_of(x) != mol_of(y)) return mol_of(x) < mol_of(y);
     return x.Q > y.Q;
   });
+  ctx->pair_mol.resize(v.size());
+  for (size_t k = 0; k < v.size(); ++k) ctx->pair_mol[k] = mol_of(v[k]);
   ctx->pairs.assign(v.size(), PairRec{});
   ctx->pairQ.resize(v.size());
   ctx->pair_shells.resize(v.size());
@@ -454,6 +490,31 @@ int layout_pairs(rchem_ctx* ctx, const std::vector<PairHost>& sel, double prim_e
     pos = end;
   }
   ctx->nprims = prims.size();
+  // per bucket, each molecule's pairs [begin, end) (contiguous: sorted by molecule)
+  ctx->bucket_segs.assign(kNumPairClasses, {});
+  if (multi) {
+    const int nm = ctx->nmol;
+    for (int c = 0; c < kNumPairClasses; ++c) {
+      auto& sg = ctx->bucket_segs[c];
+      sg.assign(ctx->buckets[c].size() * static_cast<size_t>(nm), int2{0, 0});
+      for (size_t b = 0; b < ctx->buckets[c].size(); ++b) {
+        const Bucket& bk = ctx->buckets[c][b];
+        for (int m = 0; m < nm; ++m) sg[b * nm + m] = int2{bk.begin, bk.begin};
+        int k = bk.begin;
+        while (k < bk.end) {
+          const int m = ctx->pair_mol[k];
+          int e = k;
+          while (e < bk.end && ctx->pair_mol[e] == m) ++e;
+          sg[b * nm + m] = int2{k, e};
+          k = e;
+        }
+      }
+    }
+    HIPC(ctx->pair_mol_d.ensure(sizeof(int) * std::max<size_t>(ctx->pair_mol.size(), 1)));
+    if (!ctx->pair_mol.empty())
+      HIPC(hipMemcpyAsync(ctx->pair_mol_d.p, ctx->pair_mol.data(),
+                          sizeof(int) * ctx->pair_mol.size(), hipMemcpyHostToDevice, ctx->stream));
+  }
   HIPC(ctx->alpha_d.ensure(sizeof(double) * std::max<size_t>(alphas.size(), 1)));
   if (!alphas.empty())
     HIPC(hipMemcpyAsync(ctx->alpha_d.p, alphas.data(), sizeof(double) * alphas.size(),
@@ -584,8 +645,10 @@ int compute_schwarz(rchem_ctx* ctx) {
   const int ns = static_cast<int>(ctx->shells.size());
   std::vector<PairHost> all;
   all.reserve(static_cast<size_t>(ns) * (ns + 1) / 2);
+  const bool multi = !ctx->shell_mol.empty();
   for (int a = 0; a < ns; ++a)
     for (int b = 0; b <= a; ++b) {
+      if (multi && ctx->shell_mol[a] != ctx->shell_mol[b]) continue;
       int x = a, y = b;
       if (ctx->shells[y].l > ctx->shells[x].l) std::swap(x, y);
       all.push_back(PairHost{x, y, pair_class(ctx->shells[x].l, ctx->shells[y].l),
@@ -698,6 +761,11 @@ void for_each_task(const rchem_ctx* ctx, double tau, Fn f) {
         for (size_t b = 0; b < ctx->buckets[Y].size(); ++b) {
           const Bucket& bk = ctx->buckets[Y][b];
           int lo = bk.begin, hi = bk.end;
+          if (!ctx->shell_mol.empty()) {  // the bra's molecule only
+            const int2 sg = ctx->bucket_segs[Y][b * ctx->nmol + ctx->pair_mol[i]];
+            lo = sg.x;
+            hi = sg.y;
+          }
           if (X == Y) hi = (lo > i) ? lo : std::min(hi, i + 1);
           if (tau > 0.0) {
             if (Q[i] <= 0.0) {
@@ -781,6 +849,7 @@ int prepare_plan(rchem_ctx* ctx, double tau) {
   ctx->passes.clear();
   ctx->qclasses.clear();
   std::vector<int4> bks;
+  std::vector<int2> segs;
   std::vector<int> cls_bk_off;
   for (int X = 0; X < kNumPairClasses; ++X)
     for (int Y = 0; Y <= X; ++Y) {
@@ -789,6 +858,8 @@ int prepare_plan(rchem_ctx* ctx, double tau) {
       if (nbra == 0 || nbk == 0) continue;
       cls_bk_off.push_back(static_cast<int>(bks.size()));
       for (const Bucket& bk : ctx->buckets[Y]) bks.push_back(make_int4(bk.begin, bk.end, bk.np, 0));
+      if (!ctx->shell_mol.empty())
+        segs.insert(segs.end(), ctx->bucket_segs[Y].begin(), ctx->bucket_segs[Y].end());
       ctx->qclasses.push_back(make_qclass(X, Y));
     }
   // tiles reserved per bra pair (one partial tile more per task covers the
@@ -935,6 +1006,11 @@ int prepare_plan(rchem_ctx* ctx, double tau) {
   if (!bks.empty())
     HIPC(hipMemcpyAsync(ctx->plan_buckets_d.p, bks.data(), sizeof(int4) * bks.size(),
                         hipMemcpyHostToDevice, ctx->stream));
+  if (!segs.empty()) {
+    HIPC(ctx->plan_segs_d.ensure(sizeof(int2) * segs.size()));
+    HIPC(hipMemcpyAsync(ctx->plan_segs_d.p, segs.data(), sizeof(int2) * segs.size(),
+                        hipMemcpyHostToDevice, ctx->stream));
+  }
   if (!ctx->pairQ.empty())
     HIPC(hipMemcpyAsync(ctx->plan_q_d.p, ctx->pairQ.data(), sizeof(double) * ctx->pairQ.size(),
                         hipMemcpyHostToDevice, ctx->stream));
@@ -1083,6 +1159,11 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
     pa.mid = ctx->plan_mid.as<int>();
     pa.hi = ctx->plan_hi.as<int>();
     pa.stats = ctx->plan_stats.as<double>() + 4 * static_cast<size_t>(ps.seg_begin);
+    if (!ctx->shell_mol.empty()) {
+      pa.segs = ctx->plan_segs_d.as<int2>();
+      pa.nmol = ctx->nmol;
+      pa.pair_mol = ctx->pair_mol_d.as<int>();
+    }
     PlanBuffers pb{};
     pb.off64 = ctx->plan_off64.as<int>();
     pb.off32 = ctx->plan_off32.as<int>();
@@ -1243,6 +1324,10 @@ const char* rchem_last_error(const rchem_ctx* ctx) {
 
 int rchem_basis_upload(rchem_ctx* ctx, int nshell, const double* xyz, const int* l,
                        const int* nprim, const double* exps, const double* coefs) {
+  if (ctx) {
+    ctx->shell_mol.clear();
+    ctx->nmol = 1;
+  }
   if (!ctx) return RCHEM_ERR_INVALID;
   if (nshell <= 0 || !xyz || !l || !nprim || !exps || !coefs)
     return ctx->fail(RCHEM_ERR_INVALID, "rchem_basis_upload: bad arguments");
@@ -1285,6 +1370,8 @@ int rchem_basis_from_atoms(rchem_ctx* ctx, int natom, const int* Z, const double
   ctx->shells.clear();
   ctx->exps.clear();
   ctx->coefs.clear();
+  ctx->shell_mol.clear();
+  ctx->nmol = 1;
   // per atom, per basis-set shell, per l in its angular_momentum list (basis.rs:186-205)
   for (int a = 0; a < natom; ++a) {
     bool found = false;
@@ -1952,6 +2039,404 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
 }  // namespace
 
 extern "C" {
+
+}  // extern "C"
+
+namespace {
+
+constexpr int kBatchMaxNbf = 4096;    // basis functions per chunk of molecules
+constexpr double kBatchPadF = 1.0e4;  // Fock diagonal of a padded block's unused rows
+
+// One chunk of the batched SCF: molecules [m0, m1) of the caller's lists as one
+// multi-molecule basis.  zoff/aoff: offsets of molecule m0 in Z and in atoms.
+int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
+                const double* xyz, int aoff, const std::string& want, const rchem_scf_opts* opts,
+                rchem_scf_result* res) {
+  const int B = m1 - m0;
+  auto lower = [](std::string s) {
+    for (auto& c : s) c = static_cast<char>(std::tolower(static_cast<unsigned char>(c)));
+    return s;
+  };
+  ctx->shells.clear();
+  ctx->exps.clear();
+  ctx->coefs.clear();
+  ctx->shell_mol.clear();
+  std::vector<int2> mol_atoms(B);
+  std::vector<int> nocc(B), mfirst_shell(B);
+  std::vector<double> Zc, xyzc;
+  int a = aoff;
+  for (int b = 0; b < B; ++b) {
+    mol_atoms[b] = int2{static_cast<int>(Zc.size()), 0};
+    mfirst_shell[b] = static_cast<int>(ctx->shells.size());
+    int ne = 0;
+    for (int k = 0; k < natom[m0 + b]; ++k, ++a) {
+      bool found = false;
+      for (int r = 0; r < kBasisNumRows; ++r) {
+        const BasisShellRow& row = kBasisRows[r];
+        if (lower(row.set) != want || row.Z != Z[a]) continue;
+        found = true;
+        for (int q = 0; q < 2; ++q) {
+          const int l = row.am[q];
+          if (l < 0) continue;
+          Shell sh;
+          sh.l = l;
+          sh.nprim = row.nprim;
+          sh.p0 = static_cast<int>(ctx->exps.size());
+          for (int x = 0; x < 3; ++x) sh.xyz[x] = xyz[3 * a + x];
+          for (int p = 0; p < row.nprim; ++p) {
+            ctx->exps.push_back(row.exps[p]);
+            ctx->coefs.push_back(row.coefs[q][p]);
+          }
+          ctx->shells.push_back(sh);
+          ctx->shell_mol.push_back(b);
+        }
+      }
+      if (!found)
+        return ctx->fail(RCHEM_ERR_INVALID, "element Z=" + std::to_string(Z[a]) + " not in basis");
+      ne += Z[a];
+      Zc.push_back(Z[a]);
+      for (int x = 0; x < 3; ++x) xyzc.push_back(xyz[3 * a + x]);
+    }
+    mol_atoms[b].y = static_cast<int>(Zc.size());
+    if (ne % 2 != 0)
+      return ctx->fail(RCHEM_ERR_INVALID, "molecule " + std::to_string(m0 + b) +
+                                              " has an odd electron count");
+    nocc[b] = ne / 2;
+  }
+  ctx->nmol = B;
+  TRY(finish_basis(ctx));
+  const int n = ctx->nbf, ns = static_cast<int>(ctx->shells.size());
+  const size_t n2 = static_cast<size_t>(n) * n;
+  std::vector<int> f0(B), nb(B);
+  int nmax = 1;
+  for (int b = 0; b < B; ++b) {
+    f0[b] = ctx->shells[mfirst_shell[b]].f0;
+    const int end = b + 1 < B ? ctx->shells[mfirst_shell[b + 1]].f0 : n;
+    nb[b] = end - f0[b];
+    nmax = std::max(nmax, nb[b]);
+    if (nocc[b] > nb[b])
+      return ctx->fail(RCHEM_ERR_INVALID, "molecule with more electron pairs than functions");
+  }
+  hipStream_t st = ctx->stream;
+  rocblas_handle hb = ctx->blas;
+  // one-electron matrices of every molecule (pairs within a molecule, own nuclei)
+  DevBuf dS, dT, dV, dH, dF, dD, dJ, dK;
+  for (DevBuf* x : {&dS, &dT, &dV, &dH, &dF, &dD, &dJ, &dK}) {
+    HIPC(x->ensure(n2 * sizeof(double)));
+    HIPC(fill_launch(x->as<double>(), n2, 0.0, st));
+  }
+  {
+    std::vector<ShellDev> sh(ns);
+    for (int s2 = 0; s2 < ns; ++s2) {
+      for (int x = 0; x < 3; ++x) sh[s2].xyz[x] = ctx->shells[s2].xyz[x];
+      sh[s2].l = ctx->shells[s2].l;
+      sh[s2].nprim = ctx->shells[s2].nprim;
+      sh[s2].p0 = ctx->shells[s2].p0;
+      sh[s2].f0 = ctx->shells[s2].f0;
+    }
+    std::vector<int2> pr;
+    for (int x = 0; x < ns; ++x)
+      for (int y = 0; y <= x; ++y)
+        if (ctx->shell_mol[x] == ctx->shell_mol[y]) pr.push_back(make_int2(x, y));
+    DevBuf dsh, dex, dcc, dpr, dax, daz, dsm, dma;
+    HIPC(dsh.ensure(sizeof(ShellDev) * ns));
+    HIPC(dex.ensure(sizeof(double) * ctx->exps.size()));
+    HIPC(dcc.ensure(sizeof(double) * ctx->ccoef.size()));
+    HIPC(dpr.ensure(sizeof(int2) * pr.size()));
+    HIPC(dax.ensure(sizeof(double) * xyzc.size()));
+    HIPC(daz.ensure(sizeof(double) * Zc.size()));
+    HIPC(dsm.ensure(sizeof(int) * ns));
+    HIPC(dma.ensure(sizeof(int2) * B));
+    HIPC(hipMemcpyAsync(dsh.p, sh.data(), sizeof(ShellDev) * ns, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(dex.p, ctx->exps.data(), sizeof(double) * ctx->exps.size(), hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(dcc.p, ctx->ccoef.data(), sizeof(double) * ctx->ccoef.size(), hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(dpr.p, pr.data(), sizeof(int2) * pr.size(), hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(dax.p, xyzc.data(), sizeof(double) * xyzc.size(), hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(daz.p, Zc.data(), sizeof(double) * Zc.size(), hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(dsm.p, ctx->shell_mol.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(dma.p, mol_atoms.data(), sizeof(int2) * B, hipMemcpyHostToDevice, st));
+    OneElArgs oa{};
+    oa.shells = dsh.as<ShellDev>();
+    oa.exps = dex.as<double>();
+    oa.ccoef = dcc.as<double>();
+    oa.pairs = dpr.as<int2>();
+    oa.npairs = static_cast<int>(pr.size());
+    oa.nbf = n;
+    oa.natom = static_cast<int>(Zc.size());
+    oa.atom_xyz = dax.as<double>();
+    oa.atom_z = daz.as<double>();
+    oa.boys_table = ctx->boys_d.as<double>();
+    oa.S = dS.as<double>();
+    oa.T = dT.as<double>();
+    oa.V = dV.as<double>();
+    oa.shell_mol = dsm.as<int>();
+    oa.mol_atoms = dma.as<int2>();
+    HIPC(one_electron_launch(oa, st));
+    HIPC(hipStreamSynchronize(st));
+  }
+  const double one = 1.0, zero = 0.0;
+  const rocblas_int nn2 = static_cast<rocblas_int>(n2);
+  HIPC(hipMemcpyAsync(dH.p, dT.p, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+  BLASC(rocblas_daxpy(hb, nn2, &one, dV.as<double>(), 1, dH.as<double>(), 1));
+  // padded per-molecule batches
+  const size_t m2 = static_cast<size_t>(nmax) * nmax;
+  const rocblas_stride sm = static_cast<rocblas_stride>(m2);
+  DevBuf df0, dnb, dnocc, dSp, dX, dW, dT1, dFb, dC, dDb, dw, de, dinfo, dEm;
+  HIPC(df0.ensure(sizeof(int) * B));
+  HIPC(dnb.ensure(sizeof(int) * B));
+  HIPC(dnocc.ensure(sizeof(int) * B));
+  for (DevBuf* x : {&dSp, &dX, &dW, &dT1, &dFb, &dC, &dDb}) HIPC(x->ensure(sizeof(double) * m2 * B));
+  HIPC(dw.ensure(sizeof(double) * nmax * B));
+  HIPC(de.ensure(sizeof(double) * nmax * B));
+  HIPC(dinfo.ensure(sizeof(rocblas_int) * B));
+  HIPC(dEm.ensure(sizeof(double) * B));
+  HIPC(hipMemcpyAsync(df0.p, f0.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(dnb.p, nb.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(dnocc.p, nocc.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
+  auto gemm_b = [&](rocblas_operation ta, rocblas_operation tb, const double* A, const double* Bm,
+                    double* C) {
+    return rocblas_dgemm_strided_batched(hb, ta, tb, nmax, nmax, nmax, &one, A, nmax, sm, Bm,
+                                         nmax, sm, &zero, C, nmax, sm, B);
+  };
+  // Loewdin X_b = U_b lambda_b^-1/2 U_b^T (rchem.rs:44-51), padding = identity
+  HIPC(batch_pack(dS.as<double>(), n, df0.as<int>(), dnb.as<int>(), nmax, B, 1.0, dSp.as<double>(), st));
+  HIPC(hipMemcpyAsync(dT1.p, dSp.p, sizeof(double) * m2 * B, hipMemcpyDeviceToDevice, st));
+  BLASC(rocsolver_dsyevd_strided_batched(hb, rocblas_evect_original, rocblas_fill_upper, nmax,
+                                         dT1.as<double>(), nmax, sm, dw.as<double>(), nmax,
+                                         de.as<double>(), nmax, dinfo.as<rocblas_int>(), B));
+  HIPC(batch_inv_sqrt_cols(dT1.as<double>(), dw.as<double>(), nmax, B, dW.as<double>(), st));
+  BLASC(gemm_b(rocblas_operation_none, rocblas_operation_transpose, dW.as<double>(),
+               dT1.as<double>(), dX.as<double>()));
+  // density from a padded Fock batch: F' = X^T F X, eigh, C = X C', D = C_occ C_occ^T
+  auto density_b = [&](const double* Fb) -> int {
+    BLASC(gemm_b(rocblas_operation_none, rocblas_operation_none, Fb, dX.as<double>(), dT1.as<double>()));
+    BLASC(gemm_b(rocblas_operation_transpose, rocblas_operation_none, dX.as<double>(),
+                 dT1.as<double>(), dW.as<double>()));
+    BLASC(rocsolver_dsyevd_strided_batched(hb, rocblas_evect_original, rocblas_fill_upper, nmax,
+                                           dW.as<double>(), nmax, sm, dw.as<double>(), nmax,
+                                           de.as<double>(), nmax, dinfo.as<rocblas_int>(), B));
+    BLASC(gemm_b(rocblas_operation_none, rocblas_operation_none, dX.as<double>(), dW.as<double>(),
+                 dC.as<double>()));
+    HIPC(batch_mask_cols(dC.as<double>(), nmax, dnocc.as<int>(), B, st));
+    BLASC(gemm_b(rocblas_operation_none, rocblas_operation_transpose, dC.as<double>(),
+                 dC.as<double>(), dDb.as<double>()));
+    HIPC(batch_unpack(dDb.as<double>(), nmax, df0.as<int>(), dnb.as<int>(), B, n, dD.as<double>(), st));
+    return RCHEM_OK;
+  };
+  auto density_dense = [&](const double* Fd) -> int {
+    HIPC(batch_pack(Fd, n, df0.as<int>(), dnb.as<int>(), nmax, B, kBatchPadF, dFb.as<double>(), st));
+    return density_b(dFb.as<double>());
+  };
+  auto energies = [&](const double* Fd, std::vector<double>* e) -> int {
+    HIPC(batch_energy(dD.as<double>(), dH.as<double>(), Fd, n, df0.as<int>(), dnb.as<int>(), B,
+                      dEm.as<double>(), st));
+    e->resize(B);
+    HIPC(hipMemcpyAsync(e->data(), dEm.p, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    return RCHEM_OK;
+  };
+  TRY(density_dense(dH.as<double>()));  // core-Hamiltonian guess (rchem.rs:56-61)
+  std::vector<double> e_cur(B, 0.0), e_prev(B, 0.0), errmax(B, 0.0);
+  TRY(energies(dH.as<double>(), &e_cur));
+  std::vector<int> conv_at(B, -1);
+  rchem_opts jk = opts->jk;
+  int fp64_from = jk.precision == RCHEM_PREC_MIXED ? -1 : 0;
+  const int k = std::min(std::max(opts->diis, 0), 32);
+  DevBuf dFs, dEs, dDp, dE1, dpairs, ddots, dcoef, derr;
+  if (k > 0) {
+    HIPC(dFs.ensure(sizeof(double) * m2 * B * k));
+    HIPC(dEs.ensure(sizeof(double) * m2 * B * k));
+    HIPC(dDp.ensure(sizeof(double) * m2 * B));
+    HIPC(dE1.ensure(sizeof(double) * m2 * B));
+    HIPC(dpairs.ensure(sizeof(long long) * 2 * B * k));
+    HIPC(ddots.ensure(sizeof(double) * B * k));
+    HIPC(dcoef.ensure(sizeof(double) * B * k));
+    HIPC(derr.ensure(sizeof(double) * B));
+  }
+  std::vector<double> Bm(static_cast<size_t>(B) * k * k, 0.0);
+  int count = 0, it = 0;
+  bool all_conv = false;
+  for (; it < opts->max_iter; ++it) {
+    TRY(jk_device(ctx, dD.as<double>(), dJ.as<double>(), dK.as<double>(), &jk));
+    HIPC(fock_launch(dH.as<double>(), dJ.as<double>(), dK.as<double>(), dF.as<double>(), n2, st));
+    e_prev = e_cur;
+    if (k > 0) {
+      // energy of the current density with its own Fock matrix
+      TRY(energies(dF.as<double>(), &e_cur));
+      const int slot = count % k;
+      double* Fslot = dFs.as<double>() + m2 * B * slot;
+      double* Eslot = dEs.as<double>() + m2 * B * slot;
+      HIPC(batch_pack(dF.as<double>(), n, df0.as<int>(), dnb.as<int>(), nmax, B, kBatchPadF, Fslot, st));
+      HIPC(batch_pack(dD.as<double>(), n, df0.as<int>(), dnb.as<int>(), nmax, B, 0.0, dDp.as<double>(), st));
+      BLASC(gemm_b(rocblas_operation_none, rocblas_operation_none, Fslot, dDp.as<double>(), dT1.as<double>()));
+      BLASC(gemm_b(rocblas_operation_none, rocblas_operation_none, dT1.as<double>(), dSp.as<double>(), dW.as<double>()));
+      HIPC(batch_antisym(dW.as<double>(), dE1.as<double>(), nmax, B, st));
+      BLASC(gemm_b(rocblas_operation_transpose, rocblas_operation_none, dX.as<double>(), dE1.as<double>(), dT1.as<double>()));
+      BLASC(gemm_b(rocblas_operation_none, rocblas_operation_none, dT1.as<double>(), dX.as<double>(), Eslot));
+      ++count;
+      const int m = std::min(count, k);
+      std::vector<long long> prs(static_cast<size_t>(2) * B * m);
+      for (int b = 0; b < B; ++b)
+        for (int i = 0; i < m; ++i) {
+          prs[2 * (static_cast<size_t>(b) * m + i)] = static_cast<long long>(m2 * B * slot + m2 * b);
+          prs[2 * (static_cast<size_t>(b) * m + i) + 1] = static_cast<long long>(m2 * B * i + m2 * b);
+        }
+      HIPC(hipMemcpyAsync(dpairs.p, prs.data(), sizeof(long long) * prs.size(), hipMemcpyHostToDevice, st));
+      HIPC(batch_dots(dEs.as<double>(), dpairs.as<long long>(), B * m, m2, ddots.as<double>(), st));
+      HIPC(batch_maxabs(Eslot, nmax, B, derr.as<double>(), st));
+      std::vector<double> dots(static_cast<size_t>(B) * m);
+      HIPC(hipMemcpyAsync(dots.data(), ddots.p, sizeof(double) * dots.size(), hipMemcpyDeviceToHost, st));
+      HIPC(hipMemcpyAsync(errmax.data(), derr.p, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+      HIPC(hipStreamSynchronize(st));
+      for (int b = 0; b < B; ++b)
+        for (int i = 0; i < m; ++i) {
+          double* Bb = Bm.data() + static_cast<size_t>(b) * k * k;
+          Bb[slot * k + i] = Bb[i * k + slot] = dots[static_cast<size_t>(b) * m + i];
+        }
+    } else {
+      TRY(density_dense(dF.as<double>()));
+      TRY(energies(dF.as<double>(), &e_cur));  // rchem's sum D_new o (H + F(D_old))
+    }
+    double demax = 0.0;
+    int settled = 0;
+    for (int b = 0; b < B; ++b) {
+      const double d = std::fabs(e_cur[b] - e_prev[b]);
+      demax = std::max(demax, d);
+      if (d < 1e-7) ++settled;
+    }
+    if (opts->verbose) std::printf("%4d batch max|dE| %12.3e\n", it, demax);
+    // switch once 95 % of the molecules change by less than 1e-7 Eh (a molecule
+    // whose iterations stall must not keep the others in mixed precision), or
+    // after 64 iterations
+    if (it > 0 && jk.precision == RCHEM_PREC_MIXED && (20 * settled >= 19 * B || it >= 64)) {
+      jk.precision = RCHEM_PREC_FP64;  // convergence-triggered switch to all-FP64 quartets
+      fp64_from = it + 1;
+      if (k <= 0) continue;
+    }
+    all_conv = it > 0 && jk.precision != RCHEM_PREC_MIXED;
+    for (int b = 0; b < B; ++b) {
+      const bool ok = it > 0 && jk.precision != RCHEM_PREC_MIXED &&
+                      std::fabs(e_cur[b] - e_prev[b]) < opts->thresh_e &&
+                      (k <= 0 || errmax[b] < 1e-8);
+      if (ok && conv_at[b] < 0) conv_at[b] = it + 1;
+      if (!ok) all_conv = false;
+    }
+    if (all_conv) {
+      if (k > 0) TRY(density_dense(dF.as<double>()));
+      ++it;
+      break;
+    }
+    if (k > 0) {
+      const int m = std::min(count, k);
+      std::vector<double> coef(static_cast<size_t>(B) * k, 0.0);
+      for (int b = 0; b < B; ++b) {
+        const double* Bb = Bm.data() + static_cast<size_t>(b) * k * k;
+        const int sz = m + 1;
+        std::vector<double> A(static_cast<size_t>(sz) * sz, 0.0), rhs(sz, 0.0);
+        for (int i = 0; i < m; ++i) {
+          for (int j = 0; j < m; ++j) A[i * sz + j] = Bb[i * k + j];
+          A[i * sz + m] = A[m * sz + i] = 1.0;
+        }
+        rhs[m] = 1.0;
+        bool ok = true;
+        for (int c = 0; c < sz && ok; ++c) {
+          int piv = c;
+          for (int r = c + 1; r < sz; ++r)
+            if (std::fabs(A[r * sz + c]) > std::fabs(A[piv * sz + c])) piv = r;
+          if (std::fabs(A[piv * sz + c]) < 1e-300) { ok = false; break; }
+          if (piv != c) {
+            for (int j = 0; j < sz; ++j) std::swap(A[c * sz + j], A[piv * sz + j]);
+            std::swap(rhs[c], rhs[piv]);
+          }
+          for (int r = 0; r < sz; ++r) {
+            if (r == c) continue;
+            const double f = A[r * sz + c] / A[c * sz + c];
+            for (int j = c; j < sz; ++j) A[r * sz + j] -= f * A[c * sz + j];
+            rhs[r] -= f * rhs[c];
+          }
+        }
+        const int last = (count - 1) % k;
+        for (int i = 0; i < m; ++i)
+          coef[static_cast<size_t>(b) * k + i] = ok ? rhs[i] / A[i * sz + i] : (i == last ? 1.0 : 0.0);
+      }
+      HIPC(hipMemcpyAsync(dcoef.p, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, st));
+      HIPC(batch_combine(dFs.as<double>(), m2 * B, m, k, dcoef.as<double>(), nmax, B, dFb.as<double>(), st));
+      TRY(density_b(dFb.as<double>()));
+    }
+  }
+  HIPC(hipStreamSynchronize(st));
+  for (int b = 0; b < B; ++b) {
+    rchem_scf_result& r = res[m0 + b];
+    r.e_elec = e_cur[b];
+    double enuc = 0.0;
+    for (int p = mol_atoms[b].x; p < mol_atoms[b].y; ++p)
+      for (int q = mol_atoms[b].x; q < p; ++q) {
+        double d2 = 0.0;
+        for (int x = 0; x < 3; ++x) d2 += (xyzc[3 * p + x] - xyzc[3 * q + x]) * (xyzc[3 * p + x] - xyzc[3 * q + x]);
+        enuc += Zc[p] * Zc[q] / std::sqrt(d2);
+      }
+    r.e_nuc = enuc;
+    r.e_total = r.e_elec + enuc;
+    r.iterations = conv_at[b] >= 0 ? conv_at[b] : it;
+    r.converged = conv_at[b] >= 0 ? 1 : 0;
+    r.fp64_from = fp64_from;
+  }
+  return all_conv ? RCHEM_OK : RCHEM_ERR_NOT_CONVERGED;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rchem_batch_scf(rchem_ctx* ctx, int nmol, const int* natom, const int* Z, const double* xyz,
+                    const char* basis_name, const rchem_scf_opts* opts, rchem_scf_result* res) {
+  if (!ctx) return RCHEM_ERR_INVALID;
+  if (nmol <= 0 || !natom || !Z || !xyz || !basis_name || !opts || !res)
+    return ctx->fail(RCHEM_ERR_INVALID, "rchem_batch_scf: bad arguments");
+  TRY(check_opts(ctx, &opts->jk));
+  TRY(set_device(ctx));
+  auto lower = [](std::string s) {
+    for (auto& c : s) c = static_cast<char>(std::tolower(static_cast<unsigned char>(c)));
+    return s;
+  };
+  const std::string want = lower(basis_name);
+  // functions per molecule, for chunking
+  std::vector<int> nbf(nmol, 0);
+  int a = 0;
+  for (int m = 0; m < nmol; ++m) {
+    if (natom[m] <= 0) return ctx->fail(RCHEM_ERR_INVALID, "molecule without atoms");
+    for (int k = 0; k < natom[m]; ++k, ++a)
+      for (int r = 0; r < kBasisNumRows; ++r)
+        if (lower(kBasisRows[r].set) == want && kBasisRows[r].Z == Z[a])
+          for (int q = 0; q < 2; ++q)
+            if (kBasisRows[r].am[q] >= 0) nbf[m] += ncart(kBasisRows[r].am[q]);
+  }
+  int status = RCHEM_OK, m0 = 0, aoff = 0;
+  while (m0 < nmol) {
+    int m1 = m0, tot = 0, aend = aoff;
+    while (m1 < nmol && (m1 == m0 || tot + nbf[m1] <= kBatchMaxNbf)) {
+      tot += nbf[m1];
+      aend += natom[m1];
+      ++m1;
+    }
+    const int rc = batch_chunk(ctx, m0, m1, natom, Z, xyz, aoff, want, opts, res);
+    if (rc != RCHEM_OK && rc != RCHEM_ERR_NOT_CONVERGED) {
+      ctx->shell_mol.clear();
+      ctx->nmol = 1;
+      return rc;
+    }
+    if (rc == RCHEM_ERR_NOT_CONVERGED) status = rc;
+    m0 = m1;
+    aoff = aend;
+  }
+  // the context's basis is the last chunk's; leave it as a plain basis
+  ctx->shell_mol.clear();
+  ctx->nmol = 1;
+  ctx->have_basis = false;
+  if (status == RCHEM_ERR_NOT_CONVERGED)
+    return ctx->fail(RCHEM_ERR_NOT_CONVERGED, "some molecules did not converge in max_iter");
+  return status;
+}
 
 int rchem_rhf_gradient(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
                        const rchem_scf_opts* opts, rchem_scf_result* res, double* grad) {

@@ -142,6 +142,11 @@ struct PlanArgs {
   double* stats;           // per class: {quartets64, quartets32, prim64, prim32}
   int nunits;              // bra units (class, bra pair)
   int chunk_tiles;         // tiles per chunk at most
+  // several molecules in one basis: kets restricted to the bra's molecule,
+  // segs[(bucket slot) * nmol + molecule] = its pairs [begin, end) in the bucket
+  const int2* segs;        // null: one molecule
+  int nmol;
+  const int* pair_mol;     // molecule of each pair (layout order)
 };
 
 // Device buffers the planner fills (all sized by the host from upper bounds).
