@@ -386,6 +386,17 @@ def main():
                               "FP64 VALU kernel; MI355X FP64 vector peak = FP64 matrix peak")},
         "classes": classes,
     }
+    if args.workload == "h2o64":
+        # the same mixed build through the host-buffer entry point (rchem_jk:
+        # D copied in, J and K copied out over PCIe), for reference only
+        Jh, Kh = np.zeros((n, n)), np.zeros((n, n))
+        oh = default_opts(precision=PREC_MIXED if "mixed" in runs else PREC_FP64,
+                          schwarz_thresh=1e-12)
+        rb.JK_direct(Jh, Kh, B, D, oh)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            rb.JK_direct(Jh, Kh, B, D, oh)
+        out["host_buffers_ms_per_step"] = (time.perf_counter() - t0) / 3 * 1e3
     if "mixed" in runs and "fp64" in runs:
         f = runs["fp64"]
         out["fp64"] = {"ms_per_step": f["ms"], "value": f["quartets_per_s"],

@@ -55,8 +55,13 @@ __device__ __forceinline__ float fast_rsqrt(float x) {
   return y;
 }
 
-// Exact Boys F_0..F_M into F[0..M]; `table` is the staged block of order M
-// (kBoysBlock coefficients, then the exp row), normally in LDS.
+// Order of the table block a kernel of highest order M stages: for M = 1 the
+// order-0 block, whose Taylor polynomial and its derivative give F_0 and F_1
+// in one pass (dF_0/dT = -F_1), with no exponential.
+constexpr int boys_block_order(int M) { return M == 1 ? 0 : M; }
+
+// Exact Boys F_0..F_M into F[0..M]; `table` is the staged block of order
+// boys_block_order(M) (kBoysBlock coefficients, then the exp row), normally in LDS.
 template <int M, typename R>
 __device__ __forceinline__ void boys_exact(R T, const R* __restrict__ table, R* __restrict__ F) {
   static_assert(M <= kBoysMaxM, "Boys order beyond the table");
@@ -65,11 +70,28 @@ __device__ __forceinline__ void boys_exact(R T, const R* __restrict__ table, R* 
     const int k = static_cast<int>(T * R(kBoysInvH) + R(0.5));
     const R d = R(k) * (R(1) / R(kBoysInvH)) - T;  // T0 - T, |d| <= 1/4
     const R* __restrict__ c = table + k * kBoysTerms;
+    if constexpr (M == 1) {
+      // F_0(T0 - d) = sum_j c_j d^j (c_j = F_j(T0) / j!); F_1 = d/dd of it
+      R p = c[K - 1], q = R(0);
+#pragma unroll
+      for (int j = K - 2; j >= 0; --j) {
+        q = q * d + p;
+        p = p * d + c[j];
+      }
+      F[0] = p;
+      F[1] = q;
+      return;
+    }
     R f = c[K - 1];
 #pragma unroll
     for (int j = K - 2; j >= 0; --j) f = c[j] + d * f;
     F[M] = f;
-    if constexpr (M > 0) {
+    if constexpr (M > 0 && sizeof(R) == 4) {
+      const R e = __expf(-T);  // hardware exp2: ample for the FP32 path
+      const R t2 = T + T;
+#pragma unroll
+      for (int m = M - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + e) * R(1.0 / (2 * m + 1));
+    } else if constexpr (M > 0) {
       // e^{d} by its Taylor series with constant coefficients 1/j!
       constexpr double inv_fact[14] = {1.0, 1.0, 1.0 / 2, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720,
                                        1.0 / 5040, 1.0 / 40320, 1.0 / 362880, 1.0 / 3628800,
@@ -117,11 +139,11 @@ __device__ __forceinline__ void boys_exact_rt(int M, double T, const double* __r
   }
 }
 
-// Copies the order-M block and the exp row into LDS (every thread of the
+// Copies the block of order boys_block_order(M) and the exp row into LDS (every thread of the
 // workgroup calls it; a barrier follows).
 template <int M, typename R>
 __device__ __forceinline__ void stage_boys(R* __restrict__ dst, const R* __restrict__ src) {
-  const int m = M <= kBoysMaxM ? M : kBoysMaxM;
+  const int m = boys_block_order(M <= kBoysMaxM ? M : kBoysMaxM);
   for (int k = threadIdx.x; k < kBoysBlock; k += blockDim.x) dst[k] = src[m * kBoysBlock + k];
   for (int k = threadIdx.x; k < kBoysRows; k += blockDim.x)
     dst[kBoysBlock + k] = src[kBoysExpOffset + k];

@@ -55,18 +55,27 @@ struct TileInfo {
   bool active;
 };
 
+// A bra primitive pair as the LDS-digestion kernel stages it, in the kernel's
+// arithmetic type: PA = P - A is formed in FP64 once per chunk and rounded
+// once, so the primitive loop never touches FP64 in the FP32 kernels.
+template <typename R>
+struct BraPrim {
+  R zeta, pax, pay, paz, kab, oo2z;
+};
+
 // Primitive loops: the ket primitive (per-lane, vector loads, fetched one
 // iteration ahead) is the outer loop, the bra primitive (wave-uniform) the inner
 // one, so each ket record is read once per quartet.  With BRA_LDS the bra's
-// primitive pairs are read from `sbra` in LDS (the LDS-digestion kernel stages
-// them once per chunk), otherwise from global memory.  `boys_tab` is the Boys
-// grid in LDS.
+// primitive pairs are read from `sbra` in LDS (staged once per chunk by the
+// LDS-digestion kernel) and P - Q is formed as (P - A) - (Q - A), Q - A once per
+// ket primitive; otherwise they come from global memory.  `boys_tab` is the
+// Boys grid in LDS.
 template <int LA, int LB, int LC, int LD, int MODE, int BM, typename R, bool BRA_LDS = false>
 __device__ __forceinline__ void eri_compute(const EriArgs& a, const Tile t, const int lane,
                                             TileInfo& ti,
                                             double (&out)[EriClass<LA, LB, LC, LD>::NOUT],
                                             const R* __restrict__ boys_tab,
-                                            const PrimRec* __restrict__ sbra = nullptr) {
+                                            const BraPrim<R>* __restrict__ sbra = nullptr) {
   using C = EriClass<LA, LB, LC, LD>;
   constexpr int L = C::L;
   static_assert(BM == kBoysExact || sizeof(R) == 8, "compat Boys is FP64 only");
@@ -91,24 +100,47 @@ __device__ __forceinline__ void eri_compute(const EriArgs& a, const Tile t, cons
     if (j + 1 < kp.np) nxt = kprim[(j + 1) * kp.ps];
     const R zk = R(pk.zeta), kk = R(pk.kab), oo2e = R(pk.oo2z);
     const R QCx = R(pk.px - kp.A[0]), QCy = R(pk.py - kp.A[1]), QCz = R(pk.pz - kp.A[2]);
-    for (int i = 0; i < bp.np; ++i) {
-      const PrimRec pb = BRA_LDS ? sbra[i] : a.prims[bp.p0 + i * bp.ps];
-      const R zb = R(pb.zeta);
-      const R rs = fast_rsqrt(zb + zk);  // (zeta + eta)^(-1/2)
-      const R ooze = rs * rs;
-      const R roz = zk * ooze;  // rho / zeta
-      const R roe = zb * ooze;  // rho / eta
-      const R PQx = R(pb.px - pk.px), PQy = R(pb.py - pk.py), PQz = R(pb.pz - pk.pz);
-      const R T = zb * roz * (PQx * PQx + PQy * PQy + PQz * PQz);
-      R F[L + 1];
-      boys<L, BM, R>(T, table, F);
-      const R pref = R(pb.kab) * kk * rs;
-      R G[L + 1];
+    if constexpr (BRA_LDS) {
+      const R QAx = R(pk.px - bp.A[0]), QAy = R(pk.py - bp.A[1]), QAz = R(pk.pz - bp.A[2]);
+      for (int i = 0; i < bp.np; ++i) {
+        const BraPrim<R> pb = sbra[i];
+        const R zb = pb.zeta;
+        const R rs = fast_rsqrt(zb + zk);  // (zeta + eta)^(-1/2)
+        const R ooze = rs * rs;
+        const R roz = zk * ooze;  // rho / zeta
+        const R roe = zb * ooze;  // rho / eta
+        const R PQx = pb.pax - QAx, PQy = pb.pay - QAy, PQz = pb.paz - QAz;
+        const R T = zb * roz * (PQx * PQx + PQy * PQy + PQz * PQz);
+        R F[L + 1];
+        boys<L, BM, R>(T, table, F);
+        const R pref = pb.kab * kk * rs;
+        R G[L + 1];
 #pragma unroll
-      for (int m = 0; m <= L; ++m) G[m] = pref * F[m];
-      C::template vrr<R>(G, R(pb.px - bp.A[0]), R(pb.py - bp.A[1]), R(pb.pz - bp.A[2]),
-                         -roz * PQx, -roz * PQy, -roz * PQz, QCx, QCy, QCz, roe * PQx, roe * PQy,
-                         roe * PQz, R(pb.oo2z), oo2e, R(0.5) * ooze, roz, roe, acc);
+        for (int m = 0; m <= L; ++m) G[m] = pref * F[m];
+        C::template vrr<R>(G, pb.pax, pb.pay, pb.paz, -roz * PQx, -roz * PQy, -roz * PQz, QCx,
+                           QCy, QCz, roe * PQx, roe * PQy, roe * PQz, pb.oo2z, oo2e,
+                           R(0.5) * ooze, roz, roe, acc);
+      }
+    } else {
+      for (int i = 0; i < bp.np; ++i) {
+        const PrimRec pb = a.prims[bp.p0 + i * bp.ps];
+        const R zb = R(pb.zeta);
+        const R rs = fast_rsqrt(zb + zk);  // (zeta + eta)^(-1/2)
+        const R ooze = rs * rs;
+        const R roz = zk * ooze;  // rho / zeta
+        const R roe = zb * ooze;  // rho / eta
+        const R PQx = R(pb.px - pk.px), PQy = R(pb.py - pk.py), PQz = R(pb.pz - pk.pz);
+        const R T = zb * roz * (PQx * PQx + PQy * PQy + PQz * PQz);
+        R F[L + 1];
+        boys<L, BM, R>(T, table, F);
+        const R pref = R(pb.kab) * kk * rs;
+        R G[L + 1];
+#pragma unroll
+        for (int m = 0; m <= L; ++m) G[m] = pref * F[m];
+        C::template vrr<R>(G, R(pb.px - bp.A[0]), R(pb.py - bp.A[1]), R(pb.pz - bp.A[2]),
+                           -roz * PQx, -roz * PQy, -roz * PQz, QCx, QCy, QCz, roe * PQx,
+                           roe * PQy, roe * PQz, R(pb.oo2z), oo2e, R(0.5) * ooze, roz, roe, acc);
+      }
     }
   }
 
@@ -336,7 +368,7 @@ __global__ __launch_bounds__(64 * kLdsWaves) void eri_jk_lds_kernel(const EriArg
   using C = EriClass<LA, LB, LC, LD>;
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   extern __shared__ double krows[];
-  __shared__ PrimRec sbra[kMaxPrimPairs];
+  __shared__ BraPrim<R> sbra[kMaxPrimPairs];
   __shared__ R boys_tab[kBoysStageSize];
   stage_boys<C::L>(boys_tab, boys_src<R>(a));
   const int nbf = a.nbf;
@@ -350,7 +382,11 @@ __global__ __launch_bounds__(64 * kLdsWaves) void eri_jk_lds_kernel(const EriArg
     for (int k = threadIdx.x; k < nrow; k += blockDim.x) krows[k] = 0.0;
     {
       const PairRec bp = a.pairs[c.bra];
-      for (int k = threadIdx.x; k < bp.np; k += blockDim.x) sbra[k] = a.prims[bp.p0 + k * bp.ps];
+      for (int k = threadIdx.x; k < bp.np; k += blockDim.x) {
+        const PrimRec pr = a.prims[bp.p0 + k * bp.ps];
+        sbra[k] = BraPrim<R>{R(pr.zeta), R(pr.px - bp.A[0]), R(pr.py - bp.A[1]),
+                             R(pr.pz - bp.A[2]), R(pr.kab), R(pr.oo2z)};
+      }
     }
     __syncthreads();
     double jab_acc[NA * NB];
