@@ -178,6 +178,10 @@ typedef struct rchem_scf_opts {
                           the last k Fock matrices with the orthogonalised commutator
                           X^T (FDS - SDF) X as error, converged when also
                           max |error| < 1e-8 (rchem has no DIIS, rchem.rs:72-90) */
+  int graph;           /* 1: capture an iteration's Fock build (planner, ERI classes,
+                          digestion, F = H + 2J - K) once as a hipGraph and replay it on
+                          every later iteration (recaptured at the precision switch);
+                          0 (default): launch it kernel by kernel */
 } rchem_scf_opts;
 
 typedef struct rchem_scf_result {

@@ -201,12 +201,13 @@ def schwarz(basis: Basis):
 
 
 def scf(basis: Basis, atomcoords, atomnos, nocc, thresh_e=1e-11, max_iter=1024,
-        opts: Opts | None = None, verbose=False, diis=0):
+        opts: Opts | None = None, verbose=False, diis=0, graph=False):
     """rchem.rs:10-90 on the device: returns (SCFResult, D).  diis = k > 0
     replaces rchem's plain iterations by DIIS over k Fock matrices."""
     o = SCFOpts()
     lib().rchem_scf_opts_default(ctypes.byref(o))
     o.diis = int(diis)
+    o.graph = int(bool(graph))
     o.nocc = int(nocc)
     o.max_iter = int(max_iter)
     o.thresh_e = float(thresh_e)
@@ -224,7 +225,7 @@ def scf(basis: Basis, atomcoords, atomnos, nocc, thresh_e=1e-11, max_iter=1024,
 
 
 def batch_scf(molecules, basis_set_name, thresh_e=1e-11, max_iter=1024, diis=0,
-              opts: Opts | None = None, ctx=None):
+              opts: Opts | None = None, ctx=None, graph=False):
     """Many independent molecules [(atomnos, atomcoords), ...] solved together
     on the device; returns a list of SCFResult (the context's basis is cleared)."""
     ctx = ctx or default_context()
@@ -237,6 +238,7 @@ def batch_scf(molecules, basis_set_name, thresh_e=1e-11, max_iter=1024, diis=0,
     o.max_iter = int(max_iter)
     o.thresh_e = float(thresh_e)
     o.diis = int(diis)
+    o.graph = int(bool(graph))
     o.jk = opts or default_opts()
     res = (SCFResult * len(molecules))()
     rc = lib().rchem_batch_scf(ctx.h, len(molecules), _iptr(natom), _iptr(Z), _dptr(xyz),
@@ -254,6 +256,7 @@ def rhf_gradient(basis: Basis, atomcoords, atomnos, nocc, thresh_e=1e-11, max_it
     o = SCFOpts()
     lib().rchem_scf_opts_default(ctypes.byref(o))
     o.diis = int(diis)
+    o.graph = int(bool(graph))
     o.nocc = int(nocc)
     o.max_iter = int(max_iter)
     o.thresh_e = float(thresh_e)

@@ -1795,12 +1795,71 @@ int rchem_scf_opts_default(rchem_scf_opts* o) {
   rchem_opts_default(&o->jk);
   o->verbose = 0;
   o->diis = 0;
+  o->graph = 0;
   return RCHEM_OK;
 }
 
 }  // extern "C"
 
 namespace {
+
+// An SCF iteration's Fock build -- jk_device (planner, class kernels,
+// digestion) and F = H + 2J - K -- captured once as a hipGraph and replayed.
+// Every buffer it touches keeps its address between iterations (D is rewritten
+// in place), the planner recomputes everything density-dependent on the
+// device, and its fills are kernels, so a replay is the eager build.  The first
+// build of a configuration runs eagerly (host planning, allocations); the next
+// one is captured; a change of precision recaptures.  Per-class timing (event
+// records around each launch) keeps the build eager.
+struct FockGraph {
+  bool enabled = false;
+  int eager_runs = 0;
+  int precision = -1;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~FockGraph() { reset(); }
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    eager_runs = 0;
+  }
+  int run(rchem_ctx* ctx, const rchem_opts& jk, const double* D, double* J, double* K,
+          const double* H, double* F, size_t n2) {
+    auto eager = [&]() -> int {
+      TRY(jk_device(ctx, D, J, K, &jk));
+      HIPC(fock_launch(H, J, K, F, n2, ctx->stream));
+      return RCHEM_OK;
+    };
+    if (!enabled || ctx->profiling) return eager();
+    if (jk.precision != precision) {
+      reset();
+      precision = jk.precision;
+    }
+    if (exec) {
+      HIPC(hipGraphLaunch(exec, ctx->stream));
+      return RCHEM_OK;
+    }
+    if (eager_runs == 0) {
+      ++eager_runs;
+      return eager();
+    }
+    HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+    const int rc = eager();
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+    if (rc != RCHEM_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    HIPC(ec);
+    graph = g;
+    HIPC(hipGraphInstantiateWithFlags(&exec, graph, 0));
+    HIPC(hipGraphLaunch(exec, ctx->stream));
+    return RCHEM_OK;
+  }
+};
 
 int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
             const rchem_scf_opts* opts, rchem_scf_result* res, double dconv) {
@@ -1872,6 +1931,8 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
   // iterations (and the converged energy) see the all-FP64 Fock matrix.
   constexpr double kSwitch = 1e-7;
   rchem_opts jk = opts->jk;
+  FockGraph fock_graph;
+  fock_graph.enabled = opts->graph != 0;
   res->fp64_from = jk.precision == RCHEM_PREC_MIXED ? -1 : 0;
   auto maybe_switch = [&](double de_) {
     if (jk.precision == RCHEM_PREC_MIXED && std::fabs(de_) < kSwitch) {
@@ -1896,9 +1957,8 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
     double e_prev = 0.0;
     const rocblas_int nn = static_cast<rocblas_int>(n2);
     for (; it < opts->max_iter; ++it) {
-      TRY(jk_device(ctx, dD.as<double>(), dJ.as<double>(), dK.as<double>(), &jk));
-      HIPC(fock_launch(dH.as<double>(), dJ.as<double>(), dK.as<double>(), dF.as<double>(), n2,
-                       ctx->stream));
+      TRY(fock_graph.run(ctx, jk, dD.as<double>(), dJ.as<double>(), dK.as<double>(),
+                         dH.as<double>(), dF.as<double>(), n2));
       double e_cur = 0.0;
       TRY(energy(dF.as<double>(), &e_cur));  // sum D o (H + F(D)) of the current density
       if (!std::isfinite(e_cur)) return ctx->fail(RCHEM_ERR_NONFINITE, "non-finite SCF energy");
@@ -1988,9 +2048,8 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
     }
   }
   for (; opts->diis <= 0 && it < opts->max_iter; ++it) {
-    TRY(jk_device(ctx, dD.as<double>(), dJ.as<double>(), dK.as<double>(), &jk));
-    HIPC(fock_launch(dH.as<double>(), dJ.as<double>(), dK.as<double>(), dF.as<double>(), n2,
-                     ctx->stream));
+    TRY(fock_graph.run(ctx, jk, dD.as<double>(), dJ.as<double>(), dK.as<double>(),
+                       dH.as<double>(), dF.as<double>(), n2));
     if (dconv > 0.0)
       HIPC(hipMemcpyAsync(dDo.p, dD.p, n2 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     TRY(density(dF.as<double>()));
@@ -2256,9 +2315,11 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
   std::vector<double> Bm(static_cast<size_t>(B) * k * k, 0.0);
   int count = 0, it = 0;
   bool all_conv = false;
+  FockGraph fock_graph;
+  fock_graph.enabled = opts->graph != 0;
   for (; it < opts->max_iter; ++it) {
-    TRY(jk_device(ctx, dD.as<double>(), dJ.as<double>(), dK.as<double>(), &jk));
-    HIPC(fock_launch(dH.as<double>(), dJ.as<double>(), dK.as<double>(), dF.as<double>(), n2, st));
+    TRY(fock_graph.run(ctx, jk, dD.as<double>(), dJ.as<double>(), dK.as<double>(),
+                       dH.as<double>(), dF.as<double>(), n2));
     e_prev = e_cur;
     if (k > 0) {
       // energy of the current density with its own Fock matrix
