@@ -249,10 +249,10 @@ def batch_scf(molecules, basis_set_name, thresh_e=1e-11, max_iter=1024, diis=0,
 
 
 def rhf_gradient(basis: Basis, atomcoords, atomnos, nocc, thresh_e=1e-11, max_iter=1024,
-                 opts: Opts | None = None, diis=0):
+                 opts: Opts | None = None, diis=0, graph=False):
     """Analytic RHF nuclear gradient (Eh/bohr) after rchem's SCF on the device
-    (DIIS over `diis` Fock matrices when > 0): returns (SCFResult, grad of
-    shape (natom, 3))."""
+    (DIIS over `diis` Fock matrices when > 0; the Fock build replayed as a
+    hipGraph when `graph`): returns (SCFResult, grad of shape (natom, 3))."""
     o = SCFOpts()
     lib().rchem_scf_opts_default(ctypes.byref(o))
     o.diis = int(diis)
