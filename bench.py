@@ -410,19 +410,24 @@ def main():
         for p in ("fp64", "mixed"):
             o = default_opts(precision=PREC_FP64 if p == "fp64" else PREC_MIXED,
                              schwarz_thresh=1e-12)
-            secs = {}
-            for graph in (False, True):
+            secs, energies = {}, {}
+            for graph, purify in ((False, False), (True, False), (True, True)):
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
                 res, Dc = rb.scf(B, xyz, Z, nocc=nocc_of(Z), thresh_e=1e-10, diis=8, opts=o,
-                                 graph=graph)
-                secs[graph] = time.perf_counter() - t0
+                                 graph=graph, purify=purify)
+                secs[(graph, purify)] = time.perf_counter() - t0
+                energies[(graph, purify)] = res.e_total
             scf[p] = {"e_total": res.e_total, "iterations": res.iterations,
                       "converged": bool(res.converged), "fp64_from": res.fp64_from,
-                      "seconds": secs[True], "seconds_kernel_by_kernel": secs[False]}
+                      "seconds": secs[(True, True)],
+                      "seconds_graph_eigensolver": secs[(True, False)],
+                      "seconds_kernel_by_kernel_eigensolver": secs[(False, False)],
+                      "max_delta_e_between_variants": max(energies.values()) - min(energies.values())}
             say(f"SCF {p}: {res.iterations} iterations, E = {res.e_total:.10f}, "
-                f"{secs[True]:.2f} s with the Fock build replayed as a hipGraph, "
-                f"{secs[False]:.2f} s kernel by kernel")
+                f"{secs[(True, True)]:.2f} s (hipGraph Fock build, purified densities), "
+                f"{secs[(True, False)]:.2f} s (hipGraph, eigensolver), "
+                f"{secs[(False, False)]:.2f} s (kernel by kernel, eigensolver)")
             if p == "fp64":
                 D_conv = Dc
         scf["delta_e"] = abs(scf["mixed"]["e_total"] - scf["fp64"]["e_total"])
@@ -438,7 +443,9 @@ def main():
         scf["delta_e_fock_build"] = abs(e_build["mixed"] - e_build["fp64"])
         scf["note"] = ("DIIS(8); mixed switches to all-FP64 J/K once |dE| < 1e-7 Eh; "
                        "convergence |dE| < 1e-10 Eh and max|FDS-SDF| < 1e-8; 'seconds' "
-                       "replays the captured Fock build (hipGraph) on every iteration")
+                       "replays the captured Fock build (hipGraph) on every iteration and "
+                       "forms each iteration's density by canonical purification (DGEMM) "
+                       "instead of the eigensolver")
         out["scf"] = scf
     if not args.no_cpu_baseline and args.workload == "h2o64":
         out["cpu_baseline"] = cpu_baseline(Z, xyz, D, st.quartets, args.cpu_seconds)

@@ -182,6 +182,14 @@ typedef struct rchem_scf_opts {
                           digestion, F = H + 2J - K) once as a hipGraph and replay it on
                           every later iteration (recaptured at the precision switch);
                           0 (default): launch it kernel by kernel */
+  int purify;          /* 1: each iteration's density D = X P X^T with P, the projector
+                          onto the nocc lowest eigenvectors of F' = X^T F X, from
+                          canonical purification (GEMMs only) instead of the
+                          eigensolver; the converged orbitals and orbital energies
+                          still come from the eigensolver, which also takes over
+                          an iteration whose F' has no gap.  Single-molecule SCF
+                          and gradient; the batched SCF ignores it.  0 (default):
+                          rchem's eigh every iteration (rchem.rs:74-78) */
 } rchem_scf_opts;
 
 typedef struct rchem_scf_result {

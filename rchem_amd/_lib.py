@@ -46,7 +46,8 @@ class ClassTiming(ctypes.Structure):
 class SCFOpts(ctypes.Structure):
     _fields_ = [("nocc", ctypes.c_int), ("max_iter", ctypes.c_int),
                 ("thresh_e", ctypes.c_double), ("jk", Opts), ("verbose", ctypes.c_int),
-                ("diis", ctypes.c_int), ("graph", ctypes.c_int)]
+                ("diis", ctypes.c_int), ("graph", ctypes.c_int),
+                ("purify", ctypes.c_int)]
 
 
 class SCFResult(ctypes.Structure):

@@ -201,13 +201,16 @@ def schwarz(basis: Basis):
 
 
 def scf(basis: Basis, atomcoords, atomnos, nocc, thresh_e=1e-11, max_iter=1024,
-        opts: Opts | None = None, verbose=False, diis=0, graph=False):
+        opts: Opts | None = None, verbose=False, diis=0, graph=False, purify=False):
     """rchem.rs:10-90 on the device: returns (SCFResult, D).  diis = k > 0
-    replaces rchem's plain iterations by DIIS over k Fock matrices."""
+    replaces rchem's plain iterations by DIIS over k Fock matrices; graph
+    replays the Fock build as a hipGraph; purify forms each iteration's
+    density by canonical purification (GEMMs) instead of the eigensolver."""
     o = SCFOpts()
     lib().rchem_scf_opts_default(ctypes.byref(o))
     o.diis = int(diis)
     o.graph = int(bool(graph))
+    o.purify = int(bool(purify))
     o.nocc = int(nocc)
     o.max_iter = int(max_iter)
     o.thresh_e = float(thresh_e)
@@ -249,7 +252,7 @@ def batch_scf(molecules, basis_set_name, thresh_e=1e-11, max_iter=1024, diis=0,
 
 
 def rhf_gradient(basis: Basis, atomcoords, atomnos, nocc, thresh_e=1e-11, max_iter=1024,
-                 opts: Opts | None = None, diis=0, graph=False):
+                 opts: Opts | None = None, diis=0, graph=False, purify=False):
     """Analytic RHF nuclear gradient (Eh/bohr) after rchem's SCF on the device
     (DIIS over `diis` Fock matrices when > 0; the Fock build replayed as a
     hipGraph when `graph`): returns (SCFResult, grad of shape (natom, 3))."""
@@ -257,6 +260,7 @@ def rhf_gradient(basis: Basis, atomcoords, atomnos, nocc, thresh_e=1e-11, max_it
     lib().rchem_scf_opts_default(ctypes.byref(o))
     o.diis = int(diis)
     o.graph = int(bool(graph))
+    o.purify = int(bool(purify))
     o.nocc = int(nocc)
     o.max_iter = int(max_iter)
     o.thresh_e = float(thresh_e)

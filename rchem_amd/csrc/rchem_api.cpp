@@ -63,6 +63,10 @@ hipError_t fock_launch(const double* H, const double* J, const double* K, double
 hipError_t energy_launch(const double* D, const double* H, const double* F, double* out,
                          size_t n2, hipStream_t s);
 hipError_t antisym_launch(const double* A, double* E, int n, hipStream_t s);
+hipError_t purify_start_launch(const double* F, int n, int nocc, double* lo, double* hi,
+                               double* par, double* P, hipStream_t s);
+hipError_t purify_step_launch(double* P, const double* P2, const double* P3, int n,
+                              double* coef, hipStream_t s);
 hipError_t batch_pack(const double* A, size_t n, const int* f0, const int* nb, int nmax, int nbatch,
                       double pad, double* out, hipStream_t s);
 hipError_t batch_unpack(const double* in, int nmax, const int* f0, const int* nb, int nbatch,
@@ -1796,6 +1800,7 @@ int rchem_scf_opts_default(rchem_scf_opts* o) {
   o->verbose = 0;
   o->diis = 0;
   o->graph = 0;
+  o->purify = 0;
   return RCHEM_OK;
 }
 
@@ -1899,12 +1904,64 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
   HIPC(scale_columns_launch(dtmp.as<double>(), dw.as<double>(), dW.as<double>(), n, ctx->stream));
   BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_transpose, n, n, n, &one,
                       dW.as<double>(), n, dtmp.as<double>(), n, &zero, dX.as<double>(), n));
-  auto density = [&](const double* Fm) -> int {
+  // Canonical purification of F' (dFp) into P (dP): GEMMs until tr(P - P^2)
+  // falls below kPurifyTol, then two more steps (each squares the distance of
+  // every eigenvalue from 0 or 1); false if that does not happen in
+  // kPurifyMax steps (no gap at the Fermi level), and the eigensolver takes over.
+  DevBuf dP, dP2, dP3, dpl, dph, dpar;
+  if (opts->purify) {
+    for (DevBuf* b : {&dP, &dP2, &dP3}) HIPC(b->ensure(n2 * sizeof(double)));
+    HIPC(dpl.ensure(n * sizeof(double)));
+    HIPC(dph.ensure(n * sizeof(double)));
+    HIPC(dpar.ensure(8 * sizeof(double)));
+  }
+  constexpr int kPurifyMax = 200;
+  constexpr double kPurifyTol = 1e-9;
+  auto purify = [&](bool* ok) -> int {
+    *ok = false;
+    HIPC(purify_start_launch(dFp.as<double>(), n, opts->nocc, dpl.as<double>(), dph.as<double>(),
+                             dpar.as<double>(), dP.as<double>(), ctx->stream));
+    int extra = -1;
+    for (int k = 0; k < kPurifyMax; ++k) {
+      BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
+                          dP.as<double>(), n, dP.as<double>(), n, &zero, dP2.as<double>(), n));
+      BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
+                          dP2.as<double>(), n, dP.as<double>(), n, &zero, dP3.as<double>(), n));
+      HIPC(purify_step_launch(dP.as<double>(), dP2.as<double>(), dP3.as<double>(), n,
+                              dpar.as<double>() + 4, ctx->stream));
+      double t1 = 0.0;
+      HIPC(hipMemcpyAsync(&t1, dpar.as<double>() + 5, sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+      HIPC(hipStreamSynchronize(ctx->stream));
+      // tr(P - P^2) = sum lambda (1 - lambda) >= 0 while the spectrum stays in [0, 1]
+      if (!std::isfinite(t1) || t1 < -kPurifyTol) return RCHEM_OK;
+      if (extra < 0 && t1 < kPurifyTol) extra = 2;
+      if (extra >= 0 && extra-- == 0) {
+        *ok = true;
+        return RCHEM_OK;
+      }
+    }
+    return RCHEM_OK;
+  };
+  auto density = [&](const double* Fm, bool exact) -> int {
     // F' = X^T F X, eigh, C = X C', D = C_occ C_occ^T (rchem.rs:56-59, 74-78, 101-104)
     BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, Fm, n,
                         dX.as<double>(), n, &zero, dtmp.as<double>(), n));
     BLASC(rocblas_dgemm(hb, rocblas_operation_transpose, rocblas_operation_none, n, n, n, &one,
                         dX.as<double>(), n, dtmp.as<double>(), n, &zero, dFp.as<double>(), n));
+    if (opts->purify && !exact) {
+      // the same density as X P X^T, P the projector onto F''s nocc lowest eigenvectors
+      bool ok = false;
+      TRY(purify(&ok));
+      if (ok) {
+        BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
+                            dX.as<double>(), n, dP.as<double>(), n, &zero, dtmp.as<double>(), n));
+        BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_transpose, n, n, n,
+                            &one, dtmp.as<double>(), n, dX.as<double>(), n, &zero,
+                            dD.as<double>(), n));
+        return RCHEM_OK;
+      }
+    }
     BLASC(rocsolver_dsyevd(hb, rocblas_evect_original, rocblas_fill_upper, n, dFp.as<double>(), n,
                            dw.as<double>(), de.as<double>(), dinfo.as<rocblas_int>()));
     BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
@@ -1920,7 +1977,7 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
     HIPC(hipStreamSynchronize(ctx->stream));
     return RCHEM_OK;
   };
-  TRY(density(dH.as<double>()));
+  TRY(density(dH.as<double>(), false));
   double e_new = 0.0;
   TRY(energy(dH.as<double>(), &e_new));
   res->converged = 0;
@@ -2001,7 +2058,7 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
         if (opts->verbose) std::printf("Convergence achieved!\n");
         res->converged = 1;
         ++it;
-        TRY(density(dF.as<double>()));  // C, eps and D of the converged Fock matrix
+        TRY(density(dF.as<double>(), true));  // C, eps and D of the converged Fock matrix
         break;
       }
       e_prev = e_cur;
@@ -2040,10 +2097,10 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
           BLASC(rocblas_daxpy(hb, nn, &ci, dFs.as<double>() + static_cast<size_t>(i) * n2, 1,
                               dFx.as<double>(), 1));
         }
-        TRY(density(dFx.as<double>()));
+        TRY(density(dFx.as<double>(), false));
       } else {
         count = 0;  // singular subspace: restart from the current Fock matrix
-        TRY(density(dF.as<double>()));
+        TRY(density(dF.as<double>(), false));
       }
     }
   }
@@ -2052,7 +2109,7 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
                        dH.as<double>(), dF.as<double>(), n2));
     if (dconv > 0.0)
       HIPC(hipMemcpyAsync(dDo.p, dD.p, n2 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-    TRY(density(dF.as<double>()));
+    TRY(density(dF.as<double>(), false));
     const double e_old = e_new;
     TRY(energy(dF.as<double>(), &e_new));
     if (!std::isfinite(e_new)) return ctx->fail(RCHEM_ERR_NONFINITE, "non-finite SCF energy");
@@ -2075,6 +2132,8 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
       if (opts->verbose) std::printf("Convergence achieved!\n");
       res->converged = 1;
       ++it;
+      // orbitals and orbital energies (the gradient's W) from the eigensolver
+      if (opts->purify) TRY(density(dF.as<double>(), true));
       break;
     }
   }
