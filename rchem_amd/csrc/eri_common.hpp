@@ -360,7 +360,8 @@ __global__ __launch_bounds__(256) void eri_kernel(const EriArgs a) {
 }
 
 // LDS digestion: a fixed grid of workgroups drains chunks [range[0], range[1]).
-// Dynamic LDS holds the bra's K rows: (NA + NB) x nbf doubles.
+// Dynamic LDS holds the bra's K rows: (NA + NB) x kw doubles, columns
+// [kc0, kc0 + kw) (all nbf columns for one molecule).
 constexpr int kLdsWaves = 4;  // waves per LDS-digestion workgroup (they share the K rows)
 
 template <int LA, int LB, int LC, int LD, int BM, typename R>
@@ -371,14 +372,15 @@ __global__ __launch_bounds__(64 * kLdsWaves) void eri_jk_lds_kernel(const EriArg
   __shared__ BraPrim<R> sbra[kMaxPrimPairs];
   __shared__ R boys_tab[kBoysStageSize];
   stage_boys<C::L>(boys_tab, boys_src<R>(a));
-  const int nbf = a.nbf;
-  const size_t n = static_cast<size_t>(nbf);
+  const size_t n = static_cast<size_t>(a.nbf);
+  const int kw = a.kw;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-  const int nrow = (NA + NB) * nbf;
+  const int nrow = (NA + NB) * kw;
   const int begin = a.chunk_range[0], end = a.chunk_range[1];
   for (int ch = begin + blockIdx.x; ch < end; ch += gridDim.x) {  // block-uniform
     const Chunk c = a.chunks[ch];
+    const int kc0 = a.pair_kc0 ? a.pair_kc0[c.bra] : 0;
     for (int k = threadIdx.x; k < nrow; k += blockDim.x) krows[k] = 0.0;
     {
       const PairRec bp = a.pairs[c.bra];
@@ -410,7 +412,8 @@ __global__ __launch_bounds__(64 * kLdsWaves) void eri_jk_lds_kernel(const EriArg
           unsafeAtomicAdd(&a.Jp[a.jp_ket.base + k * a.jp_ket.n + kq], 2.0 * s * jcd[k]);
         exchange_blocks<NA, NB, NC, ND>(a.D, n, fa, fb, fc, fd, out, s,
                                         [&](bool rb, int comp, int col, double v) {
-                                          atomicAdd(&krows[(rb ? NA + comp : comp) * nbf + col], v);
+                                          atomicAdd(&krows[(rb ? NA + comp : comp) * kw + col - kc0],
+                                                    v);
                                         });
       }
     }
@@ -427,7 +430,7 @@ __global__ __launch_bounds__(64 * kLdsWaves) void eri_jk_lds_kernel(const EriArg
     for (int k = threadIdx.x; k < nrow; k += blockDim.x) {
       const double v = krows[k];
       if (v != 0.0) {
-        const int r = k / nbf, col = k - r * nbf;
+        const int r = k / kw, col = kc0 + (k - r * kw);
         const int row = r < NA ? bp.fa + r : bp.fb + (r - NA);
         unsafeAtomicAdd(&a.K[row * n + col], v);
       }
@@ -455,7 +458,7 @@ hipError_t eri_launch(const EriArgs& a, int blocks, hipStream_t s) {
 template <int LA, int LB, int LC, int LD, int BM, typename R>
 hipError_t eri_jk_lds_launch(const EriArgs& a, int blocks, hipStream_t s) {
   if (blocks <= 0) return hipSuccess;
-  const size_t lds = sizeof(double) * (ncart(LA) + ncart(LB)) * static_cast<size_t>(a.nbf);
+  const size_t lds = sizeof(double) * (ncart(LA) + ncart(LB)) * static_cast<size_t>(a.kw);
   hipLaunchKernelGGL((eri_jk_lds_kernel<LA, LB, LC, LD, BM, R>), dim3(blocks),
                      dim3(64 * kLdsWaves), lds, s,
                      a);

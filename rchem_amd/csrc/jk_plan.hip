@@ -78,9 +78,10 @@ __global__ void plan_kernel(const PlanArgs a) {
     const int4 bk = a.buckets[pc.bk_off + local % pc.nbk];
     int lo = bk.x, hi = bk.y;
     if (a.segs) {  // the bra's molecule only
-      const int2 sg = a.segs[static_cast<size_t>(pc.bk_off + local % pc.nbk) * a.nmol + a.pair_mol[i]];
+      const int mol = a.pair_mol[i];
+      const int2 sg = a.segs[static_cast<size_t>(pc.bk_off + local % pc.nbk) * a.nmol + mol];
       lo = sg.x;
-      hi = sg.y;
+      hi = (a.mol_active && !a.mol_active[mol]) ? sg.x : sg.y;  // a retired molecule: none
     }
     if (pc.same) hi = (lo > i) ? lo : min(hi, i + 1);
     const double qi = a.Q[i];

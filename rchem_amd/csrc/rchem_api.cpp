@@ -219,6 +219,15 @@ struct rchem_ctx {
   std::vector<int> pair_mol;                    // layout order
   std::vector<std::vector<int2>> bucket_segs;   // [pair class][bucket * nmol + molecule]
   DevBuf pair_mol_d, plan_segs_d;
+  // batched SCF: per-molecule flags the planner reads (0 = converged and
+  // retired, its quartets skipped); used while mol_active_on
+  DevBuf mol_active_d;
+  bool mol_active_on = false;
+  // multi-molecule basis: first function of each pair's molecule (layout
+  // order) and the largest molecule's function count, the K-row window of
+  // LDS digestion
+  DevBuf pair_kc0_d;
+  int mol_kw = 0;
 
   // Schwarz bounds of every shell pair (a >= b), computed once per basis
   std::vector<PairHost> all_pairs;
@@ -518,6 +527,25 @@ int layout_pairs(rchem_ctx* ctx, const std::vector<PairHost>& sel, double prim_e
     if (!ctx->pair_mol.empty())
       HIPC(hipMemcpyAsync(ctx->pair_mol_d.p, ctx->pair_mol.data(),
                           sizeof(int) * ctx->pair_mol.size(), hipMemcpyHostToDevice, ctx->stream));
+    // each molecule's functions [f_lo, f_hi): the window its K rows need
+    std::vector<int> f_lo(nm, ctx->nbf), f_hi(nm, 0);
+    for (size_t sidx = 0; sidx < ctx->shells.size(); ++sidx) {
+      const int m = ctx->shell_mol[sidx], f = ctx->shells[sidx].f0;
+      f_lo[m] = std::min(f_lo[m], f);
+      f_hi[m] = std::max(f_hi[m], f + ncart(ctx->shells[sidx].l));
+    }
+    ctx->mol_kw = 1;
+    for (int m = 0; m < nm; ++m) ctx->mol_kw = std::max(ctx->mol_kw, f_hi[m] - f_lo[m]);
+    std::vector<int> kc0(ctx->pair_mol.size());
+    for (size_t k = 0; k < kc0.size(); ++k) {
+      // the window must fit inside [0, nbf): shift it left for the last molecules
+      kc0[k] = std::min(f_lo[ctx->pair_mol[k]], ctx->nbf - ctx->mol_kw);
+    }
+    HIPC(ctx->pair_kc0_d.ensure(sizeof(int) * std::max<size_t>(kc0.size(), 1)));
+    if (!kc0.empty())
+      HIPC(hipMemcpyAsync(ctx->pair_kc0_d.p, kc0.data(), sizeof(int) * kc0.size(),
+                          hipMemcpyHostToDevice, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));  // kc0 is a local
   }
   HIPC(ctx->alpha_d.ensure(sizeof(double) * std::max<size_t>(alphas.size(), 1)));
   if (!alphas.empty())
@@ -1167,6 +1195,7 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
       pa.segs = ctx->plan_segs_d.as<int2>();
       pa.nmol = ctx->nmol;
       pa.pair_mol = ctx->pair_mol_d.as<int>();
+      if (ctx->mol_active_on) pa.mol_active = ctx->mol_active_d.as<int>();
     }
     PlanBuffers pb{};
     pb.off64 = ctx->plan_off64.as<int>();
@@ -1196,9 +1225,12 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
       a.Jp = ctx->jp_d.as<double>();
       a.jp_bra = ctx->jp_cls[qc.X];
       a.jp_ket = ctx->jp_cls[qc.Y];
-      // K rows of the bra pair in LDS when they fit in 64 KB
-      const size_t lds =
-          sizeof(double) * (ncart(qc.la) + ncart(qc.lb)) * static_cast<size_t>(ctx->nbf);
+      // K rows of the bra pair in LDS when they fit in 64 KB: every column, or
+      // the bra's molecule's window in a multi-molecule basis
+      const bool multi = !ctx->shell_mol.empty();
+      a.kw = multi ? ctx->mol_kw : ctx->nbf;
+      a.pair_kc0 = multi ? ctx->pair_kc0_d.as<int>() : nullptr;
+      const size_t lds = sizeof(double) * (ncart(qc.la) + ncart(qc.lb)) * static_cast<size_t>(a.kw);
       const bool lds_ok = lds <= 65536 && ctx->lds_digestion && !ctx->force_generic;
       int* ranges = pb.ranges + 8 * (sg - ps.seg_begin);
       for (int prec = 0; prec < (mixed ? 2 : 1); ++prec) {
@@ -2357,6 +2389,20 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
   std::vector<double> e_cur(B, 0.0), e_prev(B, 0.0), errmax(B, 0.0);
   TRY(energies(dH.as<double>(), &e_cur));
   std::vector<int> conv_at(B, -1);
+  std::vector<double> e_final(B, 0.0);
+  // converged molecules retire: the planner skips their quartets from the
+  // next Fock build on, and their energy is kept from the iteration that
+  // converged (their later J, K, F and D are no longer meaningful)
+  std::vector<int> active(B, 1);
+  HIPC(ctx->mol_active_d.ensure(sizeof(int) * B));
+  HIPC(hipMemcpyAsync(ctx->mol_active_d.p, active.data(), sizeof(int) * B, hipMemcpyHostToDevice,
+                      st));
+  HIPC(hipStreamSynchronize(st));
+  ctx->mol_active_on = true;
+  struct ActiveOff {
+    rchem_ctx* c;
+    ~ActiveOff() { c->mol_active_on = false; }
+  } active_off{ctx};
   rchem_opts jk = opts->jk;
   int fp64_from = jk.precision == RCHEM_PREC_MIXED ? -1 : 0;
   const int k = std::min(std::max(opts->diis, 0), 32);
@@ -2420,6 +2466,10 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
     double demax = 0.0;
     int settled = 0;
     for (int b = 0; b < B; ++b) {
+      if (conv_at[b] >= 0) {
+        ++settled;
+        continue;
+      }
       const double d = std::fabs(e_cur[b] - e_prev[b]);
       demax = std::max(demax, d);
       if (d < 1e-7) ++settled;
@@ -2433,19 +2483,29 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
       fp64_from = it + 1;
       if (k <= 0) continue;
     }
-    all_conv = it > 0 && jk.precision != RCHEM_PREC_MIXED;
+    all_conv = true;
+    bool retired = false;
     for (int b = 0; b < B; ++b) {
+      if (conv_at[b] >= 0) continue;
       const bool ok = it > 0 && jk.precision != RCHEM_PREC_MIXED &&
                       std::fabs(e_cur[b] - e_prev[b]) < opts->thresh_e &&
                       (k <= 0 || errmax[b] < 1e-8);
-      if (ok && conv_at[b] < 0) conv_at[b] = it + 1;
-      if (!ok) all_conv = false;
+      if (ok) {
+        conv_at[b] = it + 1;
+        e_final[b] = e_cur[b];
+        active[b] = 0;
+        retired = true;
+      } else {
+        all_conv = false;
+      }
     }
     if (all_conv) {
-      if (k > 0) TRY(density_dense(dF.as<double>()));
       ++it;
       break;
     }
+    if (retired)
+      HIPC(hipMemcpyAsync(ctx->mol_active_d.p, active.data(), sizeof(int) * B,
+                          hipMemcpyHostToDevice, st));
     if (k > 0) {
       const int m = std::min(count, k);
       std::vector<double> coef(static_cast<size_t>(B) * k, 0.0);
@@ -2487,7 +2547,7 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
   HIPC(hipStreamSynchronize(st));
   for (int b = 0; b < B; ++b) {
     rchem_scf_result& r = res[m0 + b];
-    r.e_elec = e_cur[b];
+    r.e_elec = conv_at[b] >= 0 ? e_final[b] : e_cur[b];
     double enuc = 0.0;
     for (int p = mol_atoms[b].x; p < mol_atoms[b].y; ++p)
       for (int q = mol_atoms[b].x; q < p; ++q) {

@@ -82,6 +82,12 @@ struct EriArgs {
   const int* chunk_range;  // [begin, end) in chunks, device memory
   double* Jp;
   JpClass jp_bra, jp_ket;
+  // K rows in LDS span columns [kc0, kc0 + kw) of the bra's rows: kc0 = 0 and
+  // kw = nbf for one molecule; in a multi-molecule basis (quartets within a
+  // molecule) kc0 = pair_kc0[bra], the first function of the bra's molecule,
+  // and kw the largest molecule's function count
+  int kw;
+  const int* pair_kc0;
   // kModeGrad (wave-cooperative kernel): exponent of the first shell of each
   // primitive pair (same layout as prims), atoms of each pair's two shells,
   // and the 3 x natom gradient accumulated into
@@ -147,6 +153,7 @@ struct PlanArgs {
   const int2* segs;        // null: one molecule
   int nmol;
   const int* pair_mol;     // molecule of each pair (layout order)
+  const int* mol_active;   // per molecule: 0 = retired (no quartets); null: all active
 };
 
 // Device buffers the planner fills (all sized by the host from upper bounds).
