@@ -1,7 +1,7 @@
 // One-electron integrals S, T, V over contracted Cartesian Gaussians (gfx950).
 //
-// One thread per shell pair (a >= b) writes the full component blocks of S, T
-// and V and their transposes.  McMurchie--Davidson: per axis the Hermite
+// One thread per shell pair (a >= b), one kernel per (la, lb) class, writes the
+// full component blocks of S, T and V and their transposes.  McMurchie--Davidson: per axis the Hermite
 // expansion coefficients E_t^{ij}, overlap S_x(i,j) = E_0^{ij} sqrt(pi/p),
 // kinetic energy from overlaps with b shifted by +-2 (the identity rchem's
 // C library uses, cints.c:152-177), nuclear attraction
@@ -38,6 +38,7 @@ struct OneElArgs {
   double* V;
   const int* shell_mol;   // several molecules: the molecule of each shell, else null
   const int2* mol_atoms;  // nuclei [begin, end) of each molecule
+  int cls_begin[10];      // pairs sorted by class 3 la + lb: class c is [cls_begin[c], cls_begin[c + 1])
 };
 
 constexpr int kLmax1 = 2;              // highest l served
@@ -95,25 +96,101 @@ __device__ __forceinline__ void hermite_e(int imax, int jmax, double pa, double 
   }
 }
 
+// Hermite coefficients E_t^{ij}, i <= IMAX, j <= JMAX, one axis, with
+// compile-time bounds so that every index is static (registers, no scratch).
+template <int IMAX, int JMAX>
+__device__ __forceinline__ void hermite_e_t(double pa, double pb, double oo2p,
+                                            double (&E)[IMAX + 1][JMAX + 1][IMAX + JMAX + 1]) {
+#pragma unroll
+  for (int i = 0; i <= IMAX; ++i)
+#pragma unroll
+    for (int j = 0; j <= JMAX; ++j)
+#pragma unroll
+      for (int t = 0; t <= IMAX + JMAX; ++t) E[i][j][t] = 0.0;
+  E[0][0][0] = 1.0;
+#pragma unroll
+  for (int i = 0; i <= IMAX; ++i) {
+    if (i > 0) {
+#pragma unroll
+      for (int t = 0; t <= i; ++t) {
+        double v = pa * E[i - 1][0][t];
+        if (t > 0) v += oo2p * E[i - 1][0][t - 1];
+        if (t + 1 <= i - 1) v += (t + 1) * E[i - 1][0][t + 1];
+        E[i][0][t] = v;
+      }
+    }
+#pragma unroll
+    for (int j = 1; j <= JMAX; ++j)
+#pragma unroll
+      for (int t = 0; t <= i + j; ++t) {
+        double v = pb * E[i][j - 1][t];
+        if (t > 0) v += oo2p * E[i][j - 1][t - 1];
+        if (t + 1 <= i + j - 1) v += (t + 1) * E[i][j - 1][t + 1];
+        E[i][j][t] = v;
+      }
+  }
+}
+
+__host__ __device__ constexpr int cart_x(int l, int k) {
+  // component k of shell l in rchem's order (shell.rs:1-12): powers (l+1-a, a-b, b-1)
+  int idx = 0;
+  for (int a = 1; a <= l + 1; ++a)
+    for (int b = 1; b <= a; ++b) {
+      if (idx == k) return l + 1 - a;
+      ++idx;
+    }
+  return 0;
+}
+__host__ __device__ constexpr int cart_y(int l, int k) {
+  int idx = 0;
+  for (int a = 1; a <= l + 1; ++a)
+    for (int b = 1; b <= a; ++b) {
+      if (idx == k) return a - b;
+      ++idx;
+    }
+  return 0;
+}
+__host__ __device__ constexpr int cart_z(int l, int k) {
+  int idx = 0;
+  for (int a = 1; a <= l + 1; ++a)
+    for (int b = 1; b <= a; ++b) {
+      if (idx == k) return b - 1;
+      ++idx;
+    }
+  return 0;
+}
+
+// One thread per shell pair of class (LA, LB): pairs [cls_begin[c], cls_begin[c + 1]),
+// c = 3 LA + LB.  Every loop has compile-time bounds, so the Hermite
+// coefficients and the R_tuv table live in registers; the Boys block of order
+// LA + LB is staged in LDS.
+template <int LA, int LB>
 __global__ __launch_bounds__(128) void one_electron_kernel(const OneElArgs a) {
-  const int pid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pid >= a.npairs) return;
+  constexpr int L = LA + LB;
+  constexpr int NA = (LA + 1) * (LA + 2) / 2, NB = (LB + 1) * (LB + 2) / 2;
+  __shared__ double boys_tab[kBoysStageSize];
+  stage_boys<L>(boys_tab, a.boys_table);
+  __syncthreads();
+  const int c = 3 * LA + LB;
+  const int pid = a.cls_begin[c] + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (pid >= a.cls_begin[c + 1]) return;
   const int2 pr = a.pairs[pid];
   const ShellDev sa = a.shells[pr.x], sb = a.shells[pr.y];
-  const int la = sa.l, lb = sb.l;
-  const int na = ncart1(la), nb = ncart1(lb);
-  constexpr int NMAX = 6;
-  double s[NMAX][NMAX], t[NMAX][NMAX], v[NMAX][NMAX];
-  for (int i = 0; i < NMAX; ++i)
-    for (int j = 0; j < NMAX; ++j) s[i][j] = t[i][j] = v[i][j] = 0.0;
-  int ca[NMAX][3], cb[NMAX][3];
-  for (int k = 0; k < na; ++k) cart_comp(la, k, ca[k]);
-  for (int k = 0; k < nb; ++k) cart_comp(lb, k, cb[k]);
+  double s[NA][NB], t[NA][NB], v[NA][NB];
+#pragma unroll
+  for (int i = 0; i < NA; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) s[i][j] = t[i][j] = v[i][j] = 0.0;
   const double AB2 = (sa.xyz[0] - sb.xyz[0]) * (sa.xyz[0] - sb.xyz[0]) +
                      (sa.xyz[1] - sb.xyz[1]) * (sa.xyz[1] - sb.xyz[1]) +
                      (sa.xyz[2] - sb.xyz[2]) * (sa.xyz[2] - sb.xyz[2]);
-
-  double E[3][kLmax1 + 1][kE][kT];
+  int c0 = 0, c1 = a.natom;
+  if (a.shell_mol) {  // the pair's own molecule's nuclei
+    const int2 r = a.mol_atoms[a.shell_mol[pr.x]];
+    c0 = r.x;
+    c1 = r.y;
+  }
+  double E[3][LA + 1][LB + 3][L + 3];
   for (int ip = 0; ip < sa.nprim; ++ip)
     for (int jp = 0; jp < sb.nprim; ++jp) {
       const double al = a.exps[sa.p0 + ip], be = a.exps[sb.p0 + jp];
@@ -121,81 +198,88 @@ __global__ __launch_bounds__(128) void one_electron_kernel(const OneElArgs a) {
       const double cc = a.ccoef[sa.p0 + ip] * a.ccoef[sb.p0 + jp];
       const double kab = exp(-al * be * oop * AB2);
       double P[3];
+#pragma unroll
       for (int x = 0; x < 3; ++x) {
         P[x] = (al * sa.xyz[x] + be * sb.xyz[x]) * oop;
-        hermite_e(la, lb + 2, P[x] - sa.xyz[x], P[x] - sb.xyz[x], 0.5 * oop, E[x]);
+        hermite_e_t<LA, LB + 2>(P[x] - sa.xyz[x], P[x] - sb.xyz[x], 0.5 * oop, E[x]);
       }
       const double sq = sqrt(3.14159265358979323846 * oop);
       const double pre_s = cc * kab * sq * sq * sq;
-      // overlap and kinetic
-      for (int i = 0; i < na; ++i)
-        for (int j = 0; j < nb; ++j) {
-          const int* A = ca[i];
-          const int* B = cb[j];
+      // overlap and kinetic (b shifted by +-2, cints.c:152-177)
+#pragma unroll
+      for (int i = 0; i < NA; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+          const int A[3] = {cart_x(LA, i), cart_y(LA, i), cart_z(LA, i)};
+          const int B[3] = {cart_x(LB, j), cart_y(LB, j), cart_z(LB, j)};
           double ov[3], ovp2[3], ovm2[3];
+#pragma unroll
           for (int x = 0; x < 3; ++x) {
             ov[x] = E[x][A[x]][B[x]][0];
             ovp2[x] = E[x][A[x]][B[x] + 2][0];
-            ovm2[x] = B[x] >= 2 ? E[x][A[x]][B[x] - 2][0] : 0.0;
+            ovm2[x] = B[x] >= 2 ? E[x][A[x]][B[x] >= 2 ? B[x] - 2 : 0][0] : 0.0;
           }
           const double s0 = ov[0] * ov[1] * ov[2];
           s[i][j] += pre_s * s0;
           const int lbt = B[0] + B[1] + B[2];
-          double t1 = ovp2[0] * ov[1] * ov[2] + ov[0] * ovp2[1] * ov[2] + ov[0] * ov[1] * ovp2[2];
-          double t2 = B[0] * (B[0] - 1) * ovm2[0] * ov[1] * ov[2] +
-                      B[1] * (B[1] - 1) * ov[0] * ovm2[1] * ov[2] +
-                      B[2] * (B[2] - 1) * ov[0] * ov[1] * ovm2[2];
+          const double t1 =
+              ovp2[0] * ov[1] * ov[2] + ov[0] * ovp2[1] * ov[2] + ov[0] * ov[1] * ovp2[2];
+          const double t2 = B[0] * (B[0] - 1) * ovm2[0] * ov[1] * ov[2] +
+                            B[1] * (B[1] - 1) * ov[0] * ovm2[1] * ov[2] +
+                            B[2] * (B[2] - 1) * ov[0] * ov[1] * ovm2[2];
           t[i][j] += pre_s * (be * (2 * lbt + 3) * s0 - 2.0 * be * be * t1 - 0.5 * t2);
         }
-      // nuclear attraction: R_tuv via the Hermite recursion, one centre at a time
-      const int L = la + lb;
-      int c0 = 0, c1 = a.natom;
-      if (a.shell_mol) {  // the pair's own molecule's nuclei
-        const int2 r = a.mol_atoms[a.shell_mol[pr.x]];
-        c0 = r.x;
-        c1 = r.y;
-      }
-      for (int c = c0; c < c1; ++c) {
-        const double PC[3] = {P[0] - a.atom_xyz[3 * c], P[1] - a.atom_xyz[3 * c + 1],
-                              P[2] - a.atom_xyz[3 * c + 2]};
+      // nuclear attraction, one nucleus at a time: R_tuv by the Hermite recursion
+      const double pre_v0 = -cc * kab * 2.0 * 3.14159265358979323846 * oop;
+      for (int cn = c0; cn < c1; ++cn) {
+        const double PC[3] = {P[0] - a.atom_xyz[3 * cn], P[1] - a.atom_xyz[3 * cn + 1],
+                              P[2] - a.atom_xyz[3 * cn + 2]};
         const double T = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
-        double F[2 * kLmax1 + 1];
-        boys_exact_rt(L, T, a.boys_table, F);
-        // R[n][t][u][v], t+u+v <= L - n
-        double R[2 * kLmax1 + 1][2 * kLmax1 + 1][2 * kLmax1 + 1][2 * kLmax1 + 1];
+        double F[L + 1];
+        boys_exact<L, double>(T, boys_tab, F);
+        double R[L + 1][L + 1][L + 1][L + 1];
         double m2p = 1.0;
+#pragma unroll
         for (int n = 0; n <= L; ++n) {
           R[n][0][0][0] = m2p * F[n];
           m2p *= -2.0 * p;
         }
+#pragma unroll
         for (int n = L - 1; n >= 0; --n) {
-          const int top = L - n;
-          for (int tt = 0; tt <= top; ++tt)
-            for (int uu = 0; uu <= top - tt; ++uu)
-              for (int vv = 0; vv <= top - tt - uu; ++vv) {
+#pragma unroll
+          for (int tt = 0; tt <= L - n; ++tt)
+#pragma unroll
+            for (int uu = 0; uu <= L - n - tt; ++uu)
+#pragma unroll
+              for (int vv = 0; vv <= L - n - tt - uu; ++vv) {
                 if (tt + uu + vv == 0) continue;
                 double r;
                 if (tt > 0) {
                   r = PC[0] * R[n + 1][tt - 1][uu][vv];
-                  if (tt > 1) r += (tt - 1) * R[n + 1][tt - 2][uu][vv];
+                  if (tt > 1) r += (tt - 1) * R[n + 1][tt > 1 ? tt - 2 : 0][uu][vv];
                 } else if (uu > 0) {
                   r = PC[1] * R[n + 1][tt][uu - 1][vv];
-                  if (uu > 1) r += (uu - 1) * R[n + 1][tt][uu - 2][vv];
+                  if (uu > 1) r += (uu - 1) * R[n + 1][tt][uu > 1 ? uu - 2 : 0][vv];
                 } else {
                   r = PC[2] * R[n + 1][tt][uu][vv - 1];
-                  if (vv > 1) r += (vv - 1) * R[n + 1][tt][uu][vv - 2];
+                  if (vv > 1) r += (vv - 1) * R[n + 1][tt][uu][vv > 1 ? vv - 2 : 0];
                 }
                 R[n][tt][uu][vv] = r;
               }
         }
-        const double pre_v = -a.atom_z[c] * cc * kab * 2.0 * 3.14159265358979323846 * oop;
-        for (int i = 0; i < na; ++i)
-          for (int j = 0; j < nb; ++j) {
-            const int* A = ca[i];
-            const int* B = cb[j];
+        const double pre_v = a.atom_z[cn] * pre_v0;
+#pragma unroll
+        for (int i = 0; i < NA; ++i)
+#pragma unroll
+          for (int j = 0; j < NB; ++j) {
+            const int A[3] = {cart_x(LA, i), cart_y(LA, i), cart_z(LA, i)};
+            const int B[3] = {cart_x(LB, j), cart_y(LB, j), cart_z(LB, j)};
             double sum = 0.0;
+#pragma unroll
             for (int tt = 0; tt <= A[0] + B[0]; ++tt)
+#pragma unroll
               for (int uu = 0; uu <= A[1] + B[1]; ++uu)
+#pragma unroll
                 for (int vv = 0; vv <= A[2] + B[2]; ++vv)
                   sum += E[0][A[0]][B[0]][tt] * E[1][A[1]][B[1]][uu] * E[2][A[2]][B[2]][vv] *
                          R[0][tt][uu][vv];
@@ -204,9 +288,13 @@ __global__ __launch_bounds__(128) void one_electron_kernel(const OneElArgs a) {
       }
     }
   const size_t n = static_cast<size_t>(a.nbf);
-  for (int i = 0; i < na; ++i)
-    for (int j = 0; j < nb; ++j) {
-      const double f = comp_factor(ca[i]) * comp_factor(cb[j]);
+#pragma unroll
+  for (int i = 0; i < NA; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      int ca[3] = {cart_x(LA, i), cart_y(LA, i), cart_z(LA, i)};
+      int cb[3] = {cart_x(LB, j), cart_y(LB, j), cart_z(LB, j)};
+      const double f = comp_factor(ca) * comp_factor(cb);
       const size_t mu = sa.f0 + i, nu = sb.f0 + j;
       if (a.S) { a.S[mu * n + nu] = f * s[i][j]; a.S[nu * n + mu] = f * s[i][j]; }
       if (a.T) { a.T[mu * n + nu] = f * t[i][j]; a.T[nu * n + mu] = f * t[i][j]; }
@@ -425,10 +513,28 @@ hipError_t one_electron_grad_launch(const OneElGradArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <int LA, int LB>
+hipError_t one_electron_class(const OneElArgs& a, hipStream_t s) {
+  const int c = 3 * LA + LB;
+  const int cnt = a.cls_begin[c + 1] - a.cls_begin[c];
+  if (cnt <= 0) return hipSuccess;
+  hipLaunchKernelGGL((one_electron_kernel<LA, LB>), dim3((cnt + 127) / 128), dim3(128), 0, s, a);
+  return hipGetLastError();
+}
+
+// a.pairs must be sorted by class 3 la + lb with a.cls_begin its boundaries.
 hipError_t one_electron_launch(const OneElArgs& a, hipStream_t s) {
   if (a.npairs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(one_electron_kernel, dim3((a.npairs + 127) / 128), dim3(128), 0, s, a);
-  return hipGetLastError();
+  hipError_t e;
+  if ((e = one_electron_class<0, 0>(a, s)) != hipSuccess) return e;
+  if ((e = one_electron_class<0, 1>(a, s)) != hipSuccess) return e;
+  if ((e = one_electron_class<0, 2>(a, s)) != hipSuccess) return e;
+  if ((e = one_electron_class<1, 0>(a, s)) != hipSuccess) return e;
+  if ((e = one_electron_class<1, 1>(a, s)) != hipSuccess) return e;
+  if ((e = one_electron_class<1, 2>(a, s)) != hipSuccess) return e;
+  if ((e = one_electron_class<2, 0>(a, s)) != hipSuccess) return e;
+  if ((e = one_electron_class<2, 1>(a, s)) != hipSuccess) return e;
+  return one_electron_class<2, 2>(a, s);
 }
 
 }  // namespace rchem

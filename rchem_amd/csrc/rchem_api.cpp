@@ -105,7 +105,20 @@ struct OneElArgs {
   double* V;
   const int* shell_mol;   // several molecules: the molecule of each shell, else null
   const int2* mol_atoms;  // nuclei [begin, end) of each molecule
+  int cls_begin[10];      // pairs sorted by class 3 la + lb: class c is [cls_begin[c], cls_begin[c + 1])
 };
+// Sorts shell pairs by class 3 l_a + l_b (stable) and records the class
+// boundaries, the order one_electron_launch expects.
+void sort_pairs_by_class(const std::vector<ShellDev>& sh, std::vector<int2>* pr, int* cls_begin) {
+  auto cls = [&](const int2& q) { return 3 * sh[q.x].l + sh[q.y].l; };
+  std::stable_sort(pr->begin(), pr->end(),
+                   [&](const int2& x, const int2& y) { return cls(x) < cls(y); });
+  size_t i = 0;
+  for (int c = 0; c < 10; ++c) {
+    while (i < pr->size() && cls((*pr)[i]) < c) ++i;
+    cls_begin[c] = static_cast<int>(i);
+  }
+}
 struct OneElGradArgs {
   const ShellDev* shells;
   const double* exps;
@@ -1779,6 +1792,8 @@ int rchem_one_electron(rchem_ctx* ctx, int natom, const double* Z, const double*
   std::vector<int2> pr;
   for (int a = 0; a < ns; ++a)
     for (int b = 0; b <= a; ++b) pr.push_back(make_int2(a, b));
+  int cls_begin[10];
+  sort_pairs_by_class(sh, &pr, cls_begin);
   const size_t n2 = static_cast<size_t>(ctx->nbf) * ctx->nbf;
   DevBuf dsh, dex, dcc, dpr, dax, daz, dS, dT, dV;
   HIPC(dsh.ensure(sizeof(ShellDev) * ns));
@@ -1815,6 +1830,7 @@ int rchem_one_electron(rchem_ctx* ctx, int natom, const double* Z, const double*
   a.S = dS.as<double>();
   a.T = dT.as<double>();
   a.V = dV.as<double>();
+  for (int c = 0; c < 10; ++c) a.cls_begin[c] = cls_begin[c];
   HIPC(one_electron_launch(a, ctx->stream));
   if (S) HIPC(hipMemcpyAsync(S, dS.p, n2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   if (T) HIPC(hipMemcpyAsync(T, dT.p, n2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -2288,6 +2304,8 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
     for (int x = 0; x < ns; ++x)
       for (int y = 0; y <= x; ++y)
         if (ctx->shell_mol[x] == ctx->shell_mol[y]) pr.push_back(make_int2(x, y));
+    int cls_begin[10];
+    sort_pairs_by_class(sh, &pr, cls_begin);
     DevBuf dsh, dex, dcc, dpr, dax, daz, dsm, dma;
     HIPC(dsh.ensure(sizeof(ShellDev) * ns));
     HIPC(dex.ensure(sizeof(double) * ctx->exps.size()));
@@ -2321,6 +2339,7 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
     oa.V = dV.as<double>();
     oa.shell_mol = dsm.as<int>();
     oa.mol_atoms = dma.as<int2>();
+    for (int c = 0; c < 10; ++c) oa.cls_begin[c] = cls_begin[c];
     HIPC(one_electron_launch(oa, st));
     HIPC(hipStreamSynchronize(st));
   }
