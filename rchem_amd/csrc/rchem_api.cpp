@@ -1146,7 +1146,7 @@ int run_eri_host(rchem_ctx* ctx, int mode, int bm, double* out) {
 }
 
 constexpr int kPersistentBlocks = 2048;  // 256 CUs x 8 workgroups of 4 waves
-constexpr int kChunkTiles = 32;          // tiles per LDS-digestion chunk
+constexpr int kChunkTiles = 64;          // tiles per LDS-digestion chunk
 
 int launch_timed(rchem_ctx* ctx, const QClass& qc, int fp32, const ClassKernel& fn,
                  const EriArgs& a, int blocks) {
