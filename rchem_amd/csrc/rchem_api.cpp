@@ -137,6 +137,8 @@ struct OneElGradArgs {
 };
 hipError_t one_electron_grad_launch(const OneElGradArgs& a, hipStream_t s);
 hipError_t one_electron_launch(const OneElArgs& a, hipStream_t s);
+using EriGradFn = hipError_t (*)(const EriArgs&, hipStream_t);
+EriGradFn eri_grad_launcher(int la, int lb, int lc, int ld);
 }  // namespace rchem
 
 using namespace rchem;
@@ -2683,12 +2685,6 @@ int rchem_rhf_gradient(rchem_ctx* ctx, int natom, const double* Z, const double*
     HIPC(hipMemcpyAsync(dpa.p, pat.data(), sizeof(int2) * pat.size(), hipMemcpyHostToDevice,
                         ctx->stream));
   for (const QClass& hc : ctx->hclasses) {
-    const GenericProgram* gp = nullptr;
-    TRY(find_program(ctx, hc.la, hc.lb, hc.lc, hc.ld, 1, &gp));
-    if (!gp)
-      return ctx->fail(RCHEM_ERR_UNSUPPORTED, "gradient of ERI class " +
-                                                  class_name(hc.la, hc.lb, hc.lc, hc.ld) +
-                                                  " not served by this library");
     EriArgs a = base_args(ctx);
     a.tiles = ctx->tiles_d.as<Tile>() + hc.tile_begin;
     a.ntiles = hc.ntiles;
@@ -2696,6 +2692,19 @@ int rchem_rhf_gradient(rchem_ctx* ctx, int natom, const double* Z, const double*
     a.prim_alpha = ctx->alpha_d.as<double>();
     a.pair_atoms = dpa.as<int2>();
     a.grad = dgrad.as<double>();
+    // one quartet per lane where the class's recurrences fit a lane's registers
+    const EriGradFn lane_fn = ctx->force_generic ? nullptr
+                                                 : eri_grad_launcher(hc.la, hc.lb, hc.lc, hc.ld);
+    if (lane_fn) {
+      HIPC(lane_fn(a, ctx->stream));
+      continue;
+    }
+    const GenericProgram* gp = nullptr;
+    TRY(find_program(ctx, hc.la, hc.lb, hc.lc, hc.ld, 1, &gp));
+    if (!gp)
+      return ctx->fail(RCHEM_ERR_UNSUPPORTED, "gradient of ERI class " +
+                                                  class_name(hc.la, hc.lb, hc.lc, hc.ld) +
+                                                  " not served by this library");
     HIPC(eri_generic_launch(a, *gp, kModeGrad, kBoysExact, 0, ctx->stream));
   }
   // one-electron part

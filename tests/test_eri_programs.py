@@ -176,3 +176,52 @@ def test_grad_program_matches_finite_differences(key):
             fd = (_oracle_block(ls, sp) - _oracle_block(ls, sm)) / (2 * h)
             scale = max(1.0, np.max(np.abs(fd)))
             assert np.max(np.abs(got[centre, i] - fd)) < 2e-7 * scale, (centre, i)
+
+
+def _run_straight(g, la, lb, lc, ld, shells):
+    """Executes the straight-line per-lane gradient code (gen_eri_grad.py),
+    read as Python statements, over the same primitive loop as _execute."""
+    def py(lines):
+        out = []
+        for s in lines:
+            s = s.rstrip(";").replace("const R ", "").replace("R(", "float(")
+            out.append(s)
+        return compile("\n".join(out), "<generated>", "exec")
+    vrr, hrr = py(g["vrr"]), py(g["hrr"])
+    (A, ea, ca), (B, eb, cb), (C, ec, cc), (D, ed, cd) = shells
+    bra = _prims(A, B, la, lb, ea, eb, ca, cb)
+    ket = _prims(C, D, lc, ld, ec, ed, cc, cd)
+    acc = [0.0] * g["nacc"]
+    for (zb, P, kb, oo2z, aa) in bra:
+        for (zk, Q, kk, oo2e, ac) in ket:
+            rs = 1.0 / math.sqrt(zb + zk)
+            ooze = rs * rs
+            roz, roe = zk * ooze, zb * ooze
+            PQ = P - Q
+            T = zb * roz * float(PQ @ PQ)
+            env = dict(G=[kb * kk * rs * O.boys_exact(m, T) for m in range(g["L"] + 1)],
+                       acc=acc, oo2z=oo2z, oo2e=oo2e, oo2ze=0.5 * ooze, roz=roz, roe=roe,
+                       wA=2.0 * aa, wB=2.0 * (zb - aa), wC=2.0 * ac, float=float)
+            for i, x in enumerate("xyz"):
+                env["PA" + x], env["WP" + x] = (P - A)[i], -roz * PQ[i]
+                env["QC" + x], env["WQ" + x] = (Q - C)[i], roe * PQ[i]
+            exec(vrr, env)
+    out = [0.0] * g["nout"]
+    env = dict(acc=acc, out=out, float=float)
+    for i, x in enumerate("xyz"):
+        env["AB" + x], env["CD" + x] = (A - B)[i], (C - D)[i]
+    exec(hrr, env)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("cls", __import__("gen_eri_grad").GRAD_CLASSES)
+def test_straight_line_gradient_code_matches_tables(cls):
+    """The per-lane gradient code is a translation of grad_program's tables
+    (checked against finite differences above): same outputs."""
+    from gen_eri_grad import translate
+    la, lb, lc, ld = cls
+    shells = _random_shells(list(cls), 97 + la + 2 * lb + 4 * lc + 8 * ld)
+    ref = _execute(grad_program(la, lb, lc, ld), la, lb, lc, ld, shells)
+    got = _run_straight(translate(la, lb, lc, ld), la, lb, lc, ld, shells)
+    assert got.shape == ref.shape
+    assert np.max(np.abs(got - ref)) < 1e-13 * max(1.0, np.max(np.abs(ref)))

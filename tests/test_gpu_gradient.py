@@ -99,3 +99,24 @@ def test_diis_matches_plain_scf_energy(gpu_ctx):
         e_plain = _gpu_energy(gpu_ctx, Z, xyz, name)
         e_diis = _gpu_energy(gpu_ctx, Z, xyz, name, diis=6)
         assert abs(e_plain - e_diis) < 1e-10
+
+
+def test_per_lane_gradient_matches_wave_cooperative(gpu_ctx):
+    """The s/p gradient classes run one quartet per lane (straight-line code
+    translated from the operation tables); a context with RCHEM_ERI_GENERIC=1
+    runs every class on the wave-cooperative kernel from the same tables.
+    Both must give the same gradient (1e-10 Eh/bohr) on (H2O)_4 6-31G."""
+    Z, xyz = water_cluster(dims=(2, 2, 1), seed=11)
+    _say(f"case: (H2O)_4 6-31G gradient, per-lane vs wave-cooperative kernels, {len(Z)} atoms")
+    g_lane = _analytic(gpu_ctx, Z, xyz, "6-31G", diis=8)
+    os.environ["RCHEM_ERI_GENERIC"] = "1"
+    try:
+        ctx = rb.Context(0)
+    finally:
+        del os.environ["RCHEM_ERI_GENERIC"]
+    try:
+        g_gen = _analytic(ctx, Z, xyz, "6-31G", diis=8)
+    finally:
+        ctx.close()
+    _invariances(Z, xyz, g_lane)
+    assert np.max(np.abs(g_lane - g_gen)) < 1e-10
