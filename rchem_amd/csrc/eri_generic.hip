@@ -311,17 +311,20 @@ __device__ void epilogue(const EriArgs& a, const GenericProgram& p, const Tile& 
 #pragma unroll
       for (int c = 0; c < 9; ++c) g[c] += gam * output(p, H, c * nc + k);
     }
-    const int2 ab = a.pair_atoms[bidx], cd = a.pair_atoms[kidx];
+    // six accumulators per pair (its two atoms' x, y, z), added onto the
+    // atoms once every class is done: per-atom atomics would contend
+    double* __restrict__ gb = a.grad_pair + 6 * static_cast<size_t>(bidx);
+    double* __restrict__ gk = a.grad_pair + 6 * static_cast<size_t>(kidx);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       const double gA = deg * wave_sum(g[i]);
       const double gB = deg * wave_sum(g[3 + i]);
       const double gC = deg * wave_sum(g[6 + i]);
       if (lane == 0) {
-        unsafeAtomicAdd(&a.grad[3 * ab.x + i], gA);
-        unsafeAtomicAdd(&a.grad[3 * ab.y + i], gB);
-        unsafeAtomicAdd(&a.grad[3 * cd.x + i], gC);
-        unsafeAtomicAdd(&a.grad[3 * cd.y + i], -(gA + gB + gC));
+        unsafeAtomicAdd(&gb[i], gA);
+        unsafeAtomicAdd(&gb[3 + i], gB);
+        unsafeAtomicAdd(&gk[i], gC);
+        unsafeAtomicAdd(&gk[3 + i], -(gA + gB + gC));
       }
     }
   }

@@ -10,8 +10,11 @@
 //   Gamma = 2 D_ab D_cd - (D_ac D_bd + D_ad D_bc) / 2
 // and the quartet's degeneracy deg = 8 s (s the 8-fold symmetry weight of the
 // Fock build), so dE/dX = sum deg Gamma d(ab|cd)/dX.  Centre D follows from
-// translational invariance, d/dD = -(d/dA + d/dB + d/dC).  The bra's atoms are
-// wave-uniform (one atomic per component per wave), the ket's per lane.
+// translational invariance, d/dD = -(d/dA + d/dB + d/dC).  Contributions go
+// to six accumulators per shell pair (its two atoms' x, y, z): the bra pair's
+// once per wave, each ket pair's per lane -- thousands of addresses, where
+// per-atom accumulators would have every lane of the chip contend for a few
+// hundred -- and grad_pairs_kernel adds them onto the atoms at the end.
 #include <hip/hip_runtime.h>
 
 #include "boys.hpp"
@@ -115,20 +118,34 @@ __global__ __launch_bounds__(256) void eri_grad_kernel(const EriArgs a) {
     if (kp.flags & 1) deg *= 2.0;
     if (t.bra != kidx) deg *= 2.0;
   }
-  const int2 ab = a.pair_atoms[t.bra];
-  const int2 cd = a.pair_atoms[kidx];
+  double* __restrict__ gb = a.grad_pair + 6 * static_cast<size_t>(t.bra);
+  double* __restrict__ gk = a.grad_pair + 6 * static_cast<size_t>(kidx);
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     const double gA = deg * g[i], gB = deg * g[3 + i], gC = deg * g[6 + i];
     const double sA = wave_sum_g(gA), sB = wave_sum_g(gB);
     if (lane == 0) {
-      unsafeAtomicAdd(&a.grad[3 * ab.x + i], sA);
-      unsafeAtomicAdd(&a.grad[3 * ab.y + i], sB);
+      unsafeAtomicAdd(&gb[i], sA);
+      unsafeAtomicAdd(&gb[3 + i], sB);
     }
     if (active) {
-      unsafeAtomicAdd(&a.grad[3 * cd.x + i], gC);
-      unsafeAtomicAdd(&a.grad[3 * cd.y + i], -(gA + gB + gC));
+      unsafeAtomicAdd(&gk[i], gC);
+      unsafeAtomicAdd(&gk[3 + i], -(gA + gB + gC));
     }
+  }
+}
+
+// grad[atom] += the pair accumulators of every pair on that atom
+__global__ void grad_pairs_kernel(const double* __restrict__ gp, const int2* __restrict__ atoms,
+                                  int npairs, double* __restrict__ grad) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  const int2 at = atoms[p];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const double x = gp[6 * static_cast<size_t>(p) + i], y = gp[6 * static_cast<size_t>(p) + 3 + i];
+    if (x != 0.0) unsafeAtomicAdd(&grad[3 * at.x + i], x);
+    if (y != 0.0) unsafeAtomicAdd(&grad[3 * at.y + i], y);
   }
 }
 
@@ -142,9 +159,18 @@ hipError_t eri_grad_launch_t(const EriArgs& a, hipStream_t s) {
 
 }  // namespace
 
+// Adds the pair accumulators (6 per pair) onto the atoms.
+hipError_t grad_pairs_launch(const double* gp, const int2* atoms, int npairs, double* grad,
+                             hipStream_t s) {
+  if (npairs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(grad_pairs_kernel, dim3((npairs + 255) / 256), dim3(256), 0, s, gp, atoms,
+                     npairs, grad);
+  return hipGetLastError();
+}
+
 // The per-lane gradient launcher of class (la lb|lc ld), or nullptr when the
 // class runs on the wave-cooperative kernel.  Uses a.tiles/a.ntiles (host
-// tiles), a.D, a.prim_alpha, a.pair_atoms, a.grad.
+// tiles), a.D, a.prim_alpha, a.grad_pair (zeroed by the caller).
 using EriGradFn = hipError_t (*)(const EriArgs&, hipStream_t);
 EriGradFn eri_grad_launcher(int la, int lb, int lc, int ld) {
 #define RCHEM_GRAD_CASE(A, B, C, D) \

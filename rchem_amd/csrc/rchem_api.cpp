@@ -139,6 +139,8 @@ hipError_t one_electron_grad_launch(const OneElGradArgs& a, hipStream_t s);
 hipError_t one_electron_launch(const OneElArgs& a, hipStream_t s);
 using EriGradFn = hipError_t (*)(const EriArgs&, hipStream_t);
 EriGradFn eri_grad_launcher(int la, int lb, int lc, int ld);
+hipError_t grad_pairs_launch(const double* gp, const int2* atoms, int npairs, double* grad,
+                             hipStream_t s);
 }  // namespace rchem
 
 using namespace rchem;
@@ -2684,8 +2686,14 @@ int rchem_rhf_gradient(rchem_ctx* ctx, int natom, const double* Z, const double*
   if (!pat.empty())
     HIPC(hipMemcpyAsync(dpa.p, pat.data(), sizeof(int2) * pat.size(), hipMemcpyHostToDevice,
                         ctx->stream));
+  DevBuf dgp;  // per-pair accumulators of the per-lane kernels
+  const int npairs = static_cast<int>(ctx->pairs.size());
+  HIPC(dgp.ensure(sizeof(double) * 6 * std::max(npairs, 1)));
+  HIPC(fill_launch(dgp.as<double>(), 6 * static_cast<size_t>(std::max(npairs, 1)), 0.0,
+                   ctx->stream));
   for (const QClass& hc : ctx->hclasses) {
     EriArgs a = base_args(ctx);
+    a.grad_pair = dgp.as<double>();
     a.tiles = ctx->tiles_d.as<Tile>() + hc.tile_begin;
     a.ntiles = hc.ntiles;
     a.D = ctx->scf_D.as<double>();
@@ -2707,6 +2715,8 @@ int rchem_rhf_gradient(rchem_ctx* ctx, int natom, const double* Z, const double*
                                                   " not served by this library");
     HIPC(eri_generic_launch(a, *gp, kModeGrad, kBoysExact, 0, ctx->stream));
   }
+  HIPC(grad_pairs_launch(dgp.as<double>(), dpa.as<int2>(), npairs, dgrad.as<double>(),
+                         ctx->stream));
   // one-electron part
   std::vector<ShellDev> sh(ns);
   for (int s = 0; s < ns; ++s) {
@@ -2751,6 +2761,7 @@ int rchem_rhf_gradient(rchem_ctx* ctx, int natom, const double* Z, const double*
   g.D = ctx->scf_D.as<double>();
   g.W = dW.as<double>();
   g.grad = dgrad.as<double>();
+
   HIPC(one_electron_grad_launch(g, ctx->stream));
   HIPC(hipMemcpyAsync(grad, dgrad.p, sizeof(double) * 3 * natom, hipMemcpyDeviceToHost,
                       ctx->stream));

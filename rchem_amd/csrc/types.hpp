@@ -94,6 +94,9 @@ struct EriArgs {
   const double* prim_alpha;
   const int2* pair_atoms;
   double* grad;
+  // per-lane gradient kernel: 6 accumulators per pair (its two atoms x, y, z),
+  // reduced onto the atoms afterwards (spreads the atomics over many addresses)
+  double* grad_pair;
 };
 
 // A class served by the wave-cooperative kernel: its recurrences as operation
