@@ -238,20 +238,29 @@ __device__ __forceinline__ void exchange_blocks(const double* __restrict__ D, si
 }
 
 // Coulomb sums of one quartet: jab[ab] = sum_cd (ab|cd) D_cd and
-// jcd[cd] = sum_ab (ab|cd) D_ab (unweighted).
+// jcd[cd] = sum_ab (ab|cd) D_ab (unweighted).  The ket block D_cd comes from
+// the pair-indexed copy Dp (component k at dket[k * dket_stride]) when given,
+// else from D.
 template <int NA, int NB, int NC, int ND>
 __device__ __forceinline__ void coulomb_sums(const double* __restrict__ D, size_t n, int fa,
                                              int fb, int fc, int fd, const double* out,
-                                             double* jab, double* jcd) {
+                                             double* jab, double* jcd,
+                                             const double* __restrict__ dket = nullptr,
+                                             int dket_stride = 0) {
   double dab[NA * NB], dcd[NC * ND];
 #pragma unroll
   for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
     for (int ib = 0; ib < NB; ++ib) dab[ia * NB + ib] = D[(fa + ia) * n + fb + ib];
+  if (dket) {
 #pragma unroll
-  for (int ic = 0; ic < NC; ++ic)
+    for (int k = 0; k < NC * ND; ++k) dcd[k] = dket[k * dket_stride];
+  } else {
 #pragma unroll
-    for (int id = 0; id < ND; ++id) dcd[ic * ND + id] = D[(fc + ic) * n + fd + id];
+    for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+      for (int id = 0; id < ND; ++id) dcd[ic * ND + id] = D[(fc + ic) * n + fd + id];
+  }
 #pragma unroll
   for (int k = 0; k < NC * ND; ++k) jcd[k] = 0.0;
 #pragma unroll
@@ -402,11 +411,12 @@ __global__ __launch_bounds__(64 * kLdsWaves) void eri_jk_lds_kernel(const EriArg
       const int fa = ti.bp.fa, fb = ti.bp.fb, fc = ti.kp.fa, fd = ti.kp.fb;
       const double s = symmetry_weight(ti);
       double jab[NA * NB], jcd[NC * ND];
-      coulomb_sums<NA, NB, NC, ND>(a.D, n, fa, fb, fc, fd, out, jab, jcd);
+      const int kq = ti.kidx - a.jp_ket.begin;
+      coulomb_sums<NA, NB, NC, ND>(a.D, n, fa, fb, fc, fd, out, jab, jcd,
+                                   a.Dp ? a.Dp + a.jp_ket.base + kq : nullptr, a.jp_ket.n);
 #pragma unroll
       for (int k = 0; k < NA * NB; ++k) jab_acc[k] += 2.0 * s * jab[k];
       if (ti.active) {
-        const int kq = ti.kidx - a.jp_ket.begin;
 #pragma unroll
         for (int k = 0; k < NC * ND; ++k)
           unsafeAtomicAdd(&a.Jp[a.jp_ket.base + k * a.jp_ket.n + kq], 2.0 * s * jcd[k]);

@@ -32,6 +32,25 @@ __global__ void jp_scatter_kernel(const PairRec* __restrict__ pairs, const doubl
   }
 }
 
+// Dp of pair p = D[fa + ia, fb + ib], in the layout of Jp: the ket pairs of a
+// tile are consecutive, so a wave reads its 64 ket density blocks coalesced
+// instead of gathering them from all over D.  One thread per pair.
+__global__ void dpair_gather_kernel(const PairRec* __restrict__ pairs, const double* __restrict__ D,
+                                    const JpClass* __restrict__ cls, const int* __restrict__ ncomp,
+                                    int ncls, int npairs, double* __restrict__ Dp, int nbf) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  int X = 0;
+  while (X + 1 < ncls && cls[X + 1].begin <= p) ++X;
+  const JpClass jc = cls[X];
+  const int nb = ncomp[2 * X + 1];
+  const int nc = ncomp[2 * X] * nb;
+  const PairRec pr = pairs[p];
+  const size_t n = static_cast<size_t>(nbf);
+  for (int k = 0; k < nc; ++k)
+    Dp[jc.base + k * jc.n + (p - jc.begin)] = D[(pr.fa + k / nb) * n + pr.fb + k % nb];
+}
+
 // J = Jh + Jh^T, K = Kh + Kh^T (the lower-half accumulators of eri_kernel).
 __global__ void symmetrize_kernel(const double* __restrict__ Jh, const double* __restrict__ Kh,
                                   double* __restrict__ J, double* __restrict__ K, int n) {
@@ -49,6 +68,15 @@ hipError_t jp_scatter_launch(const PairRec* pairs, const double* Jp, const JpCla
   if (npairs <= 0) return hipSuccess;
   hipLaunchKernelGGL(jp_scatter_kernel, dim3((npairs + 255) / 256), dim3(256), 0, s, pairs, Jp,
                      cls, ncomp, ncls, npairs, Jh, nbf);
+  return hipGetLastError();
+}
+
+hipError_t dpair_gather_launch(const PairRec* pairs, const double* D, const JpClass* cls,
+                               const int* ncomp, int ncls, int npairs, double* Dp, int nbf,
+                               hipStream_t s) {
+  if (npairs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(dpair_gather_kernel, dim3((npairs + 255) / 256), dim3(256), 0, s, pairs, D,
+                     cls, ncomp, ncls, npairs, Dp, nbf);
   return hipGetLastError();
 }
 

@@ -52,6 +52,9 @@ struct HostProgram {
 };
 #include "generated/eri_programs.inc"
 constexpr int kNumHostPrograms = sizeof(kHostPrograms) / sizeof(kHostPrograms[0]);
+hipError_t dpair_gather_launch(const PairRec* pairs, const double* D, const JpClass* cls,
+                               const int* ncomp, int ncls, int npairs, double* Dp, int nbf,
+                               hipStream_t s);
 hipError_t jp_scatter_launch(const PairRec* pairs, const double* Jp, const JpClass* cls,
                              const int* ncomp, int ncls, int npairs, double* Jh, int nbf,
                              hipStream_t s);
@@ -238,6 +241,7 @@ struct rchem_ctx {
   DevBuf pair_mol_d, plan_segs_d;
   // batched SCF: per-molecule flags the planner reads (0 = converged and
   // retired, its quartets skipped); used while mol_active_on
+  DevBuf dp_d;  // pair-indexed density blocks (layout of jp_d), rebuilt every Fock build
   DevBuf mol_active_d;
   bool mol_active_on = false;
   // multi-molecule basis: first function of each pair's molecule (layout
@@ -1189,6 +1193,12 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
     HIPC(dmax_launch(dD, n2, ctx->plan_dmax.as<unsigned long long>(), ctx->stream));
   }
   HIPC(fill_launch(ctx->jp_d.as<double>(), std::max<size_t>(ctx->jp_size, 1), 0.0, ctx->stream));
+  // the pairs' density blocks in the Jp layout, for the LDS kernels' Coulomb sums
+  HIPC(ctx->dp_d.ensure(sizeof(double) * std::max<size_t>(ctx->jp_size, 1)));
+  HIPC(dpair_gather_launch(ctx->pairs_d.as<PairRec>(), dD, ctx->jp_cls_d.as<JpClass>(),
+                           ctx->jp_ncomp_d.as<int>(), kNumPairClasses,
+                           static_cast<int>(ctx->pairs.size()), ctx->dp_d.as<double>(), ctx->nbf,
+                           ctx->stream));
   for (const auto& ps : ctx->passes) {
     PlanArgs pa{};
     pa.classes = ctx->plan_classes_d.as<PlanClass>() + ps.seg_begin;
@@ -1242,6 +1252,7 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
       a.Jp = ctx->jp_d.as<double>();
       a.jp_bra = ctx->jp_cls[qc.X];
       a.jp_ket = ctx->jp_cls[qc.Y];
+      a.Dp = ctx->dp_d.as<double>();
       // K rows of the bra pair in LDS when they fit in 64 KB: every column, or
       // the bra's molecule's window in a multi-molecule basis
       const bool multi = !ctx->shell_mol.empty();

@@ -82,6 +82,7 @@ struct EriArgs {
   const int* chunk_range;  // [begin, end) in chunks, device memory
   double* Jp;
   JpClass jp_bra, jp_ket;
+  const double* Dp;  // the density blocks of the pairs, in Jp's layout (null: gather from D)
   // K rows in LDS span columns [kc0, kc0 + kw) of the bra's rows: kc0 = 0 and
   // kw = nbf for one molecule; in a multi-molecule basis (quartets within a
   // molecule) kc0 = pair_kc0[bra], the first function of the bra's molecule,
