@@ -242,7 +242,8 @@ def classes(lmax):
 
 def tu_source(la, lb, lc, ld):
     """Kernel variants of one class.  s/p classes carry everything (both Boys
-    modes, FP32, LDS digestion); d classes the production and parity set."""
+    modes, FP32, LDS digestion); d classes the production and parity set, with
+    LDS digestion up to L = 4."""
     key = f"{la}{lb}{lc}{ld}"
     sp = max(la, lb, lc, ld) <= 1
     A = f"{la}, {lb}, {lc}, {ld}"
@@ -270,6 +271,10 @@ def tu_source(la, lb, lc, ld):
         L += [f"  if (fp32) return bm == kBoysExact ? eri_jk_lds_launch<{A}, kBoysExact, float> : nullptr;",
               f"  return bm == kBoysExact ? eri_jk_lds_launch<{A}, kBoysExact, double>",
               f"                          : eri_jk_lds_launch<{A}, kBoysCompat, double>;"]
+    elif la + lb + lc + ld <= 4:
+        # d classes whose state fits a lane: LDS digestion for the production Boys
+        L += [f"  if (fp32) return bm == kBoysExact ? eri_jk_lds_launch<{A}, kBoysExact, float> : nullptr;",
+              f"  return bm == kBoysExact ? eri_jk_lds_launch<{A}, kBoysExact, double> : nullptr;"]
     else:
         L += ["  (void)bm;", "  (void)fp32;", "  return nullptr;"]
     L += ["}", "", "}  // namespace rchem", ""]

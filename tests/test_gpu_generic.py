@@ -87,3 +87,26 @@ def test_dshell_jk_matches_tensor(gpu_ctx, generic_ctx, generic):
     ref = np.array([O.contracted_eri(Bo, *q, boys=O.BOYS_EXACT) for q in idx])
     got = np.array([I[tuple(q)] for q in idx])
     assert np.max(np.abs(got - ref)) < 1e-12
+
+
+def test_dshell_jk_mixed_precision(gpu_ctx):
+    """The FP32 kernels of the d classes (per-lane, LDS digestion up to L = 4):
+    a mixed build with a loose threshold (many FP32 quartets) stays within
+    FP32 rounding of the all-FP64 build."""
+    from rchem_amd._lib import PREC_MIXED
+    B, _ = _dshell_basis(gpu_ctx)
+    n = B.nbf
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((n, n))
+    D = 0.05 * (X + X.T)
+    _say(f"case: d-shell basis nbf={n}, mixed precision (threshold 1e-3) vs FP64")
+    J = np.zeros((n, n)); K = np.zeros((n, n))
+    rb.JK_direct(J, K, B, D, default_opts(schwarz_thresh=1e-14))
+    Jm = np.zeros((n, n)); Km = np.zeros((n, n))
+    rb.JK_direct(Jm, Km, B, D, default_opts(schwarz_thresh=1e-14, precision=PREC_MIXED,
+                                            mixed_thresh=1e-3))
+    st = rb.jk_last_stats(B)
+    assert st.quartets_fp32 > 0
+    scale = max(np.max(np.abs(J)), np.max(np.abs(K)))
+    assert np.max(np.abs(Jm - J)) < 1e-6 * scale
+    assert np.max(np.abs(Km - K)) < 1e-6 * scale
