@@ -43,7 +43,6 @@ struct OneElArgs {
 
 constexpr int kLmax1 = 2;              // highest l served
 constexpr int kE = kLmax1 + 3;         // E index range for j up to lb + 2
-constexpr int kT = 2 * kLmax1 + 3;     // Hermite t range
 
 __device__ __forceinline__ int ncart1(int l) { return (l + 1) * (l + 2) / 2; }
 
@@ -68,32 +67,6 @@ __device__ __forceinline__ double comp_factor(const int* c) {
   for (int k = 0; k < 3; ++k)
     if (c[k] == 2) f *= 0.57735026918962576451;
   return f;
-}
-
-// E_t^{ij} for i <= imax, j <= jmax, one axis, without the exp(-mu X^2) factor.
-__device__ __forceinline__ void hermite_e(int imax, int jmax, double pa, double pb, double oo2p,
-                                          double E[kLmax1 + 1][kE][kT]) {
-  for (int i = 0; i <= imax; ++i)
-    for (int j = 0; j <= jmax; ++j)
-      for (int t = 0; t < kT; ++t) E[i][j][t] = 0.0;
-  E[0][0][0] = 1.0;
-  for (int i = 0; i <= imax; ++i) {
-    if (i > 0) {
-      for (int t = 0; t <= i; ++t) {
-        double v = pa * E[i - 1][0][t];
-        if (t > 0) v += oo2p * E[i - 1][0][t - 1];
-        if (t + 1 <= i - 1) v += (t + 1) * E[i - 1][0][t + 1];
-        E[i][0][t] = v;
-      }
-    }
-    for (int j = 1; j <= jmax; ++j)
-      for (int t = 0; t <= i + j; ++t) {
-        double v = pb * E[i][j - 1][t];
-        if (t > 0) v += oo2p * E[i][j - 1][t - 1];
-        if (t + 1 <= i + j - 1) v += (t + 1) * E[i][j - 1][t + 1];
-        E[i][j][t] = v;
-      }
-  }
 }
 
 // Hermite coefficients E_t^{ij}, i <= IMAX, j <= JMAX, one axis, with
