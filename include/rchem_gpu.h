@@ -199,7 +199,8 @@ typedef struct rchem_scf_result {
   int iterations;
   int converged;
   int fp64_from;   /* mixed precision: first iteration with all-FP64 J/K (the switch
-                      comes when |dE| < 1e-7 Eh); 0 for FP64 runs, -1 if never */
+                      comes when |dE| < 1e-7 Eh; in a batched SCF per molecule, at
+                      iteration 64 at the latest); 0 for FP64 runs, -1 if never */
 } rchem_scf_result;
 
 int rchem_scf_opts_default(rchem_scf_opts *o);

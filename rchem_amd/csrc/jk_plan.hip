@@ -77,17 +77,20 @@ __global__ void plan_kernel(const PlanArgs a) {
     const int i = pc.bra_begin + local / pc.nbk;
     const int4 bk = a.buckets[pc.bk_off + local % pc.nbk];
     int lo = bk.x, hi = bk.y;
+    bool all64 = false;
     if (a.segs) {  // the bra's molecule only
       const int mol = a.pair_mol[i];
       const int2 sg = a.segs[static_cast<size_t>(pc.bk_off + local % pc.nbk) * a.nmol + mol];
+      const int state = a.mol_active ? a.mol_active[mol] : 1;
       lo = sg.x;
-      hi = (a.mol_active && !a.mol_active[mol]) ? sg.x : sg.y;  // a retired molecule: none
+      hi = state == 0 ? sg.x : sg.y;  // a retired molecule: none
+      all64 = state == 2;             // a molecule past its precision switch: no FP32
     }
     if (pc.same) hi = (lo > i) ? lo : min(hi, i + 1);
     const double qi = a.Q[i];
     if (a.tau > 0.0) hi = (qi > 0.0) ? first_below(a.Q, lo, hi, a.tau / qi) : lo;
     int mid = hi;
-    if (a.mixed_thresh > 0.0 && hi > lo) {
+    if (a.mixed_thresh > 0.0 && hi > lo && !all64) {
       const double dm = *a.dmax;
       const double need = (qi > 0.0 && dm > 0.0) ? a.mixed_thresh / (qi * dm) : HUGE_VAL;
       mid = first_below(a.Q, lo, hi, need);

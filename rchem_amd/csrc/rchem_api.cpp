@@ -2427,7 +2427,11 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
   // converged molecules retire: the planner skips their quartets from the
   // next Fock build on, and their energy is kept from the iteration that
   // converged (their later J, K, F and D are no longer meaningful)
-  std::vector<int> active(B, 1);
+  // per molecule: 1 active, 2 active past its own switch to all-FP64 quartets
+  // (mixed precision; each molecule switches once its energy changes by less
+  // than 1e-7 Eh), 0 retired
+  std::vector<int> active(B, opts->jk.precision == RCHEM_PREC_MIXED ? 1 : 2);
+  std::vector<int> fp64_at(B, opts->jk.precision == RCHEM_PREC_MIXED ? -1 : 0);
   HIPC(ctx->mol_active_d.ensure(sizeof(int) * B));
   HIPC(hipMemcpyAsync(ctx->mol_active_d.p, active.data(), sizeof(int) * B, hipMemcpyHostToDevice,
                       st));
@@ -2438,7 +2442,6 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
     ~ActiveOff() { c->mol_active_on = false; }
   } active_off{ctx};
   rchem_opts jk = opts->jk;
-  int fp64_from = jk.precision == RCHEM_PREC_MIXED ? -1 : 0;
   const int k = std::min(std::max(opts->diis, 0), 32);
   DevBuf dFs, dEs, dDp, dE1, dpairs, ddots, dcoef, derr;
   if (k > 0) {
@@ -2498,30 +2501,30 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
       TRY(energies(dF.as<double>(), &e_cur));  // rchem's sum D_new o (H + F(D_old))
     }
     double demax = 0.0;
-    int settled = 0;
+    bool changed = false;
+    int mixed_left = 0;
     for (int b = 0; b < B; ++b) {
-      if (conv_at[b] >= 0) {
-        ++settled;
-        continue;
-      }
+      if (conv_at[b] >= 0) continue;
       const double d = std::fabs(e_cur[b] - e_prev[b]);
       demax = std::max(demax, d);
-      if (d < 1e-7) ++settled;
+      // per-molecule convergence-triggered switch to all-FP64 quartets (every
+      // molecule by iteration 64, so a stalled one cannot stay in FP32)
+      if (active[b] == 1 && it > 0 && (d < 1e-7 || it >= 64)) {
+        active[b] = 2;
+        fp64_at[b] = it + 1;
+        changed = true;
+      }
+      if (active[b] == 1) ++mixed_left;
     }
     if (opts->verbose) std::printf("%4d batch max|dE| %12.3e\n", it, demax);
-    // switch once 95 % of the molecules change by less than 1e-7 Eh (a molecule
-    // whose iterations stall must not keep the others in mixed precision), or
-    // after 64 iterations
-    if (it > 0 && jk.precision == RCHEM_PREC_MIXED && (20 * settled >= 19 * B || it >= 64)) {
-      jk.precision = RCHEM_PREC_FP64;  // convergence-triggered switch to all-FP64 quartets
-      fp64_from = it + 1;
-      if (k <= 0) continue;
-    }
+    if (jk.precision == RCHEM_PREC_MIXED && mixed_left == 0)
+      jk.precision = RCHEM_PREC_FP64;  // no FP32 quartets left: drop the FP32 kernels
     all_conv = true;
     bool retired = false;
     for (int b = 0; b < B; ++b) {
       if (conv_at[b] >= 0) continue;
-      const bool ok = it > 0 && jk.precision != RCHEM_PREC_MIXED &&
+      // converged: its energy of this iteration came from an all-FP64 Fock matrix
+      const bool ok = it > 0 && fp64_at[b] >= 0 && fp64_at[b] <= it &&
                       std::fabs(e_cur[b] - e_prev[b]) < opts->thresh_e &&
                       (k <= 0 || errmax[b] < 1e-8);
       if (ok) {
@@ -2537,7 +2540,7 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
       ++it;
       break;
     }
-    if (retired)
+    if (retired || changed)
       HIPC(hipMemcpyAsync(ctx->mol_active_d.p, active.data(), sizeof(int) * B,
                           hipMemcpyHostToDevice, st));
     if (k > 0) {
@@ -2593,7 +2596,7 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
     r.e_total = r.e_elec + enuc;
     r.iterations = conv_at[b] >= 0 ? conv_at[b] : it;
     r.converged = conv_at[b] >= 0 ? 1 : 0;
-    r.fp64_from = fp64_from;
+    r.fp64_from = fp64_at[b];
   }
   return all_conv ? RCHEM_OK : RCHEM_ERR_NOT_CONVERGED;
 }

@@ -157,7 +157,8 @@ struct PlanArgs {
   const int2* segs;        // null: one molecule
   int nmol;
   const int* pair_mol;     // molecule of each pair (layout order)
-  const int* mol_active;   // per molecule: 0 = retired (no quartets); null: all active
+  const int* mol_active;   // per molecule: 0 retired (no quartets), 1 active, 2 active with
+                           // every quartet in FP64 (past its precision switch); null: all 1
 };
 
 // Device buffers the planner fills (all sized by the host from upper bounds).
