@@ -299,32 +299,42 @@ struct OneElGradArgs {
   const double* D;
   const double* W;
   double* grad;
+  int cls_begin[10];  // pairs sorted by class 3 la + lb: class c is [cls_begin[c], cls_begin[c + 1])
 };
 
 constexpr int kIg = kLmax1 + 2;      // i range for a + 1
 constexpr int kTg = 2 * kLmax1 + 4;  // Hermite t range for (a + 1, b + 2)
 constexpr int kRg = 2 * kLmax1 + 2;  // R_tuv index range for L = la + lb + 1
 
-__device__ __forceinline__ void hermite_eg(int imax, int jmax, double pa, double pb, double oo2p,
-                                           double E[kIg][kE][kTg]) {
-  for (int i = 0; i <= imax; ++i)
-    for (int j = 0; j <= jmax; ++j)
+template <int IMAX, int JMAX>
+__device__ __forceinline__ void hermite_eg_t(double pa, double pb, double oo2p,
+                                             double (&E)[kIg][kE][kTg]) {
+  static_assert(IMAX < kIg && JMAX < kE && IMAX + JMAX < kTg, "Hermite table bounds");
+#pragma unroll
+  for (int i = 0; i <= IMAX; ++i)
+#pragma unroll
+    for (int j = 0; j <= JMAX; ++j)
+#pragma unroll
       for (int t = 0; t < kTg; ++t) E[i][j][t] = 0.0;
   E[0][0][0] = 1.0;
-  for (int i = 0; i <= imax; ++i) {
+#pragma unroll
+  for (int i = 0; i <= IMAX; ++i) {
     if (i > 0) {
+#pragma unroll
       for (int t = 0; t <= i; ++t) {
         double v = pa * E[i - 1][0][t];
-        if (t > 0) v += oo2p * E[i - 1][0][t - 1];
-        if (t + 1 <= i - 1) v += (t + 1) * E[i - 1][0][t + 1];
+        if (t > 0) v += oo2p * E[i - 1][0][t > 0 ? t - 1 : 0];
+        if (t + 1 <= i - 1) v += (t + 1) * E[i - 1][0][t + 1 < kTg ? t + 1 : 0];
         E[i][0][t] = v;
       }
     }
-    for (int j = 1; j <= jmax; ++j)
+#pragma unroll
+    for (int j = 1; j <= JMAX; ++j)
+#pragma unroll
       for (int t = 0; t <= i + j; ++t) {
         double v = pb * E[i][j - 1][t];
-        if (t > 0) v += oo2p * E[i][j - 1][t - 1];
-        if (t + 1 <= i + j - 1) v += (t + 1) * E[i][j - 1][t + 1];
+        if (t > 0) v += oo2p * E[i][j - 1][t > 0 ? t - 1 : 0];
+        if (t + 1 <= i + j - 1) v += (t + 1) * E[i][j - 1][t + 1 < kTg ? t + 1 : 0];
         E[i][j][t] = v;
       }
   }
@@ -365,29 +375,35 @@ __device__ __forceinline__ double nuc_g(const double (&E)[3][kIg][kE][kTg],
   return sum;
 }
 
+// One thread per (shell pair of class (LA, LB), nucleus), nucleus fastest.
+// The class is a template argument so every loop bound is known at compile
+// time and the Hermite and R_tuv arrays stay in registers; the Boys block of
+// order LA + LB + 1 is staged in LDS.
+template <int LA, int LB>
 __global__ __launch_bounds__(128) void one_electron_grad_kernel(const OneElGradArgs a) {
+  constexpr int L = LA + LB + 1;
+  constexpr int na = (LA + 1) * (LA + 2) / 2, nb = (LB + 1) * (LB + 2) / 2;
+  __shared__ double boys_tab[kBoysStageSize];
+  stage_boys<L>(boys_tab, a.boys_table);
+  __syncthreads();
+  const int c_cls = 3 * LA + LB;
+  const int p0 = a.cls_begin[c_cls], np = a.cls_begin[c_cls + 1] - p0;
   const long tid = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (tid >= static_cast<long>(a.npairs) * a.natom) return;
-  const int pid = static_cast<int>(tid / a.natom), c = static_cast<int>(tid % a.natom);
+  if (tid >= static_cast<long>(np) * a.natom) return;
+  const int pid = p0 + static_cast<int>(tid / a.natom), c = static_cast<int>(tid % a.natom);
   const int2 pr = a.pairs[pid];
   const ShellDev sa = a.shells[pr.x], sb = a.shells[pr.y];
-  const int la = sa.l, lb = sb.l;
-  const int na = ncart1(la), nb = ncart1(lb);
   const int atA = a.shell_atom[pr.x], atB = a.shell_atom[pr.y];
   const double f = pr.x != pr.y ? 2.0 : 1.0;  // the (b, a) block of the sum
-  int ca[6][3], cb[6][3];
-  for (int k = 0; k < na; ++k) cart_comp(la, k, ca[k]);
-  for (int k = 0; k < nb; ++k) cart_comp(lb, k, cb[k]);
   const double AB2 = (sa.xyz[0] - sb.xyz[0]) * (sa.xyz[0] - sb.xyz[0]) +
                      (sa.xyz[1] - sb.xyz[1]) * (sa.xyz[1] - sb.xyz[1]) +
                      (sa.xyz[2] - sb.xyz[2]) * (sa.xyz[2] - sb.xyz[2]);
   const double Cx = a.atom_xyz[3 * c], Cy = a.atom_xyz[3 * c + 1], Cz = a.atom_xyz[3 * c + 2];
   const size_t n = static_cast<size_t>(a.nbf);
-  const int L = la + lb + 1;
   double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0}, gC[3] = {0.0, 0.0, 0.0};
   double E[3][kIg][kE][kTg];
   double R[kRg][kRg][kRg], Rn[kRg][kRg][kRg];
-  double F[kRg];
+  double F[L + 1];
   for (int ip = 0; ip < sa.nprim; ++ip)
     for (int jp = 0; jp < sb.nprim; ++jp) {
       const double al = a.exps[sa.p0 + ip], be = a.exps[sb.p0 + jp];
@@ -395,60 +411,70 @@ __global__ __launch_bounds__(128) void one_electron_grad_kernel(const OneElGradA
       const double cc = a.ccoef[sa.p0 + ip] * a.ccoef[sb.p0 + jp];
       const double kab = exp(-al * be * oop * AB2);
       double P[3];
+#pragma unroll
       for (int x = 0; x < 3; ++x) {
         P[x] = (al * sa.xyz[x] + be * sb.xyz[x]) * oop;
-        hermite_eg(la + 1, lb + 2, P[x] - sa.xyz[x], P[x] - sb.xyz[x], 0.5 * oop, E[x]);
+        hermite_eg_t<LA + 1, LB + 2>(P[x] - sa.xyz[x], P[x] - sb.xyz[x], 0.5 * oop, E[x]);
       }
       // Hermite Coulomb integrals R_tuv(p, P - C), level by level (R_{n} from R_{n+1})
       const double PC[3] = {P[0] - Cx, P[1] - Cy, P[2] - Cz};
       const double T = p * (PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2]);
-      boys_exact_rt(L, T, a.boys_table, F);
+      boys_exact<L, double>(T, boys_tab, F);
       double m2p = 1.0;
+#pragma unroll
       for (int k = 0; k < L; ++k) m2p *= -2.0 * p;
       Rn[0][0][0] = m2p * F[L];
+#pragma unroll
       for (int lev = L - 1; lev >= 0; --lev) {
         m2p /= -2.0 * p;
-        const int top = L - lev;
-        for (int tt = 0; tt <= top; ++tt)
-          for (int uu = 0; uu <= top - tt; ++uu)
-            for (int vv = 0; vv <= top - tt - uu; ++vv) {
+#pragma unroll
+        for (int tt = 0; tt <= L - lev; ++tt)
+#pragma unroll
+          for (int uu = 0; uu <= L - lev - tt; ++uu)
+#pragma unroll
+            for (int vv = 0; vv <= L - lev - tt - uu; ++vv) {
               double r;
               if (tt + uu + vv == 0) {
                 r = m2p * F[lev];
               } else if (tt > 0) {
                 r = PC[0] * Rn[tt - 1][uu][vv];
-                if (tt > 1) r += (tt - 1) * Rn[tt - 2][uu][vv];
+                if (tt > 1) r += (tt - 1) * Rn[tt > 1 ? tt - 2 : 0][uu][vv];
               } else if (uu > 0) {
                 r = PC[1] * Rn[tt][uu - 1][vv];
-                if (uu > 1) r += (uu - 1) * Rn[tt][uu - 2][vv];
+                if (uu > 1) r += (uu - 1) * Rn[tt][uu > 1 ? uu - 2 : 0][vv];
               } else {
                 r = PC[2] * Rn[tt][uu][vv - 1];
-                if (vv > 1) r += (vv - 1) * Rn[tt][uu][vv - 2];
+                if (vv > 1) r += (vv - 1) * Rn[tt][uu][vv > 1 ? vv - 2 : 0];
               }
               R[tt][uu][vv] = r;
             }
-        for (int tt = 0; tt <= top; ++tt)
-          for (int uu = 0; uu <= top - tt; ++uu)
-            for (int vv = 0; vv <= top - tt - uu; ++vv) Rn[tt][uu][vv] = R[tt][uu][vv];
+#pragma unroll
+        for (int tt = 0; tt <= L - lev; ++tt)
+#pragma unroll
+          for (int uu = 0; uu <= L - lev - tt; ++uu)
+#pragma unroll
+            for (int vv = 0; vv <= L - lev - tt - uu; ++vv) Rn[tt][uu][vv] = R[tt][uu][vv];
       }
-      if (L == 0) R[0][0][0] = Rn[0][0][0];
       const double sq = sqrt(3.14159265358979323846 * oop);
       const double pre_s = cc * kab * sq * sq * sq;
       const double pre_v = -a.atom_z[c] * cc * kab * 2.0 * 3.14159265358979323846 * oop;
+#pragma unroll
       for (int i = 0; i < na; ++i)
+#pragma unroll
         for (int j = 0; j < nb; ++j) {
-          const int* A = ca[i];
-          const int* B = cb[j];
+          const int A[3] = {cart_x(LA, i), cart_y(LA, i), cart_z(LA, i)};
+          const int B[3] = {cart_x(LB, j), cart_y(LB, j), cart_z(LB, j)};
           const double w = f * comp_factor(A) * comp_factor(B);
           const size_t mu = sa.f0 + i, nu = sb.f0 + j;
           const double d2 = 2.0 * a.D[mu * n + nu], w2 = 2.0 * a.W[mu * n + nu];
+#pragma unroll
           for (int k = 0; k < 3; ++k) {
             int Ap[3] = {A[0], A[1], A[2]}, Am[3] = {A[0], A[1], A[2]};
             int Bp[3] = {B[0], B[1], B[2]}, Bm[3] = {B[0], B[1], B[2]};
             Ap[k] += 1;
-            Am[k] -= 1;
+            Am[k] = Am[k] > 0 ? Am[k] - 1 : 0;
             Bp[k] += 1;
-            Bm[k] -= 1;
+            Bm[k] = Bm[k] > 0 ? Bm[k] - 1 : 0;
             double dVA = 2.0 * al * nuc_g(E, R, Ap, B);
             if (A[k] > 0) dVA -= A[k] * nuc_g(E, R, Am, B);
             double dVB = 2.0 * be * nuc_g(E, R, A, Bp);
@@ -478,12 +504,29 @@ __global__ __launch_bounds__(128) void one_electron_grad_kernel(const OneElGradA
   }
 }
 
-hipError_t one_electron_grad_launch(const OneElGradArgs& a, hipStream_t s) {
-  const long total = static_cast<long>(a.npairs) * a.natom;
+template <int LA, int LB>
+hipError_t one_electron_grad_class(const OneElGradArgs& a, hipStream_t s) {
+  const int c = 3 * LA + LB;
+  const long total = static_cast<long>(a.cls_begin[c + 1] - a.cls_begin[c]) * a.natom;
   if (total <= 0) return hipSuccess;
-  hipLaunchKernelGGL(one_electron_grad_kernel, dim3(static_cast<unsigned>((total + 127) / 128)),
-                     dim3(128), 0, s, a);
+  hipLaunchKernelGGL((one_electron_grad_kernel<LA, LB>),
+                     dim3(static_cast<unsigned>((total + 127) / 128)), dim3(128), 0, s, a);
   return hipGetLastError();
+}
+
+// a.pairs must be sorted by class 3 la + lb with a.cls_begin its boundaries.
+hipError_t one_electron_grad_launch(const OneElGradArgs& a, hipStream_t s) {
+  if (a.npairs <= 0 || a.natom <= 0) return hipSuccess;
+  hipError_t e;
+  if ((e = one_electron_grad_class<0, 0>(a, s)) != hipSuccess) return e;
+  if ((e = one_electron_grad_class<0, 1>(a, s)) != hipSuccess) return e;
+  if ((e = one_electron_grad_class<0, 2>(a, s)) != hipSuccess) return e;
+  if ((e = one_electron_grad_class<1, 0>(a, s)) != hipSuccess) return e;
+  if ((e = one_electron_grad_class<1, 1>(a, s)) != hipSuccess) return e;
+  if ((e = one_electron_grad_class<1, 2>(a, s)) != hipSuccess) return e;
+  if ((e = one_electron_grad_class<2, 0>(a, s)) != hipSuccess) return e;
+  if ((e = one_electron_grad_class<2, 1>(a, s)) != hipSuccess) return e;
+  return one_electron_grad_class<2, 2>(a, s);
 }
 
 template <int LA, int LB>

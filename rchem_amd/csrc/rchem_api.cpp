@@ -137,6 +137,7 @@ struct OneElGradArgs {
   const double* D;
   const double* W;
   double* grad;
+  int cls_begin[10];
 };
 hipError_t one_electron_grad_launch(const OneElGradArgs& a, hipStream_t s);
 hipError_t one_electron_launch(const OneElArgs& a, hipStream_t s);
@@ -2743,6 +2744,8 @@ int rchem_rhf_gradient(rchem_ctx* ctx, int natom, const double* Z, const double*
   std::vector<int2> pr;
   for (int a = 0; a < ns; ++a)
     for (int b = 0; b <= a; ++b) pr.push_back(make_int2(a, b));
+  int gcls_begin[10];
+  sort_pairs_by_class(sh, &pr, gcls_begin);
   HIPC(dsh.ensure(sizeof(ShellDev) * ns));
   HIPC(dex.ensure(sizeof(double) * ctx->exps.size()));
   HIPC(dcc.ensure(sizeof(double) * ctx->ccoef.size()));
@@ -2775,6 +2778,7 @@ int rchem_rhf_gradient(rchem_ctx* ctx, int natom, const double* Z, const double*
   g.D = ctx->scf_D.as<double>();
   g.W = dW.as<double>();
   g.grad = dgrad.as<double>();
+  for (int c = 0; c < 10; ++c) g.cls_begin[c] = gcls_begin[c];
 
   HIPC(one_electron_grad_launch(g, ctx->stream));
   HIPC(hipMemcpyAsync(grad, dgrad.p, sizeof(double) * 3 * natom, hipMemcpyDeviceToHost,
