@@ -380,27 +380,19 @@ __device__ __forceinline__ double nuc_g(const double (&E)[3][kIg][kE][kTg],
 // time and the Hermite and R_tuv arrays stay in registers; the Boys block of
 // order LA + LB + 1 is staged in LDS.
 template <int LA, int LB>
-__global__ __launch_bounds__(128) void one_electron_grad_kernel(const OneElGradArgs a) {
+__device__ __forceinline__ void one_electron_grad_terms(const OneElGradArgs& a, int pid, int c,
+                                                        const double* __restrict__ boys_tab,
+                                                        double* gA, double* gB, double* gC) {
   constexpr int L = LA + LB + 1;
   constexpr int na = (LA + 1) * (LA + 2) / 2, nb = (LB + 1) * (LB + 2) / 2;
-  __shared__ double boys_tab[kBoysStageSize];
-  stage_boys<L>(boys_tab, a.boys_table);
-  __syncthreads();
-  const int c_cls = 3 * LA + LB;
-  const int p0 = a.cls_begin[c_cls], np = a.cls_begin[c_cls + 1] - p0;
-  const long tid = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (tid >= static_cast<long>(np) * a.natom) return;
-  const int pid = p0 + static_cast<int>(tid / a.natom), c = static_cast<int>(tid % a.natom);
   const int2 pr = a.pairs[pid];
   const ShellDev sa = a.shells[pr.x], sb = a.shells[pr.y];
-  const int atA = a.shell_atom[pr.x], atB = a.shell_atom[pr.y];
   const double f = pr.x != pr.y ? 2.0 : 1.0;  // the (b, a) block of the sum
   const double AB2 = (sa.xyz[0] - sb.xyz[0]) * (sa.xyz[0] - sb.xyz[0]) +
                      (sa.xyz[1] - sb.xyz[1]) * (sa.xyz[1] - sb.xyz[1]) +
                      (sa.xyz[2] - sb.xyz[2]) * (sa.xyz[2] - sb.xyz[2]);
   const double Cx = a.atom_xyz[3 * c], Cy = a.atom_xyz[3 * c + 1], Cz = a.atom_xyz[3 * c + 2];
   const size_t n = static_cast<size_t>(a.nbf);
-  double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0}, gC[3] = {0.0, 0.0, 0.0};
   double E[3][kIg][kE][kTg];
   double R[kRg][kRg][kRg], Rn[kRg][kRg][kRg];
   double F[L + 1];
@@ -497,10 +489,47 @@ __global__ __launch_bounds__(128) void one_electron_grad_kernel(const OneElGradA
           }
         }
     }
+}
+
+__device__ __forceinline__ double wave_sum_1(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Lanes are (pair, nucleus) with the nucleus fastest, so a wave mostly holds
+// one pair: its d/dA and d/dB terms are summed across the wave first (one
+// atomic per component instead of 64 on the same few addresses).
+template <int LA, int LB>
+__global__ __launch_bounds__(128) void one_electron_grad_kernel(const OneElGradArgs a) {
+  __shared__ double boys_tab[kBoysStageSize];
+  stage_boys<LA + LB + 1>(boys_tab, a.boys_table);
+  __syncthreads();
+  const int c_cls = 3 * LA + LB;
+  const int p0 = a.cls_begin[c_cls], np = a.cls_begin[c_cls + 1] - p0;
+  const long tid = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool valid = tid < static_cast<long>(np) * a.natom;
+  const int pid = valid ? p0 + static_cast<int>(tid / a.natom) : p0;
+  const int c = valid ? static_cast<int>(tid % a.natom) : 0;
+  double gA[3] = {0.0, 0.0, 0.0}, gB[3] = {0.0, 0.0, 0.0}, gC[3] = {0.0, 0.0, 0.0};
+  if (valid) one_electron_grad_terms<LA, LB>(a, pid, c, boys_tab, gA, gB, gC);
+  const int2 pr = a.pairs[pid];
+  const int atA = a.shell_atom[pr.x], atB = a.shell_atom[pr.y];
+  const int pid0 = __shfl(pid, 0, 64);
+  const bool uniform = __all(!valid || pid == pid0);
+#pragma unroll
   for (int k = 0; k < 3; ++k) {
-    unsafeAtomicAdd(&a.grad[3 * atA + k], gA[k]);
-    unsafeAtomicAdd(&a.grad[3 * atB + k], gB[k]);
-    unsafeAtomicAdd(&a.grad[3 * c + k], gC[k]);
+    if (uniform) {
+      const double sA = wave_sum_1(gA[k]), sB = wave_sum_1(gB[k]);
+      if ((threadIdx.x & 63) == 0) {
+        unsafeAtomicAdd(&a.grad[3 * atA + k], sA);
+        unsafeAtomicAdd(&a.grad[3 * atB + k], sB);
+      }
+    } else if (valid) {
+      unsafeAtomicAdd(&a.grad[3 * atA + k], gA[k]);
+      unsafeAtomicAdd(&a.grad[3 * atB + k], gB[k]);
+    }
+    if (valid) unsafeAtomicAdd(&a.grad[3 * c + k], gC[k]);
   }
 }
 
