@@ -248,8 +248,9 @@ def main():
                     help="also run one device copy of this many MiB (for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the Fock build is not sharded across GPUs in this "
-              "build (see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the library does not shard the Fock build across "
+              "GPUs (that would split the quartet list across ranks and all-reduce J and K once "
+              "per SCF iteration); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
