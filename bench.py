@@ -132,7 +132,7 @@ def batch_workload(args, ctx, t_start):
         "data": "synthetic",
         "config": {"workload": f"{len(mols)} random-walk molecules, 3-30 atoms from H/C/N/O, seed "
                                f"512, STO-3G ({min(nbf)}-{max(nbf)} functions), DIIS(8), "
-                               f"|dE| < 1e-10 Eh, max 128 iterations, chunks of <= 4096 functions"},
+                               f"|dE| < 1e-10 Eh, max 128 iterations, chunks of <= 8192 functions"},
         "fp64_seconds": t64,
         "mixed_seconds": tmx,
         "fp64_molecules_per_s": len(mols) / t64,

@@ -216,17 +216,18 @@ int rchem_scf(rchem_ctx *ctx, int natom, const double *Z, const double *xyz,
  * independent closed-shell molecules, molecule m with natom[m] atoms whose
  * atomic numbers and coordinates (bohr) follow those of molecule m-1 in Z and
  * xyz, all in basis basis_name.  Molecules are grouped into chunks of at most
- * 4096 basis functions; a chunk is one multi-molecule basis whose Fock builds
+ * 8192 basis functions; a chunk is one multi-molecule basis whose Fock builds
  * form only quartets within a molecule (one launch per class for every
  * molecule of the chunk), and whose SCF step runs batched over the molecules
  * on the device (strided-batched rocSOLVER eigensolves, per-molecule DIIS when
  * opts->diis > 0, else rchem's plain iterations).  opts->nocc is ignored (each
- * molecule has half its electron count); mixed precision switches the whole
- * chunk to FP64 quartets once 95 % of its molecules change by less than
- * 1e-7 Eh per iteration (or after 64 iterations).  The chunk iterates until
- * every molecule has converged or max_iter; res[m].converged and .iterations
- * are per molecule.  res receives
- * nmol results.  The context's basis is cleared afterwards. */
+ * molecule has half its electron count); in mixed precision each molecule
+ * switches to all-FP64 quartets once its energy changes by less than 1e-7 Eh
+ * per iteration (at iteration 64 at the latest).  A converged molecule retires
+ * from the Fock builds; the chunk iterates until every molecule has converged
+ * or max_iter.  res receives nmol results; converged, iterations and fp64_from
+ * are per molecule.  opts->graph replays each chunk's Fock build as a hipGraph;
+ * opts->purify is ignored.  The context's basis is cleared afterwards. */
 int rchem_batch_scf(rchem_ctx *ctx, int nmol, const int *natom, const int *Z,
                     const double *xyz, const char *basis_name,
                     const rchem_scf_opts *opts, rchem_scf_result *res);

@@ -2226,7 +2226,7 @@ extern "C" {
 
 namespace {
 
-constexpr int kBatchMaxNbf = 4096;    // basis functions per chunk of molecules
+constexpr int kBatchMaxNbf = 8192;    // basis functions per chunk of molecules
 constexpr double kBatchPadF = 1.0e4;  // Fock diagonal of a padded block's unused rows
 
 // One chunk of the batched SCF: molecules [m0, m1) of the caller's lists as one
