@@ -300,6 +300,9 @@ __device__ void epilogue(const EriArgs& a, const GenericProgram& p, const Tile& 
     const int nc = NA * NB * NC * ND;
     const double deg = 8.0 * sym_weight(bp, kp, bidx, kidx);
     const double* __restrict__ D = a.D;
+    // a per-centre program holds that centre's 3 x nc outputs: map them onto
+    // rows 3 centre .. 3 centre + 2 of g, the others stay zero
+    const int c0 = p.centre < 0 ? 0 : 3 * p.centre, nrow = p.centre < 0 ? 9 : 3;
     double g[9];
 #pragma unroll
     for (int c = 0; c < 9; ++c) g[c] = 0.0;
@@ -309,7 +312,8 @@ __device__ void epilogue(const EriArgs& a, const GenericProgram& p, const Tile& 
       const double gam = 2.0 * D[A * n + B] * D[C * n + Dd] -
                          0.5 * (D[A * n + C] * D[B * n + Dd] + D[A * n + Dd] * D[B * n + C]);
 #pragma unroll
-      for (int c = 0; c < 9; ++c) g[c] += gam * output(p, H, c * nc + k);
+      for (int c = 0; c < 9; ++c)
+        if (c >= c0 && c < c0 + nrow) g[c] += gam * output(p, H, (c - c0) * nc + k);
     }
     // six accumulators per pair (its two atoms' x, y, z), added onto the
     // atoms once every class is done: per-atom atomics would contend

@@ -283,7 +283,8 @@ def tu_source(la, lb, lc, ld):
 
 def write_programs(path, lmax):
     """Operation tables of the wave-cooperative kernel: the integrals of every
-    class, and the gradient's derivative integrals where they fit its LDS."""
+    class and the gradient's derivative integrals (grad = 1), split into one
+    program per centre (grad = 2, 3, 4) where the full one exceeds its LDS."""
     from gen_eri_prog import eri_program, grad_program
 
     def arr(ctype, name, vals, per_line=8):
@@ -299,8 +300,13 @@ def write_programs(path, lmax):
     out = ["// Generated by gen_eri.py -- do not edit.", ""]
     entries = []
     for (la, lb, lc, ld) in classes(lmax):
-        for kind, fn in (("E", eri_program), ("G", grad_program)):
-            pr = fn(la, lb, lc, ld)
+        variants = [("E", 0, eri_program(la, lb, lc, ld))]
+        g = grad_program(la, lb, lc, ld)
+        if g["P"] > 0:
+            variants.append(("G", 1, g))
+        else:  # too large for a workgroup's LDS: one program per centre (grad = 2 + centre)
+            variants += [(f"G{c}_", 2 + c, grad_program(la, lb, lc, ld, c)) for c in range(3)]
+        for kind, gflag, pr in variants:
             if pr["P"] == 0:
                 continue  # does not fit a workgroup's LDS
             key = f"{kind}{la}{lb}{lc}{ld}"
@@ -312,7 +318,7 @@ def write_programs(path, lmax):
             out += arr("int", f"kProgOut_{key}", [str(x) for o in pr["outs"] for x in (o[0], o[1], o[2], 0)])
             out += arr("double", f"kProgNorm_{key}", [repr(o[3]) for o in pr["outs"]], 4)
             entries.append(
-                f"    {{{la}, {lb}, {lc}, {ld}, {int(kind == 'G')}, {pr['L']}, {pr['nacc']}, "
+                f"    {{{la}, {lb}, {lc}, {ld}, {gflag}, {pr['L']}, {pr['nacc']}, "
                 f"{pr['nv']}, {pr['nh']}, {len(pr['outs'])}, {len(pr['vops'])}, "
                 f"{len(pr['vgroups'])}, {len(pr['accs'])}, {len(pr['hops'])}, "
                 f"{len(pr['hgroups'])}, {pr['P']}, kProgV_{key}, kProgVG_{key}, kProgA_{key}, "

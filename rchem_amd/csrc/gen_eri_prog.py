@@ -253,13 +253,16 @@ def _choose_p(L, nacc, nv, nh):
     return p if lds_bytes(L, nacc, nv, nh, p) <= 64 * 1024 else 0
 
 
-def _assemble(L, accsets, vrr_targets, outs_spec):
+def _assemble(L, accsets, vrr_targets, outs_spec, weights=None):
+    """weights[s]: the accumulation weight of set s (0: 1, 1: 2 alpha_a,
+    2: 2 alpha_b, 3: 2 alpha_c); by default the set's own index."""
     v = vrr_program(L, vrr_targets)
     acc_list = []
     for s, (E, F, base) in enumerate(accsets):
+        w = s if weights is None else weights[s]
         for ie, e in enumerate(E):
             for jf, f in enumerate(F):
-                acc_list.append((v["target_slot"][(e, f)] | s << 16, base + ie * len(F) + jf))
+                acc_list.append((v["target_slot"][(e, f)] | w << 16, base + ie * len(F) + jf))
     h = _hrr(accsets, outs_spec)
     nacc = h["nacc"]
     P = _choose_p(L, nacc, v["nslots"], h["nslots"])
@@ -281,24 +284,29 @@ def eri_program(la, lb, lc, ld):
     return _assemble(la + lb + lc + ld, [(E, F, 0)], [(e, f) for e in E for f in F], outs)
 
 
-def grad_program(la, lb, lc, ld):
+def grad_program(la, lb, lc, ld, centre=None):
     """Derivatives of (ab|cd) with respect to the centres A, B and C (D follows
     from translational invariance), outputs ordered (A, B, C) x (x, y, z) x
     canonical components.  Sets: 0 unweighted, 1 by 2 alpha_a, 2 by 2 alpha_b,
-    3 by 2 alpha_c."""
-    sets = [
-        (_range(la - 1, la + lb), _range(lc - 1, lc + ld)),   # minus terms
+    3 by 2 alpha_c.  With centre = 0, 1 or 2 only that centre's derivatives
+    ((x, y, z) x components, sets 0 and 1 + centre): for the classes whose
+    full program does not fit a workgroup's LDS, run once per centre."""
+    plus = [
         (_range(la + 1, la + lb + 1), _range(lc, lc + ld)),   # A plus
         (_range(la, la + lb + 1), _range(lc, lc + ld)),       # B plus
         (_range(la, la + lb), _range(lc + 1, lc + ld + 1)),   # C plus
     ]
+    minus = (_range(la - 1, la + lb), _range(lc - 1, lc + ld))
+    centres = range(3) if centre is None else [centre]
+    sets = [minus] + ([plus[c] for c in centres])
+    weights = [0] + [1 + c for c in centres]
     accsets, base = [], 0
     for E, F in sets:
         accsets.append((E, F, base))
         base += len(E) * len(F)
     targets = sorted({(e, f) for E, F in sets for e in E for f in F})
     outs = []
-    for centre in range(3):
+    for ci, centre in enumerate(centres):
         for i in range(3):
             for a in cart(la):
                 for b in cart(lb):
@@ -314,8 +322,8 @@ def grad_program(la, lb, lc, ld):
                                 minus = list(t)
                                 minus[centre] = dec(t[centre], i)
                                 minus = (0, *minus)
-                            outs.append(((1 + centre, *plus), minus, n, nrm))
-    return _assemble(la + lb + lc + ld + 1, accsets, targets, outs)
+                            outs.append(((1 + ci, *plus), minus, n, nrm))
+    return _assemble(la + lb + lc + ld + 1, accsets, targets, outs, weights)
 
 
 if __name__ == "__main__":

@@ -39,7 +39,8 @@ hipError_t eri_generic_launch(const EriArgs& a, const GenericProgram& p, int mod
 size_t eri_generic_lds_bytes(const GenericProgram& p);
 
 // Operation tables of the wave-cooperative kernel (host copies): the integrals
-// of every class, and the derivative integrals of the gradient (grad = 1).
+// of every class, and the derivative integrals of the gradient (grad = 1, or
+// one program per centre, grad = 2 + centre, where the full one is too large).
 struct HostProgram {
   int la, lb, lc, ld, grad, L, nacc, nv, nh, nout, nvops, nvgroups, naccs, nhops, nhgroups, P;
   const uint32_t* vops;
@@ -626,6 +627,7 @@ int upload_programs(rchem_ctx* ctx) {
     g.la = h.la; g.lb = h.lb; g.lc = h.lc; g.ld = h.ld;
     g.L = h.L; g.nacc = h.nacc; g.nv = h.nv; g.nh = h.nh; g.nout = h.nout;
     g.nvgroups = h.nvgroups; g.nhgroups = h.nhgroups; g.naccs = h.naccs; g.P = h.P;
+    g.centre = h.grad >= 2 ? h.grad - 2 : -1;
     g.vops = static_cast<const uint4*>(put(h.vops, h.nvops, 16));
     g.vgroups = static_cast<const uint4*>(put(h.vgroups, h.nvgroups, 16));
     g.accs = static_cast<const uint2*>(put(h.accs, h.naccs, 8));
@@ -639,8 +641,9 @@ int upload_programs(rchem_ctx* ctx) {
   return RCHEM_OK;
 }
 
-// The wave-cooperative program of class (la lb|lc ld): integrals (grad = 0)
-// or derivative integrals (grad = 1); nullptr when the library has none.
+// The wave-cooperative program of class (la lb|lc ld): integrals (grad = 0),
+// derivative integrals (grad = 1) or those of one centre (grad = 2 + centre);
+// nullptr when the library has none.
 int find_program(rchem_ctx* ctx, int la, int lb, int lc, int ld, int grad,
                  const GenericProgram** out) {
   *out = nullptr;
@@ -2724,11 +2727,19 @@ int rchem_rhf_gradient(rchem_ctx* ctx, int natom, const double* Z, const double*
     }
     const GenericProgram* gp = nullptr;
     TRY(find_program(ctx, hc.la, hc.lb, hc.lc, hc.ld, 1, &gp));
-    if (!gp)
-      return ctx->fail(RCHEM_ERR_UNSUPPORTED, "gradient of ERI class " +
-                                                  class_name(hc.la, hc.lb, hc.lc, hc.ld) +
-                                                  " not served by this library");
-    HIPC(eri_generic_launch(a, *gp, kModeGrad, kBoysExact, 0, ctx->stream));
+    if (gp) {
+      HIPC(eri_generic_launch(a, *gp, kModeGrad, kBoysExact, 0, ctx->stream));
+      continue;
+    }
+    // one program per centre
+    for (int c = 0; c < 3; ++c) {
+      TRY(find_program(ctx, hc.la, hc.lb, hc.lc, hc.ld, 2 + c, &gp));
+      if (!gp)
+        return ctx->fail(RCHEM_ERR_UNSUPPORTED, "gradient of ERI class " +
+                                                    class_name(hc.la, hc.lb, hc.lc, hc.ld) +
+                                                    " not served by this library");
+      HIPC(eri_generic_launch(a, *gp, kModeGrad, kBoysExact, 0, ctx->stream));
+    }
   }
   HIPC(grad_pairs_launch(dgp.as<double>(), dpa.as<int2>(), npairs, dgrad.as<double>(),
                          ctx->stream));

@@ -115,6 +115,8 @@ struct GenericProgram {
   int nout;       // outputs (integrals, or 9 x integrals for the gradient)
   int nvgroups, nhgroups, naccs;
   int P;          // primitive quartets per batch (power of two)
+  int centre;     // gradient program of one centre (0 A, 1 B, 2 C; outputs 3 x components),
+                  // -1: all three (outputs 9 x components) or not a gradient program
   const uint4* vops;      // VRR operations
   const uint4* vgroups;   // (op_begin, op_end, side | axis << 1 | level_end << 3, 0)
   const uint2* accs;      // (slot | set << 16, accumulator)

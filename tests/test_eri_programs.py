@@ -225,3 +225,39 @@ def test_straight_line_gradient_code_matches_tables(cls):
     got = _run_straight(translate(la, lb, lc, ld), la, lb, lc, ld, shells)
     assert got.shape == ref.shape
     assert np.max(np.abs(got - ref)) < 1e-13 * max(1.0, np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("key", ["1100", "2111"])
+def test_per_centre_grad_programs_match_full_program(key):
+    """A per-centre gradient program (the form the kernel runs where the full
+    program exceeds a workgroup's LDS) gives exactly that centre's slice of
+    the full program's outputs."""
+    la, lb, lc, ld = (int(x) for x in key)
+    shells = _random_shells([la, lb, lc, ld], int(key) + 71)
+    full = _execute(grad_program(la, lb, lc, ld), la, lb, lc, ld, shells).reshape(3, -1)
+    for centre in range(3):
+        part = _execute(grad_program(la, lb, lc, ld, centre), la, lb, lc, ld, shells)
+        assert np.max(np.abs(part - full[centre])) < 1e-13 * max(1.0, np.max(np.abs(full)))
+
+
+@pytest.mark.parametrize("key", ["2221", "2222"])
+def test_per_centre_grad_programs_finite_differences(key):
+    """(dd|dp) and (dd|dd): per-centre derivative integrals against central
+    differences (h = 1e-4 bohr) of the oracle's contracted integrals."""
+    la, lb, lc, ld = (int(x) for x in key)
+    ls = [la, lb, lc, ld]
+    shells = _random_shells(ls, int(key) + 29)
+    h = 1e-4
+    for centre in range(3):
+        pr = grad_program(la, lb, lc, ld, centre)
+        got = _execute(pr, la, lb, lc, ld, shells).reshape(3, -1)
+        for i in range(3):
+            sp = [list(s) for s in shells]
+            sm = [list(s) for s in shells]
+            sp[centre][0] = shells[centre][0].copy()
+            sm[centre][0] = shells[centre][0].copy()
+            sp[centre][0][i] += h
+            sm[centre][0][i] -= h
+            fd = (_oracle_block(ls, sp) - _oracle_block(ls, sm)) / (2 * h)
+            scale = max(1.0, np.max(np.abs(fd)))
+            assert np.max(np.abs(got[i] - fd)) < 2e-7 * scale, (centre, i)
