@@ -241,9 +241,8 @@ int rchem_batch_scf(rchem_ctx *ctx, int nmol, const int *natom, const int *Z,
  *   Gamma = 2 D_mn D_ls - (D_ml D_ns + D_ms D_nl) / 2,  W = C_occ eps_occ C_occ^T,
  * with the derivative integrals recomputed on the device (no quartet is
  * stored).  grad: 3 x natom doubles, Eh/bohr, atom-major.  Every shell centre
- * must coincide with one of the nuclei.  res as for rchem_scf.  Classes whose
- * derivative recurrences exceed one workgroup's LDS ((dd|dp) and (dd|dd))
- * return RCHEM_ERR_UNSUPPORTED. */
+ * must coincide with one of the nuclei.  res as for rchem_scf.  Every class
+ * with l <= 2 is served (s, p and d shells). */
 int rchem_rhf_gradient(rchem_ctx *ctx, int natom, const double *Z, const double *xyz,
                        const rchem_scf_opts *opts, rchem_scf_result *res, double *grad);
 
