@@ -116,7 +116,9 @@ int rchem_one_electron(rchem_ctx *ctx, int natom, const double *Z, const double 
 int rchem_jk(rchem_ctx *ctx, const double *D, double *J, double *K, const rchem_opts *opts);
 
 /* Same with device pointers (HBM-resident D, J, K; n x n doubles each).
- * Runs on the context's stream and returns without synchronising. */
+ * Runs on the context's stream and returns without synchronising, so D is
+ * not inspected on the host: a non-finite entry of D propagates into J and K
+ * (never screened away), where rchem_jk returns RCHEM_ERR_NONFINITE. */
 int rchem_jk_device(rchem_ctx *ctx, const double *dD, double *dJ, double *dK,
                     const rchem_opts *opts);
 
@@ -242,7 +244,10 @@ int rchem_batch_scf(rchem_ctx *ctx, int nmol, const int *natom, const int *Z,
  * with the derivative integrals recomputed on the device (no quartet is
  * stored).  grad: 3 x natom doubles, Eh/bohr, atom-major.  Every shell centre
  * must coincide with one of the nuclei.  res as for rchem_scf.  Every class
- * with l <= 2 is served (s, p and d shells). */
+ * with l <= 2 is served (s, p and d shells).  If the SCF does not converge in
+ * opts->max_iter iterations the call returns RCHEM_ERR_NOT_CONVERGED, fills
+ * res, and leaves grad untouched (the formula holds at a stationary density
+ * only). */
 int rchem_rhf_gradient(rchem_ctx *ctx, int natom, const double *Z, const double *xyz,
                        const rchem_scf_opts *opts, rchem_scf_result *res, double *grad);
 

@@ -394,9 +394,13 @@ int set_device(rchem_ctx* ctx) {
 
 // Finishes a basis: function offsets, radial norms folded into coefficients.
 int finish_basis(rchem_ctx* ctx) {
+  // a rejected basis leaves the context without one (never half of the old one)
+  ctx->have_basis = false;
   int f = 0;
   ctx->ccoef.resize(ctx->coefs.size());
   for (auto& s : ctx->shells) {
+    for (int x = 0; x < 3; ++x)
+      if (!std::isfinite(s.xyz[x])) return ctx->fail(RCHEM_ERR_INVALID, "non-finite centre");
     if (s.l < 0 || s.l > kMaxL)
       return ctx->fail(RCHEM_ERR_UNSUPPORTED,
                        "angular momentum " + std::to_string(s.l) + " not supported (max 2)");
@@ -409,6 +413,8 @@ int finish_basis(rchem_ctx* ctx) {
       const double a = ctx->exps[s.p0 + p];
       if (!(a > 0.0) || !std::isfinite(a))
         return ctx->fail(RCHEM_ERR_INVALID, "non-positive or non-finite exponent");
+      if (!std::isfinite(ctx->coefs[s.p0 + p]))
+        return ctx->fail(RCHEM_ERR_INVALID, "non-finite contraction coefficient");
       // exponent-dependent part of PGTO::normalization (basis.rs:140-149)
       const int L = s.l;
       const double nrad = std::sqrt(std::pow(2.0, 2.0 * L + 1.5) * std::pow(a, L + 1.5) /
@@ -1399,6 +1405,7 @@ int rchem_basis_upload(rchem_ctx* ctx, int nshell, const double* xyz, const int*
   if (!ctx) return RCHEM_ERR_INVALID;
   if (nshell <= 0 || !xyz || !l || !nprim || !exps || !coefs)
     return ctx->fail(RCHEM_ERR_INVALID, "rchem_basis_upload: bad arguments");
+  ctx->have_basis = false;
   ctx->shells.assign(nshell, Shell{});
   ctx->exps.clear();
   ctx->coefs.clear();
@@ -1435,6 +1442,7 @@ int rchem_basis_from_atoms(rchem_ctx* ctx, int natom, const int* Z, const double
   for (int r = 0; r < kBasisNumRows; ++r)
     if (lower(kBasisRows[r].set) == want) known = true;
   if (!known) return ctx->fail(RCHEM_ERR_INVALID, std::string("unknown basis set ") + basis_name);
+  ctx->have_basis = false;
   ctx->shells.clear();
   ctx->exps.clear();
   ctx->coefs.clear();
@@ -1798,6 +1806,10 @@ int rchem_one_electron(rchem_ctx* ctx, int natom, const double* Z, const double*
   if (!ctx->have_basis) return ctx->fail(RCHEM_ERR_NO_BASIS, "no basis uploaded");
   if (natom < 0 || (natom > 0 && (!Z || !xyz)))
     return ctx->fail(RCHEM_ERR_INVALID, "bad nuclei");
+  for (int a = 0; a < natom; ++a)
+    if (!std::isfinite(Z[a]) || !std::isfinite(xyz[3 * a]) || !std::isfinite(xyz[3 * a + 1]) ||
+        !std::isfinite(xyz[3 * a + 2]))
+      return ctx->fail(RCHEM_ERR_INVALID, "non-finite nuclear charge or position");
   TRY(set_device(ctx));
   const int ns = static_cast<int>(ctx->shells.size());
   std::vector<ShellDev> sh(ns);
@@ -2665,7 +2677,11 @@ int rchem_rhf_gradient(rchem_ctx* ctx, int natom, const double* Z, const double*
   if (!grad) return ctx->fail(RCHEM_ERR_INVALID, "null gradient");
   if (natom <= 0 || !Z || !xyz) return ctx->fail(RCHEM_ERR_INVALID, "bad nuclei");
   const int rc = scf_run(ctx, natom, Z, xyz, opts, res, 1e-9);
-  if (rc != RCHEM_OK && rc != RCHEM_ERR_NOT_CONVERGED) return rc;
+  // The gradient formula holds at a stationary density only, and W needs the
+  // orbitals and orbital energies of the converged Fock matrix (with purified
+  // densities the eigensolver runs only on convergence): without convergence
+  // no gradient is formed and grad is left as the caller passed it.
+  if (rc != RCHEM_OK) return rc;
   const int n = ctx->nbf, ns = static_cast<int>(ctx->shells.size());
   const size_t n2 = static_cast<size_t>(n) * n;
   // energy-weighted density W = C_occ diag(eps_occ) C_occ^T (column-major C)

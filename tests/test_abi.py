@@ -44,3 +44,20 @@ def test_opts_defaults_without_gpu():
     o = _lib.default_opts()
     assert o.boys_mode == _lib.BOYS_EXACT and o.precision == _lib.PREC_FP64
     assert o.schwarz_thresh == 1e-12 and o.mixed_thresh == 1e-7
+
+
+def test_context_without_device_fails_cleanly():
+    """Where no gfx950 device is visible (this CPU-only run), creating a
+    context returns RCHEM_ERR_NO_DEVICE with a message and no handle -- there
+    is no CPU fallback to fall into."""
+    import pytest
+    torch = pytest.importorskip("torch")
+    if torch.cuda.is_available():
+        pytest.skip("a device is visible")
+    L = _lib.lib()
+    h = ctypes.c_void_p(1)
+    assert L.rchem_ctx_create(0, ctypes.byref(h)) == 3
+    assert h.value is None
+    assert L.rchem_last_error(None)
+    assert L.rchem_opts_default(None) == 1
+    assert L.rchem_jk(None, None, None, None, None) == 1
