@@ -122,6 +122,24 @@ int rchem_jk(rchem_ctx *ctx, const double *D, double *J, double *K, const rchem_
 int rchem_jk_device(rchem_ctx *ctx, const double *dD, double *dJ, double *dK,
                     const rchem_opts *opts);
 
+/* The same build without allocating: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in rchem.  The build's accumulators (the
+ * lower-half J and K, the pair-indexed Coulomb accumulators and density
+ * blocks, the planner's statistics) live in a fixed 48 MiB device array that
+ * the library holds on every GPU it runs on until the process ends; the call
+ * allocates and frees nothing.  It uses the context's plan for
+ * opts->schwarz_thresh, which the first rchem_jk / rchem_jk_device call with
+ * the same threshold builds (that call allocates): without one, or when the
+ * basis needs more than 48 MiB of accumulators (about 3 n^2 doubles: n up to
+ * roughly 1400 functions), it returns RCHEM_ERR_INVALID and launches nothing.
+ * Every call on one GPU shares the array, whatever its context, and keeps the
+ * last call's contents: calls that overlap on one GPU (two streams, two
+ * threads, two contexts) race and must be serialised by the caller.  Per-class
+ * timing is not recorded.  Same results as rchem_jk_device. */
+int rchem_jk_device_noalloc(rchem_ctx *ctx, const double *dD, double *dJ, double *dK,
+                            const rchem_opts *opts);
+
 /* build_I(basis) (basis.rs:430-460): the full n^4 tensor [mu,nu,la,si]
  * row-major into host memory (n^4 doubles).  No screening. */
 int rchem_eri_tensor(rchem_ctx *ctx, double *I, const rchem_opts *opts);

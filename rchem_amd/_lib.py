@@ -74,6 +74,9 @@ SIGNATURES = {
     "rchem_jk": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, ctypes.POINTER(Opts)]),
     "rchem_jk_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                        ctypes.c_void_p, ctypes.POINTER(Opts)]),
+    "rchem_jk_device_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.POINTER(Opts)]),
     "rchem_eri_tensor": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(Opts)]),
     "rchem_schwarz": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(Opts)]),
     "rchem_eri_shell_quartets": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip, _dp,

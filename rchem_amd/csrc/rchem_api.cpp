@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -306,6 +307,7 @@ struct rchem_ctx {
   DevBuf scf_D, scf_C, scf_w, scf_Dold;
   DevBuf work_Jh, work_Kh, work_D, work_J, work_K;
   rchem_jk_stats last_stats{};
+  const double* last_stats_dev = nullptr;  // per-class planner statistics of the last build
   bool last_mixed = false;
   bool last_mixed_pending = false;  // the build being launched is mixed
 
@@ -1167,8 +1169,8 @@ constexpr int kPersistentBlocks = 2048;  // 256 CUs x 8 workgroups of 4 waves
 constexpr int kChunkTiles = 64;          // tiles per LDS-digestion chunk
 
 int launch_timed(rchem_ctx* ctx, const QClass& qc, int fp32, const ClassKernel& fn,
-                 const EriArgs& a, int blocks) {
-  const bool timed = ctx->profiling;
+                 const EriArgs& a, int blocks, bool may_time = true) {
+  const bool timed = ctx->profiling && may_time;
   std::pair<hipEvent_t, hipEvent_t> ev{};
   if (timed) {
     TRY(take_events(ctx, &ev));
@@ -1186,28 +1188,88 @@ int launch_timed(rchem_ctx* ctx, const QClass& qc, int fp32, const ClassKernel& 
   return RCHEM_OK;
 }
 
-int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rchem_opts* o) {
+// Per-build accumulators of a Fock build: the lower-half J and K, the
+// pair-indexed Coulomb accumulators and density blocks, the planner's
+// per-class statistics and max |D|.  Every one is cleared or fully rewritten
+// at the start of a build.
+struct JkBuffers {
+  double* Jh;
+  double* Kh;
+  double* jp;
+  double* dp;
+  double* stats;
+  double* dmax;
+};
+
+// Doubles a build's JkBuffers take for the current plan, each part rounded up
+// to 256 bytes.
+size_t jk_buffer_doubles(const rchem_ctx* ctx) {
+  auto up = [](size_t d) { return (d + 31) / 32 * 32; };
+  const size_t n2 = static_cast<size_t>(ctx->nbf) * ctx->nbf;
+  const size_t jp = std::max<size_t>(ctx->jp_size, 1);
+  const size_t nc = std::max<size_t>(ctx->pclasses.size(), 1);
+  return 2 * up(n2) + 2 * up(jp) + up(4 * nc) + up(1);
+}
+
+// The build's accumulators carved out of one device range of
+// jk_buffer_doubles(ctx) doubles.
+JkBuffers jk_buffers_in(const rchem_ctx* ctx, double* base) {
+  auto up = [](size_t d) { return (d + 31) / 32 * 32; };
+  const size_t n2 = static_cast<size_t>(ctx->nbf) * ctx->nbf;
+  const size_t jp = std::max<size_t>(ctx->jp_size, 1);
+  const size_t nc = std::max<size_t>(ctx->pclasses.size(), 1);
+  JkBuffers b{};
+  double* p = base;
+  b.Jh = p;
+  p += up(n2);
+  b.Kh = p;
+  p += up(n2);
+  b.jp = p;
+  p += up(jp);
+  b.dp = p;
+  p += up(jp);
+  b.stats = p;
+  p += up(4 * nc);
+  b.dmax = p;
+  return b;
+}
+
+// One Fock build.  With ext == nullptr the accumulators are the context's own
+// (allocated on first use) and per-class timing follows ctx->profiling; with
+// ext the build allocates nothing and records no events (the plan for the
+// threshold must exist already).
+int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rchem_opts* o,
+              const JkBuffers* ext = nullptr) {
   TRY(prepare_plan(ctx, o->schwarz_thresh));
   const size_t n2 = static_cast<size_t>(ctx->nbf) * ctx->nbf;
   const bool mixed = o->precision == RCHEM_PREC_MIXED;
   ctx->last_mixed_pending = mixed;
-  HIPC(ctx->work_Jh.ensure(n2 * sizeof(double)));
-  HIPC(ctx->work_Kh.ensure(n2 * sizeof(double)));
-  HIPC(fill_launch(ctx->work_Jh.as<double>(), n2, 0.0, ctx->stream));
-  HIPC(fill_launch(ctx->work_Kh.as<double>(), n2, 0.0, ctx->stream));
-  const int nc = static_cast<int>(ctx->pclasses.size());
-  HIPC(fill_launch(ctx->plan_stats.as<double>(), 4 * static_cast<size_t>(std::max(nc, 1)), 0.0,
-                   ctx->stream));
-  if (mixed) {
-    HIPC(fill_launch(ctx->plan_dmax.as<double>(), 1, 0.0, ctx->stream));
-    HIPC(dmax_launch(dD, n2, ctx->plan_dmax.as<unsigned long long>(), ctx->stream));
+  JkBuffers bf{};
+  if (ext) {
+    bf = *ext;
+  } else {
+    HIPC(ctx->work_Jh.ensure(n2 * sizeof(double)));
+    HIPC(ctx->work_Kh.ensure(n2 * sizeof(double)));
+    HIPC(ctx->dp_d.ensure(sizeof(double) * std::max<size_t>(ctx->jp_size, 1)));
+    bf = JkBuffers{ctx->work_Jh.as<double>(), ctx->work_Kh.as<double>(), ctx->jp_d.as<double>(),
+                   ctx->dp_d.as<double>(), ctx->plan_stats.as<double>(),
+                   ctx->plan_dmax.as<double>()};
   }
-  HIPC(fill_launch(ctx->jp_d.as<double>(), std::max<size_t>(ctx->jp_size, 1), 0.0, ctx->stream));
+  ctx->last_stats_dev = bf.stats;
+  const bool timed_ok = ext == nullptr;
+  HIPC(fill_launch(bf.Jh, n2, 0.0, ctx->stream));
+  HIPC(fill_launch(bf.Kh, n2, 0.0, ctx->stream));
+  const int nc = static_cast<int>(ctx->pclasses.size());
+  HIPC(fill_launch(bf.stats, 4 * static_cast<size_t>(std::max(nc, 1)), 0.0, ctx->stream));
+  if (mixed) {
+    HIPC(fill_launch(bf.dmax, 1, 0.0, ctx->stream));
+    HIPC(dmax_launch(dD, n2, reinterpret_cast<unsigned long long*>(bf.dmax), ctx->stream));
+  }
+  HIPC(fill_launch(bf.jp, std::max<size_t>(ctx->jp_size, 1), 0.0, ctx->stream));
   // the pairs' density blocks in the Jp layout, for the LDS kernels' Coulomb sums
-  HIPC(ctx->dp_d.ensure(sizeof(double) * std::max<size_t>(ctx->jp_size, 1)));
   HIPC(dpair_gather_launch(ctx->pairs_d.as<PairRec>(), dD, ctx->jp_cls_d.as<JpClass>(),
                            ctx->jp_ncomp_d.as<int>(), kNumPairClasses,
-                           static_cast<int>(ctx->pairs.size()), ctx->dp_d.as<double>(), ctx->nbf,
+                           static_cast<int>(ctx->pairs.size()), bf.dp, ctx->nbf,
                            ctx->stream));
   for (const auto& ps : ctx->passes) {
     PlanArgs pa{};
@@ -1221,13 +1283,13 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
     pa.pairs = ctx->pairs_d.as<PairRec>();
     pa.tau = o->schwarz_thresh;
     pa.mixed_thresh = mixed ? o->mixed_thresh : 0.0;
-    pa.dmax = ctx->plan_dmax.as<double>();
+    pa.dmax = bf.dmax;
     pa.cnt64 = ctx->plan_cnt64.as<int>();
     pa.cnt32 = ctx->plan_cnt32.as<int>();
     pa.lo = ctx->plan_lo.as<int>();
     pa.mid = ctx->plan_mid.as<int>();
     pa.hi = ctx->plan_hi.as<int>();
-    pa.stats = ctx->plan_stats.as<double>() + 4 * static_cast<size_t>(ps.seg_begin);
+    pa.stats = bf.stats + 4 * static_cast<size_t>(ps.seg_begin);
     if (!ctx->shell_mol.empty()) {
       pa.segs = ctx->plan_segs_d.as<int2>();
       pa.nmol = ctx->nmol;
@@ -1257,12 +1319,12 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
       const int cap = ctx->seg_cap[sg];
       EriArgs a = base_args(ctx);
       a.D = dD;
-      a.J = ctx->work_Jh.as<double>();
-      a.K = ctx->work_Kh.as<double>();
-      a.Jp = ctx->jp_d.as<double>();
+      a.J = bf.Jh;
+      a.K = bf.Kh;
+      a.Jp = bf.jp;
       a.jp_bra = ctx->jp_cls[qc.X];
       a.jp_ket = ctx->jp_cls[qc.Y];
-      a.Dp = ctx->dp_d.as<double>();
+      a.Dp = bf.dp;
       // K rows of the bra pair in LDS when they fit in 64 KB: every column, or
       // the bra's molecule's window in a multi-molecule basis
       const bool multi = !ctx->shell_mol.empty();
@@ -1280,7 +1342,7 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
           a.chunks = prec ? ctx->plan_chunks32.as<Chunk>() : ctx->plan_chunks64.as<Chunk>();
           a.chunk_range = ranges + (prec ? 6 : 4);
           a.tile_range = nullptr;
-          TRY(launch_timed(ctx, qc, prec, fn, a, std::min(kPersistentBlocks, cap)));
+          TRY(launch_timed(ctx, qc, prec, fn, a, std::min(kPersistentBlocks, cap), timed_ok));
         } else {
           TRY(class_kernel(ctx, qc.la, qc.lb, qc.lc, qc.ld, kModeJK, bm, prec, &fn));
           if (!fn.valid())
@@ -1288,20 +1350,63 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
                                                         class_name(qc.la, qc.lb, qc.lc, qc.ld) +
                                                         " not served by this library");
           a.tile_range = ranges + (prec ? 2 : 0);
-          TRY(launch_timed(ctx, qc, prec, fn, a, std::min(kPersistentBlocks, (cap + 3) / 4)));
+          TRY(launch_timed(ctx, qc, prec, fn, a, std::min(kPersistentBlocks, (cap + 3) / 4),
+                           timed_ok));
         }
       }
     }
   }
-  HIPC(jp_scatter_launch(ctx->pairs_d.as<PairRec>(), ctx->jp_d.as<double>(),
-                         ctx->jp_cls_d.as<JpClass>(), ctx->jp_ncomp_d.as<int>(), kNumPairClasses,
-                         static_cast<int>(ctx->pairs.size()), ctx->work_Jh.as<double>(), ctx->nbf,
-                         ctx->stream));
-  HIPC(symmetrize_launch(ctx->work_Jh.as<double>(), ctx->work_Kh.as<double>(), dJ, dK, ctx->nbf,
-                         ctx->stream));
+  HIPC(jp_scatter_launch(ctx->pairs_d.as<PairRec>(), bf.jp, ctx->jp_cls_d.as<JpClass>(),
+                         ctx->jp_ncomp_d.as<int>(), kNumPairClasses,
+                         static_cast<int>(ctx->pairs.size()), bf.Jh, ctx->nbf, ctx->stream));
+  HIPC(symmetrize_launch(bf.Jh, bf.Kh, dJ, dK, ctx->nbf, ctx->stream));
   ctx->last_stats = ctx->planned;
   ctx->last_mixed = mixed;
   return RCHEM_OK;
+}
+
+// Device memory of rchem_jk_device_noalloc: the accumulators of one Fock
+// build, in a file-scope array every GPU holds from module load to process end
+// (nothing allocated per call).  Host code never passes its name to a kernel:
+// in host code it stands for a host-side shadow, so the device address of each
+// GPU's copy is looked up once per device (hipGetSymbolAddress, with that
+// device current) when the first context on the device is created, and only
+// that address is used afterwards.  Calls on one GPU share the array.
+constexpr size_t kNoAllocScratchBytes = size_t(48) << 20;
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+struct NoAllocScratch {
+  std::mutex mu;
+  double* ptr[kNoAllocMaxDevices] = {};
+  hipError_t status[kNoAllocMaxDevices] = {};
+  bool tried[kNoAllocMaxDevices] = {};
+};
+NoAllocScratch& noalloc_scratch() {
+  static NoAllocScratch s;
+  return s;
+}
+
+// Resolves the current device's copy of `scratch` (once per device; the
+// caller has made `device` current).
+void noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kNoAllocMaxDevices) return;
+  NoAllocScratch& s = noalloc_scratch();
+  std::lock_guard<std::mutex> lock(s.mu);
+  if (s.tried[device]) return;
+  s.tried[device] = true;
+  void* p = nullptr;
+  s.status[device] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (s.status[device] == hipSuccess) s.ptr[device] = static_cast<double*>(p);
+}
+
+double* noalloc_scratch_ptr(int device, hipError_t* status) {
+  *status = hipErrorInvalidDevice;
+  if (device < 0 || device >= kNoAllocMaxDevices) return nullptr;
+  NoAllocScratch& s = noalloc_scratch();
+  std::lock_guard<std::mutex> lock(s.mu);
+  *status = s.tried[device] ? s.status[device] : hipErrorNotInitialized;
+  return s.ptr[device];
 }
 
 }  // namespace
@@ -1358,6 +1463,7 @@ int rchem_ctx_create(int device, rchem_ctx** out) {
     delete ctx;
     return RCHEM_ERR_HIP;
   }
+  noalloc_scratch_init(device);  // device is current (hipSetDevice above)
   std::vector<double> tab = make_boys_table();
   std::vector<float> tabf(tab.begin(), tab.end());
   if (ctx->boys_d.ensure(tab.size() * sizeof(double)) != hipSuccess ||
@@ -1522,6 +1628,39 @@ int rchem_jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK,
   return jk_device(ctx, dD, dJ, dK, &o);
 }
 
+int rchem_jk_device_noalloc(rchem_ctx* ctx, const double* dD, double* dJ, double* dK,
+                            const rchem_opts* opts) {
+  if (!ctx) return RCHEM_ERR_INVALID;
+  if (!ctx->have_basis) return ctx->fail(RCHEM_ERR_NO_BASIS, "no basis uploaded");
+  if (!dD || !dJ || !dK) return ctx->fail(RCHEM_ERR_INVALID, "null device pointer");
+  rchem_opts o;
+  rchem_opts_default(&o);
+  if (opts) o = *opts;
+  TRY(check_opts(ctx, &o));
+  // the plan (pair lists, tile lists, planner tables) belongs to the context
+  // and is built by the first rchem_jk / rchem_jk_device call at a threshold;
+  // building it allocates, so this entry point only uses an existing one
+  if (ctx->plan_tau != o.schwarz_thresh || ctx->pairs_tau != o.schwarz_thresh)
+    return ctx->fail(RCHEM_ERR_INVALID,
+                     "rchem_jk_device_noalloc: no plan for this Schwarz threshold on the "
+                     "context (call rchem_jk_device once with the same options first)");
+  const size_t need = jk_buffer_doubles(ctx) * sizeof(double);
+  if (need > kNoAllocScratchBytes)
+    return ctx->fail(RCHEM_ERR_INVALID,
+                     "rchem_jk_device_noalloc: this basis needs " + std::to_string(need >> 20) +
+                         " MiB of build accumulators, beyond the fixed " +
+                         std::to_string(kNoAllocScratchBytes >> 20) + " MiB");
+  hipError_t st = hipSuccess;
+  double* base = noalloc_scratch_ptr(ctx->device, &st);
+  if (!base)
+    return ctx->fail(RCHEM_ERR_HIP, std::string("rchem_jk_device_noalloc: device scratch "
+                                                "address unavailable: ") +
+                                        hipGetErrorString(st));
+  TRY(set_device(ctx));
+  const JkBuffers bf = jk_buffers_in(ctx, base);
+  return jk_device(ctx, dD, dJ, dK, &o, &bf);
+}
+
 int rchem_jk(rchem_ctx* ctx, const double* D, double* J, double* K, const rchem_opts* opts) {
   if (!ctx) return RCHEM_ERR_INVALID;
   if (!ctx->have_basis) return ctx->fail(RCHEM_ERR_NO_BASIS, "no basis uploaded");
@@ -1552,6 +1691,11 @@ int rchem_set_profiling(rchem_ctx* ctx, int enable) {
   return RCHEM_OK;
 }
 
+// where the last build's per-class planner statistics went
+static const double* last_stats_src(const rchem_ctx* ctx) {
+  return ctx->last_stats_dev ? ctx->last_stats_dev : ctx->plan_stats.as<double>();
+}
+
 int rchem_reset_class_timings(rchem_ctx* ctx) {
   if (!ctx) return RCHEM_ERR_INVALID;
   TRY(set_device(ctx));
@@ -1567,7 +1711,7 @@ int rchem_jk_class_timings(rchem_ctx* ctx, rchem_class_timing* out, int max, int
   if (ctx->last_mixed && !ctx->pclasses.empty()) {
     // work per precision of the last build, counted by the planner on the device
     std::vector<double> dv(4 * ctx->pclasses.size());
-    HIPC(hipMemcpyAsync(dv.data(), ctx->plan_stats.p, sizeof(double) * dv.size(),
+    HIPC(hipMemcpyAsync(dv.data(), last_stats_src(ctx), sizeof(double) * dv.size(),
                         hipMemcpyDeviceToHost, ctx->stream));
     HIPC(hipStreamSynchronize(ctx->stream));
     for (auto& t : ctx->timers) {
@@ -1612,7 +1756,7 @@ int rchem_jk_last_stats(rchem_ctx* ctx, rchem_jk_stats* st) {
     // the planner counted FP64 and FP32 quartets of the last build on the device
     TRY(set_device(ctx));
     std::vector<double> dv(4 * ctx->pclasses.size());
-    HIPC(hipMemcpyAsync(dv.data(), ctx->plan_stats.p, sizeof(double) * dv.size(),
+    HIPC(hipMemcpyAsync(dv.data(), last_stats_src(ctx), sizeof(double) * dv.size(),
                         hipMemcpyDeviceToHost, ctx->stream));
     HIPC(hipStreamSynchronize(ctx->stream));
     double q32 = 0.0;
