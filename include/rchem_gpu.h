@@ -186,6 +186,26 @@ int rchem_reset_class_timings(rchem_ctx *ctx);
 /* Writes up to max entries; *count receives the number of classes timed. */
 int rchem_jk_class_timings(rchem_ctx *ctx, rchem_class_timing *out, int max, int *count);
 
+/* Launch tuning of the Fock build (no rchem counterpart; rchem has no GPU).
+ * Each (la lb|lc ld) class and precision (fp32 = 1: the FP32 kernels of a
+ * mixed build) has a launch shape of its LDS-digestion kernel (an index into
+ * the library's compiled shapes, rchem_lds_variant_info), a persistent
+ * workgroup count and a chunk size in 64-lane tiles.  A context starts from the
+ * table data/tuning_gfx950.txt beside the library (RCHEM_TUNING_FILE names
+ * another; empty: none), written by rchem_amd/autotune.py; classes without an
+ * entry use shape 0, 2048 workgroups, 64 tiles.  Lines of the table:
+ * "la lb lc ld fp32 variant blocks chunk_tiles", '#' comments.  A shape a class
+ * is not compiled in falls back to shape 0.  Setting an entry re-plans the
+ * next build (which allocates). */
+int rchem_lds_variant_info(int variant, int *waves, int *min_waves_per_simd, int *prefetch);
+int rchem_set_class_tuning(rchem_ctx *ctx, int la, int lb, int lc, int ld, int fp32, int variant,
+                           int blocks, int chunk_tiles);
+int rchem_get_class_tuning(const rchem_ctx *ctx, int la, int lb, int lc, int ld, int fp32,
+                           int *variant, int *blocks, int *chunk_tiles);
+/* Replaces the context's table by the file at path (NULL or "": no table,
+ * every class at its defaults); *entries (may be NULL) receives the count. */
+int rchem_load_tuning(rchem_ctx *ctx, const char *path, int *entries);
+
 /* ---- SCF ------------------------------------------------------------------ */
 
 typedef struct rchem_scf_opts {

@@ -87,6 +87,11 @@ SIGNATURES = {
     "rchem_jk_class_timings": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ClassTiming),
                                               ctypes.c_int, _ip]),
     "rchem_scf_opts_default": (ctypes.c_int, [ctypes.POINTER(SCFOpts)]),
+    "rchem_lds_variant_info": (ctypes.c_int, [ctypes.c_int, _ip, _ip, _ip]),
+    "rchem_set_class_tuning": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 8),
+    "rchem_get_class_tuning": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 5 +
+                               [_ip, _ip, _ip]),
+    "rchem_load_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _ip]),
     # cints.h compatibility exports (include/rchem_cints_compat.h)
     "coulomb_repulsion": (ctypes.c_double, ([ctypes.c_double] * 4 + [ctypes.c_int] * 3 +
                                             [ctypes.c_double]) * 4),

@@ -1,0 +1,194 @@
+"""Autotuner of the Fock build's launch choices, per (la lb|lc ld) class and
+precision, on the device it runs on.
+
+For every class kernel the build launches it measures, with the library's own
+per-class HIP-event timings:
+  1. every compiled launch shape of the LDS-digestion kernel (waves per
+     workgroup, register bound for occupancy, one-ahead ket fetch;
+     rchem_lds_variant_info), all classes switched together, one timed set of
+     builds per shape;
+  2. persistent workgroup counts (512 ... 8192) at each class's best shape;
+  3. chunk sizes (16 ... 256 tiles) at the best shape and count.
+A choice is kept for a class only where it beats the class's best so far by
+more than the measured noise (1 %).  The FP64 kernels are timed in the
+all-FP64 build and in the FP64 part of the mixed build and tuned on the sum;
+the FP32 kernels in the mixed build.  The table is written in the format
+rchem_load_tuning reads (data/tuning_gfx950.txt beside the library is the
+default every context loads), and a JSON report of default vs tuned ms per
+class goes to stdout.
+
+    python -m rchem_amd.autotune [--workload h2o64|h2o32|eri300] [--out PATH]
+
+The workload is a Fock build on the density the bench uses (core-Hamiltonian
+guess) so that the mixed-precision split is the bench's.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_OUT = os.path.join(HERE, "data", "tuning_gfx950.txt")
+BLOCKS = [512, 1024, 2048, 4096, 8192]
+CHUNKS = [16, 32, 64, 128, 256]
+NOISE = 0.01
+
+
+def _log(msg):
+    print(f"[autotune] {msg}", file=sys.stderr, flush=True)
+
+
+def core_guess_density(S, H, nocc):
+    lam, U = np.linalg.eigh(S)
+    X = U @ np.diag(lam ** -0.5) @ U.T
+    _, Cp = np.linalg.eigh(X.T @ H @ X)
+    C = X @ Cp
+    return C[:, :nocc] @ C[:, :nocc].T
+
+
+class Tuner:
+    def __init__(self, ctx, basis, D, steps):
+        from ._lib import PREC_FP64, PREC_MIXED, default_opts, lib
+        from .device import DeviceMatrix
+        self.L = lib()
+        self.ctx = ctx
+        self.B = basis
+        n = basis.nbf
+        self.dD, self.dJ, self.dK = DeviceMatrix(n), DeviceMatrix(n), DeviceMatrix(n)
+        self.dD.upload(D)
+        self.steps = steps
+        self.opts = {"fp64": default_opts(precision=PREC_FP64, schwarz_thresh=1e-12),
+                     "mixed": default_opts(precision=PREC_MIXED, schwarz_thresh=1e-12)}
+        nv = 0
+        while self.L.rchem_lds_variant_info(nv, None, None, None) == 0:
+            nv += 1
+        self.nvariants = nv
+
+    def build(self, mode):
+        self.ctx.check(self.L.rchem_jk_device(self.ctx.h, self.dD.ptr, self.dJ.ptr, self.dK.ptr,
+                                              ctypes.byref(self.opts[mode])))
+
+    def time_mode(self, mode):
+        """Per-class ms per build {(la, lb, lc, ld, fp32): ms} in one precision mode."""
+        from ._lib import ClassTiming
+        from .device import synchronize
+        self.build(mode)  # re-plans after a tuning change, warms up
+        synchronize()
+        L, h = self.L, self.ctx.h
+        self.ctx.check(L.rchem_reset_class_timings(h))
+        self.ctx.check(L.rchem_set_profiling(h, 1))
+        for _ in range(self.steps):
+            self.build(mode)
+        synchronize()
+        self.ctx.check(L.rchem_set_profiling(h, 0))
+        t = (ClassTiming * 128)()
+        cnt = ctypes.c_int()
+        self.ctx.check(L.rchem_jk_class_timings(h, t, 128, ctypes.byref(cnt)))
+        return {(x.la, x.lb, x.lc, x.ld, x.fp32): x.ms / self.steps for x in t[:cnt.value]}
+
+    def time_all(self):
+        """Per (class, fp32) ms summed over the FP64 and the mixed build."""
+        out = {}
+        for mode in ("fp64", "mixed"):
+            for k, v in self.time_mode(mode).items():
+                out[k] = out.get(k, 0.0) + v
+        return out
+
+    def set(self, key, variant, blocks, chunk):
+        la, lb, lc, ld, fp32 = key
+        self.ctx.check(self.L.rchem_set_class_tuning(self.ctx.h, la, lb, lc, ld, fp32, variant,
+                                                     blocks, chunk))
+
+
+def tune(tuner):
+    from ._lib import lib
+    L = lib()
+    L.rchem_load_tuning(tuner.ctx.h, None, None)  # start from the defaults
+    base = tuner.time_all()
+    keys = sorted(base)
+    _log(f"defaults: {len(keys)} class kernels, {sum(base.values()):.2f} ms "
+         f"(FP64 + mixed build)")
+    best = {k: {"variant": 0, "blocks": 2048, "chunk": 64, "ms": base[k]} for k in keys}
+
+    def sweep(name, values, apply):
+        for val in values:
+            for k in keys:
+                v, b, c = apply(best[k], val)
+                tuner.set(k, v, b, c)
+            t = tuner.time_all()
+            for k in keys:
+                if k in t and t[k] < best[k]["ms"] * (1.0 - NOISE):
+                    v, b, c = apply(best[k], val)
+                    best[k] = {"variant": v, "blocks": b, "chunk": c, "ms": t[k]}
+            _log(f"{name} {val}: {sum(t.get(k, 0.0) for k in keys):.2f} ms")
+        for k in keys:
+            tuner.set(k, best[k]["variant"], best[k]["blocks"], best[k]["chunk"])
+
+    sweep("shape", range(1, tuner.nvariants), lambda cur, v: (v, cur["blocks"], cur["chunk"]))
+    sweep("workgroups", BLOCKS, lambda cur, b: (cur["variant"], b, cur["chunk"]))
+    sweep("chunk", CHUNKS, lambda cur, c: (cur["variant"], cur["blocks"], c))
+    final = tuner.time_all()
+    return base, best, final
+
+
+def write_table(path, best, device, workload):
+    lines = ["# Fock-build launch tuning, written by rchem_amd/autotune.py",
+             f"# device: {device}; workload: {workload}",
+             "# la lb lc ld fp32 variant blocks chunk_tiles   (ms per FP64 + mixed build)"]
+    for k in sorted(best):
+        b = best[k]
+        lines.append(f"{k[0]} {k[1]} {k[2]} {k[3]} {k[4]} {b['variant']} {b['blocks']} "
+                     f"{b['chunk']}   # {b['ms']:.3f}")
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--workload", choices=["h2o64", "h2o32", "eri300"], default="h2o64")
+    ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--steps", type=int, default=3)
+    args = ap.parse_args(argv)
+    from . import basis as rb
+    from .inputs import nocc_of, synthetic_shells, water_cluster
+
+    t0 = time.perf_counter()
+    ctx = rb.Context(0)
+    if args.workload in ("h2o64", "h2o32"):
+        dims = (4, 4, 4) if args.workload == "h2o64" else (2, 4, 4)
+        Z, xyz = water_cluster(dims=dims, seed=64 if args.workload == "h2o64" else 32)
+        B = rb.Basis.new(Z, xyz, "6-31G", ctx=ctx)
+        S, T, V = rb.one_electron(B, xyz, Z)
+        D = core_guess_density(S, T + V, nocc_of(Z))
+        workload = f"{args.workload} 6-31G Fock build, core-guess density, Schwarz 1e-12"
+    else:
+        sh = synthetic_shells(nshell=300, seed=2000, box=20.0 * (300 / 2000.0) ** (1 / 3), nprim=3)
+        B = rb.Basis.from_shells(*sh, ctx=ctx)
+        X = np.random.default_rng(2000).standard_normal((B.nbf, B.nbf)) / B.nbf
+        D = X + X.T
+        workload = "300-shell s/p/d microbenchmark, Schwarz 1e-12"
+    tuner = Tuner(ctx, B, D, args.steps)
+    _log(f"{workload}: nbf {B.nbf}, {tuner.nvariants} launch shapes")
+    base, best, final = tune(tuner)
+    dev = os.environ.get("RCHEM_DEVICE_NAME", "gfx950")
+    write_table(args.out, best, dev, workload)
+    report = {"workload": workload, "table": os.path.relpath(args.out),
+              "seconds": time.perf_counter() - t0,
+              "default_ms": sum(base.values()), "tuned_ms": sum(final.values()),
+              "classes": [{"class": f"({k[0]}{k[1]}|{k[2]}{k[3]})" + (" fp32" if k[4] else ""),
+                           "default_ms": base[k], "tuned_ms": final.get(k),
+                           "variant": best[k]["variant"], "blocks": best[k]["blocks"],
+                           "chunk_tiles": best[k]["chunk"]} for k in sorted(best)]}
+    print(json.dumps(report), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

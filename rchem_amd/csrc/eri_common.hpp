@@ -70,7 +70,8 @@ struct BraPrim {
 // LDS-digestion kernel) and P - Q is formed as (P - A) - (Q - A), Q - A once per
 // ket primitive; otherwise they come from global memory.  `boys_tab` is the
 // Boys grid in LDS.
-template <int LA, int LB, int LC, int LD, int MODE, int BM, typename R, bool BRA_LDS = false>
+template <int LA, int LB, int LC, int LD, int MODE, int BM, typename R, bool BRA_LDS = false,
+          bool PF = true>
 __device__ __forceinline__ void eri_compute(const EriArgs& a, const Tile t, const int lane,
                                             TileInfo& ti,
                                             double (&out)[EriClass<LA, LB, LC, LD>::NOUT],
@@ -96,8 +97,10 @@ __device__ __forceinline__ void eri_compute(const EriArgs& a, const Tile t, cons
   const PrimRec* __restrict__ kprim = a.prims + kp.p0;
   PrimRec nxt = kprim[0];
   for (int j = 0; j < kp.np; ++j) {
-    const PrimRec pk = nxt;
-    if (j + 1 < kp.np) nxt = kprim[(j + 1) * kp.ps];
+    // with PF the next record is in flight during this iteration's work;
+    // without, the load is issued here and other waves cover its latency
+    const PrimRec pk = PF ? nxt : kprim[j * kp.ps];
+    if (PF && j + 1 < kp.np) nxt = kprim[(j + 1) * kp.ps];
     const R zk = R(pk.zeta), kk = R(pk.kab), oo2e = R(pk.oo2z);
     const R QCx = R(pk.px - kp.A[0]), QCy = R(pk.py - kp.A[1]), QCz = R(pk.pz - kp.A[2]);
     if constexpr (BRA_LDS) {
@@ -370,11 +373,11 @@ __global__ __launch_bounds__(256) void eri_kernel(const EriArgs a) {
 
 // LDS digestion: a fixed grid of workgroups drains chunks [range[0], range[1]).
 // Dynamic LDS holds the bra's K rows: (NA + NB) x kw doubles, columns
-// [kc0, kc0 + kw) (all nbf columns for one molecule).
-constexpr int kLdsWaves = 4;  // waves per LDS-digestion workgroup (they share the K rows)
-
-template <int LA, int LB, int LC, int LD, int BM, typename R>
-__global__ __launch_bounds__(64 * kLdsWaves) void eri_jk_lds_kernel(const EriArgs a) {
+// [kc0, kc0 + kw) (all nbf columns for one molecule).  W waves per workgroup
+// share them; OCC > 0 bounds the registers so that OCC waves fit a SIMD; PF
+// selects the one-ahead ket fetch (kLdsVariants lists the compiled shapes).
+template <int LA, int LB, int LC, int LD, int BM, typename R, int W, int OCC, bool PF>
+__global__ __launch_bounds__(64 * W, OCC > 0 ? OCC : 1) void eri_jk_lds_kernel(const EriArgs a) {
   using C = EriClass<LA, LB, LC, LD>;
   constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
   extern __shared__ double krows[];
@@ -403,11 +406,11 @@ __global__ __launch_bounds__(64 * kLdsWaves) void eri_jk_lds_kernel(const EriArg
     double jab_acc[NA * NB];
 #pragma unroll
     for (int k = 0; k < NA * NB; ++k) jab_acc[k] = 0.0;
-    for (int t = c.tb + wave; t < c.te; t += kLdsWaves) {  // wave-uniform
+    for (int t = c.tb + wave; t < c.te; t += W) {  // wave-uniform
       TileInfo ti;
       double out[C::NOUT];
-      eri_compute<LA, LB, LC, LD, kModeJK, BM, R, true>(a, a.tiles[t], lane, ti, out, boys_tab,
-                                                        sbra);
+      eri_compute<LA, LB, LC, LD, kModeJK, BM, R, true, PF>(a, a.tiles[t], lane, ti, out,
+                                                            boys_tab, sbra);
       const int fa = ti.bp.fa, fb = ti.bp.fb, fc = ti.kp.fa, fd = ti.kp.fb;
       const double s = symmetry_weight(ti);
       double jab[NA * NB], jcd[NC * ND];
@@ -465,14 +468,25 @@ hipError_t eri_launch(const EriArgs& a, int blocks, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int LA, int LB, int LC, int LD, int BM, typename R>
+template <int LA, int LB, int LC, int LD, int BM, typename R, int W = 4, int OCC = 0,
+          bool PF = true>
 hipError_t eri_jk_lds_launch(const EriArgs& a, int blocks, hipStream_t s) {
   if (blocks <= 0) return hipSuccess;
   const size_t lds = sizeof(double) * (ncart(LA) + ncart(LB)) * static_cast<size_t>(a.kw);
-  hipLaunchKernelGGL((eri_jk_lds_kernel<LA, LB, LC, LD, BM, R>), dim3(blocks),
-                     dim3(64 * kLdsWaves), lds, s,
-                     a);
+  hipLaunchKernelGGL((eri_jk_lds_kernel<LA, LB, LC, LD, BM, R, W, OCC, PF>), dim3(blocks),
+                     dim3(64 * W), lds, s, a);
   return hipGetLastError();
+}
+
+// The LDS-digestion launcher of shape kLdsVariants[V] (exact Boys; FP64 or FP32).
+template <int LA, int LB, int LC, int LD, int V>
+EriLaunchFn eri_jk_lds_variant(int bm, int fp32) {
+  constexpr LdsVariant v = kLdsVariants[V];
+  if (bm != kBoysExact) return nullptr;
+  if (fp32) return eri_jk_lds_launch<LA, LB, LC, LD, kBoysExact, float, v.waves,
+                                     v.min_waves_per_simd, v.prefetch != 0>;
+  return eri_jk_lds_launch<LA, LB, LC, LD, kBoysExact, double, v.waves, v.min_waves_per_simd,
+                           v.prefetch != 0>;
 }
 
 }  // namespace rchem

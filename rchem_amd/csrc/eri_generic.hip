@@ -381,7 +381,7 @@ size_t eri_generic_lds_bytes(const GenericProgram& p) {
 // kernels would use (this kernel launches 4 one-wave workgroups per block);
 // without one, one workgroup per tile.
 hipError_t eri_generic_launch(const EriArgs& a, const GenericProgram& p, int mode, int bm,
-                              int blocks, hipStream_t s) {
+                              int blocks, hipStream_t s, size_t lds_limit) {
   unsigned grid;
   if (a.tile_range) {
     if (blocks <= 0) return hipSuccess;
@@ -392,7 +392,7 @@ hipError_t eri_generic_launch(const EriArgs& a, const GenericProgram& p, int mod
   }
   if (p.P < 1 || p.P > 64 || (p.P & (p.P - 1)) != 0 || p.L > kBoysMaxM) return hipErrorInvalidValue;
   const size_t lds = eri_generic_lds_bytes(p);
-  if (lds > 64 * 1024) return hipErrorInvalidValue;
+  if (lds > lds_limit) return hipErrorInvalidValue;
   const bool compat = bm == kBoysCompat;
   switch (mode) {
     case kModeJK:

@@ -30,6 +30,7 @@ import math
 import sys
 
 AX = "xyz"
+N_LDS_VARIANTS = 8  # entries of kLdsVariants (types.hpp)
 
 
 def cart(l):
@@ -266,17 +267,24 @@ def tu_source(la, lb, lc, ld):
     L += [f"  if (mode == kModeTensor) return eri_launch<{A}, kModeTensor, kBoysCompat, double>;",
           f"  if (mode == kModeQuartet) return eri_launch<{A}, kModeQuartet, kBoysCompat, double>;",
           "  return nullptr;", "}", "",
-          f"EriLaunchFn eri_jk_lds_launcher_c{key}(int bm, int fp32) {{"]
+          "// LDS-digestion kernel of launch shape kLdsVariants[v]; nullptr when the class",
+          "// is not compiled in that shape (the caller falls back to shape 0).",
+          f"EriLaunchFn eri_jk_lds_launcher_c{key}(int bm, int fp32, int v) {{"]
     if sp:
-        L += [f"  if (fp32) return bm == kBoysExact ? eri_jk_lds_launch<{A}, kBoysExact, float> : nullptr;",
-              f"  return bm == kBoysExact ? eri_jk_lds_launch<{A}, kBoysExact, double>",
-              f"                          : eri_jk_lds_launch<{A}, kBoysCompat, double>;"]
+        # every shape for the s/p classes (the tuning table chooses), and the
+        # compat-Boys parity kernel in shape 0
+        L += ["  if (v == 0 && !fp32 && bm == kBoysCompat)",
+              f"    return eri_jk_lds_launch<{A}, kBoysCompat, double>;",
+              "  switch (v) {"]
+        L += [f"    case {v}: return eri_jk_lds_variant<{A}, {v}>(bm, fp32);"
+              for v in range(N_LDS_VARIANTS)]
+        L += ["    default: return nullptr;", "  }"]
     elif la + lb + lc + ld <= 4:
-        # d classes whose state fits a lane: LDS digestion for the production Boys
-        L += [f"  if (fp32) return bm == kBoysExact ? eri_jk_lds_launch<{A}, kBoysExact, float> : nullptr;",
-              f"  return bm == kBoysExact ? eri_jk_lds_launch<{A}, kBoysExact, double> : nullptr;"]
+        # d classes whose state fits a lane: LDS digestion for the production
+        # Boys, default shape only (compile time)
+        L += [f"  return v == 0 ? eri_jk_lds_variant<{A}, 0>(bm, fp32) : nullptr;"]
     else:
-        L += ["  (void)bm;", "  (void)fp32;", "  return nullptr;"]
+        L += ["  (void)bm;", "  (void)fp32;", "  (void)v;", "  return nullptr;"]
     L += ["}", "", "}  // namespace rchem", ""]
     return "\n".join(L)
 
@@ -377,19 +385,21 @@ def main():
     keys = [f"{a}{b}{c}{d}" for a, b, c, d in classes(lmax)]
     if only:
         keys = [k for k in keys if k in only]
-    disp = ["// Generated by gen_eri.py -- do not edit."]
+    disp = ["// Generated by gen_eri.py -- do not edit.",
+            f"static_assert(kNumLdsVariants == {N_LDS_VARIANTS}, \"gen_eri.py N_LDS_VARIANTS\");"]
     disp += [f"EriLaunchFn eri_launcher_c{k}(int mode, int bm, int fp32);" for k in keys]
-    disp += [f"EriLaunchFn eri_jk_lds_launcher_c{k}(int bm, int fp32);" for k in keys]
+    disp += [f"EriLaunchFn eri_jk_lds_launcher_c{k}(int bm, int fp32, int v);" for k in keys]
     disp += ["", "// nullptr when the class or the variant is not compiled in.",
              "EriLaunchFn eri_launcher(int la, int lb, int lc, int ld, int mode, int bm, int fp32) {"]
     for k in keys:
         disp.append(f"  if (la == {k[0]} && lb == {k[1]} && lc == {k[2]} && ld == {k[3]}) "
                     f"return eri_launcher_c{k}(mode, bm, fp32);")
     disp += ["  return nullptr;", "}", "",
-             "EriLaunchFn eri_jk_lds_launcher(int la, int lb, int lc, int ld, int bm, int fp32) {"]
+             "EriLaunchFn eri_jk_lds_launcher(int la, int lb, int lc, int ld, int bm, int fp32,",
+             "                                int v) {"]
     for k in keys:
         disp.append(f"  if (la == {k[0]} && lb == {k[1]} && lc == {k[2]} && ld == {k[3]}) "
-                    f"return eri_jk_lds_launcher_c{k}(bm, fp32);")
+                    f"return eri_jk_lds_launcher_c{k}(bm, fp32, v);")
     disp += ["  return nullptr;", "}", ""]
     with open(os.path.join(out_dir, "eri_dispatch.inc"), "w") as fh:
         fh.write("\n".join(disp))

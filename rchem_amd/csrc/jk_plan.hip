@@ -12,7 +12,8 @@
 //   3. scan:   exclusive prefix sums of the tile counts (hipCUB);
 //   4. expand: each task writes its tiles;
 //   5. chunks: the tiles of one (class, bra pair) unit are cut into chunks of
-//              at most chunk_tiles tiles (a workgroup's unit of work when K is
+//              at most the class's chunk size per precision (from the tuning
+//              table; a workgroup's unit of work when K is
 //              digested in LDS), counted, scanned and written; per class the
 //              [begin, end) tile and chunk ranges of each precision go to device
 //              memory, where the persistent ERI kernels read them.
@@ -171,9 +172,9 @@ __global__ void chunk_count_kernel(const PlanArgs a, const int* __restrict__ off
   }
   int c, bra, t0, t1;
   unit_tasks(a, u, &c, &bra, &t0, &t1);
-  const int ch = a.chunk_tiles;
-  chc64[u] = (off64[t1] - off64[t0] + ch - 1) / ch;
-  chc32[u] = (off32[t1] - off32[t0] + ch - 1) / ch;
+  const int ch64 = a.classes[c].chunk64, ch32 = a.classes[c].chunk32;
+  chc64[u] = (off64[t1] - off64[t0] + ch64 - 1) / ch64;
+  chc32[u] = (off32[t1] - off32[t0] + ch32 - 1) / ch32;
 }
 
 __global__ void chunk_expand_kernel(const PlanArgs a, const int* __restrict__ off64,
@@ -184,13 +185,14 @@ __global__ void chunk_expand_kernel(const PlanArgs a, const int* __restrict__ of
   if (u >= a.nunits) return;
   int c, bra, t0, t1;
   unit_tasks(a, u, &c, &bra, &t0, &t1);
-  const int ch = a.chunk_tiles;
   {
+    const int ch = a.classes[c].chunk64;
     const int tb = min(off64[t0], cap), te = min(off64[t1], cap);
     int o = choff64[u];
     for (int k = tb; k < te && o < cap; k += ch, ++o) chunks64[o] = Chunk{bra, k, min(k + ch, te), 0};
   }
   {
+    const int ch = a.classes[c].chunk32;
     const int tb = min(off32[t0], cap), te = min(off32[t1], cap);
     int o = choff32[u];
     for (int k = tb; k < te && o < cap; k += ch, ++o) chunks32[o] = Chunk{bra, k, min(k + ch, te), 0};

@@ -4,6 +4,7 @@
 // shell form, the shell-pair and tile lists that drive the ERI kernels, and
 // the SCF driver.  All arithmetic on integrals and matrices runs on the GPU;
 // the host sorts pair lists and drives launches.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -33,9 +34,10 @@ EriLaunchFn eri_launcher(int la, int lb, int lc, int ld, int mode, int bm, int f
 hipError_t dmax_launch(const double* D, size_t n2, unsigned long long* out, hipStream_t s);
 size_t plan_scan_bytes(int ntasks);
 hipError_t plan_launch(const PlanArgs& a, const PlanBuffers& b, hipStream_t s);
-EriLaunchFn eri_jk_lds_launcher(int la, int lb, int lc, int ld, int bm, int fp32);
+EriLaunchFn eri_jk_lds_launcher(int la, int lb, int lc, int ld, int bm, int fp32, int v);
 hipError_t eri_generic_launch(const EriArgs& a, const GenericProgram& p, int mode, int bm,
-                              int blocks, hipStream_t s);
+                              int blocks, hipStream_t s,
+                              size_t lds_limit);
 
 size_t eri_generic_lds_bytes(const GenericProgram& p);
 
@@ -214,6 +216,18 @@ struct ClassTimer {
   int launches = 0;
 };
 
+// Launch choices of one (la lb|lc ld) class in one precision of the Fock
+// build: LDS-digestion kernel shape (kLdsVariants), persistent workgroups, and
+// tiles per chunk.  Defaults: shape 0, 2048 workgroups (256 CUs x 8), 64 tiles.
+constexpr int kDefaultBlocks = 2048;
+constexpr int kDefaultChunkTiles = 64;
+struct ClassTune {
+  int la = 0, lb = 0, lc = 0, ld = 0, fp32 = 0;
+  int variant = 0;
+  int blocks = kDefaultBlocks;
+  int chunk_tiles = kDefaultChunkTiles;
+};
+
 // Host-side record of a shell pair before it is laid out.
 struct PairHost {
   int a, b;     // shell indices, l(a) >= l(b)
@@ -311,6 +325,12 @@ struct rchem_ctx {
   bool last_mixed = false;
   bool last_mixed_pending = false;  // the build being launched is mixed
 
+  // LDS per workgroup of the device (hipDeviceProp_t::sharedMemPerBlock)
+  size_t lds_block = 65536;
+  // per (class, precision) launch choices of the Fock build, from the tuning
+  // table (defaults where it has no entry)
+  std::vector<ClassTune> tuning;
+
   bool profiling = false;
   bool lds_digestion = true;  // RCHEM_LDS_DIGESTION=0 selects global-atomic digestion
   bool force_generic = false;  // RCHEM_ERI_GENERIC=1: every class on the wave-cooperative kernel
@@ -387,6 +407,67 @@ std::vector<double> make_boys_table() {
     tab[kBoysExpOffset + r] = static_cast<double>(expl(-T));
   }
   return tab;
+}
+
+// The tuning entry of class (la lb|lc ld) in one precision, or the defaults.
+ClassTune tune_for(const rchem_ctx* ctx, int la, int lb, int lc, int ld, int fp32) {
+  for (const ClassTune& t : ctx->tuning)
+    if (t.la == la && t.lb == lb && t.lc == lc && t.ld == ld && t.fp32 == fp32) return t;
+  ClassTune d;
+  d.la = la; d.lb = lb; d.lc = lc; d.ld = ld; d.fp32 = fp32;
+  return d;
+}
+
+bool tune_valid(int variant, int blocks, int chunk_tiles) {
+  return variant >= 0 && variant < kNumLdsVariants && blocks >= 1 && blocks <= 65536 &&
+         chunk_tiles >= 1 && chunk_tiles <= 4096;
+}
+
+void set_tune(rchem_ctx* ctx, const ClassTune& t) {
+  for (ClassTune& x : ctx->tuning)
+    if (x.la == t.la && x.lb == t.lb && x.lc == t.lc && x.ld == t.ld && x.fp32 == t.fp32) {
+      x = t;
+      return;
+    }
+  ctx->tuning.push_back(t);
+}
+
+// Reads a tuning table: one line per (class, precision),
+//   la lb lc ld fp32 variant blocks chunk_tiles
+// with '#' starting a comment; lines that do not parse or are out of range are
+// skipped.  Returns the number of entries taken.
+int load_tuning(rchem_ctx* ctx, const std::string& path) {
+  FILE* f = std::fopen(path.c_str(), "r");
+  if (!f) return 0;
+  int taken = 0;
+  char line[512];
+  while (std::fgets(line, sizeof(line), f)) {
+    if (char* h = std::strchr(line, '#')) *h = '\0';
+    ClassTune t;
+    if (std::sscanf(line, "%d %d %d %d %d %d %d %d", &t.la, &t.lb, &t.lc, &t.ld, &t.fp32,
+                    &t.variant, &t.blocks, &t.chunk_tiles) != 8)
+      continue;
+    if (t.la < 0 || t.la > kMaxL || t.lb < 0 || t.lb > kMaxL || t.lc < 0 || t.lc > kMaxL ||
+        t.ld < 0 || t.ld > kMaxL || (t.fp32 != 0 && t.fp32 != 1) ||
+        !tune_valid(t.variant, t.blocks, t.chunk_tiles))
+      continue;
+    set_tune(ctx, t);
+    ++taken;
+  }
+  std::fclose(f);
+  return taken;
+}
+
+// Where the library's own tuning table lies: data/tuning_gfx950.txt beside
+// the shared library (RCHEM_TUNING_FILE overrides; empty: no table).
+std::string tuning_path() {
+  if (const char* e = std::getenv("RCHEM_TUNING_FILE")) return e;
+  Dl_info info{};
+  if (dladdr(reinterpret_cast<void*>(&tuning_path), &info) == 0 || !info.dli_fname) return "";
+  std::string so = info.dli_fname;
+  const size_t slash = so.find_last_of('/');
+  const std::string dir = slash == std::string::npos ? "." : so.substr(0, slash);
+  return dir + "/data/tuning_gfx950.txt";
 }
 
 int set_device(rchem_ctx* ctx) {
@@ -602,9 +683,10 @@ struct ClassKernel {
   EriLaunchFn fn = nullptr;
   const GenericProgram* gp = nullptr;
   int mode = 0, bm = 0;
+  size_t lds_limit = 65536;  // the device's LDS per workgroup (wave-cooperative kernel)
   bool valid() const { return fn || gp; }
   hipError_t operator()(const EriArgs& a, int blocks, hipStream_t s) const {
-    return fn ? fn(a, blocks, s) : eri_generic_launch(a, *gp, mode, bm, blocks, s);
+    return fn ? fn(a, blocks, s) : eri_generic_launch(a, *gp, mode, bm, blocks, s, lds_limit);
   }
 };
 
@@ -659,7 +741,7 @@ int find_program(rchem_ctx* ctx, int la, int lb, int lc, int ld, int grad,
   for (int i = 0; i < kNumHostPrograms; ++i) {
     const HostProgram& h = kHostPrograms[i];
     if (h.la == la && h.lb == lb && h.lc == lc && h.ld == ld && h.grad == grad) {
-      if (eri_generic_lds_bytes(ctx->gprog[i]) <= 64 * 1024) *out = &ctx->gprog[i];
+      if (eri_generic_lds_bytes(ctx->gprog[i]) <= ctx->lds_block) *out = &ctx->gprog[i];
       return RCHEM_OK;
     }
   }
@@ -673,6 +755,7 @@ int class_kernel(rchem_ctx* ctx, int la, int lb, int lc, int ld, int mode, int b
   *out = ClassKernel{};
   out->mode = mode;
   out->bm = bm;
+  out->lds_limit = ctx->lds_block;
   if (!ctx->force_generic) {
     out->fn = eri_launcher(la, lb, lc, ld, mode, bm, fp32);
     if (out->fn) return RCHEM_OK;
@@ -988,6 +1071,8 @@ int prepare_plan(rchem_ctx* ctx, double tau) {
     pc.task_begin = cur.tasks;
     pc.same = qc.X == qc.Y;
     pc.unit_begin = cur.units;
+    pc.chunk64 = tune_for(ctx, qc.la, qc.lb, qc.lc, qc.ld, 0).chunk_tiles;
+    pc.chunk32 = tune_for(ctx, qc.la, qc.lb, qc.lc, qc.ld, 1).chunk_tiles;
     ctx->pclasses.push_back(pc);
     ctx->seg_class.push_back(c);
     ctx->seg_cap.push_back(static_cast<int>(cap));
@@ -1165,8 +1250,9 @@ int run_eri_host(rchem_ctx* ctx, int mode, int bm, double* out) {
   return RCHEM_OK;
 }
 
-constexpr int kPersistentBlocks = 2048;  // 256 CUs x 8 workgroups of 4 waves
-constexpr int kChunkTiles = 64;          // tiles per LDS-digestion chunk
+// LDS an LDS-digestion workgroup needs beside the K rows (Boys block, bra
+// primitive pairs; 11 KB in FP64)
+constexpr size_t kLdsStaticReserve = 12 * 1024;
 
 int launch_timed(rchem_ctx* ctx, const QClass& qc, int fp32, const ClassKernel& fn,
                  const EriArgs& a, int blocks, bool may_time = true) {
@@ -1277,7 +1363,6 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
     pa.nclasses = ps.nseg;
     pa.ntasks = ps.tasks;
     pa.nunits = ps.units;
-    pa.chunk_tiles = kChunkTiles;
     pa.buckets = ctx->plan_buckets_d.as<int4>();
     pa.Q = ctx->plan_q_d.as<double>();
     pa.pairs = ctx->pairs_d.as<PairRec>();
@@ -1330,19 +1415,27 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
       const bool multi = !ctx->shell_mol.empty();
       a.kw = multi ? ctx->mol_kw : ctx->nbf;
       a.pair_kc0 = multi ? ctx->pair_kc0_d.as<int>() : nullptr;
+      // K rows in LDS when they fit the device's LDS per workgroup beside the
+      // kernel's own arrays (Boys block, bra primitive pairs: < 12 KB)
       const size_t lds = sizeof(double) * (ncart(qc.la) + ncart(qc.lb)) * static_cast<size_t>(a.kw);
-      const bool lds_ok = lds <= 65536 && ctx->lds_digestion && !ctx->force_generic;
+      const bool lds_ok = lds + kLdsStaticReserve <= ctx->lds_block && ctx->lds_digestion &&
+                          !ctx->force_generic;
       int* ranges = pb.ranges + 8 * (sg - ps.seg_begin);
       for (int prec = 0; prec < (mixed ? 2 : 1); ++prec) {
         const int bm = prec ? kBoysExact : o->boys_mode;
         a.tiles = prec ? ctx->plan_tiles32.as<Tile>() : ctx->plan_tiles64.as<Tile>();
+        const ClassTune tn = tune_for(ctx, qc.la, qc.lb, qc.lc, qc.ld, prec);
         ClassKernel fn;
-        fn.fn = lds_ok ? eri_jk_lds_launcher(qc.la, qc.lb, qc.lc, qc.ld, bm, prec) : nullptr;
+        if (lds_ok) {
+          fn.fn = eri_jk_lds_launcher(qc.la, qc.lb, qc.lc, qc.ld, bm, prec, tn.variant);
+          if (!fn.fn && tn.variant != 0)  // shape not compiled for this class
+            fn.fn = eri_jk_lds_launcher(qc.la, qc.lb, qc.lc, qc.ld, bm, prec, 0);
+        }
         if (fn.valid()) {
           a.chunks = prec ? ctx->plan_chunks32.as<Chunk>() : ctx->plan_chunks64.as<Chunk>();
           a.chunk_range = ranges + (prec ? 6 : 4);
           a.tile_range = nullptr;
-          TRY(launch_timed(ctx, qc, prec, fn, a, std::min(kPersistentBlocks, cap), timed_ok));
+          TRY(launch_timed(ctx, qc, prec, fn, a, std::min(tn.blocks, cap), timed_ok));
         } else {
           TRY(class_kernel(ctx, qc.la, qc.lb, qc.lc, qc.ld, kModeJK, bm, prec, &fn));
           if (!fn.valid())
@@ -1350,8 +1443,7 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
                                                         class_name(qc.la, qc.lb, qc.lc, qc.ld) +
                                                         " not served by this library");
           a.tile_range = ranges + (prec ? 2 : 0);
-          TRY(launch_timed(ctx, qc, prec, fn, a, std::min(kPersistentBlocks, (cap + 3) / 4),
-                           timed_ok));
+          TRY(launch_timed(ctx, qc, prec, fn, a, std::min(tn.blocks, (cap + 3) / 4), timed_ok));
         }
       }
     }
@@ -1449,6 +1541,11 @@ int rchem_ctx_create(int device, rchem_ctx** out) {
   }
   auto* ctx = new rchem_ctx();
   ctx->device = device;
+  ctx->lds_block = prop.sharedMemPerBlock;
+  {
+    const std::string tp = tuning_path();
+    if (!tp.empty()) load_tuning(ctx, tp);
+  }
   if (const char* e = std::getenv("RCHEM_LDS_DIGESTION")) ctx->lds_digestion = std::atoi(e) != 0;
   if (const char* e = std::getenv("RCHEM_ERI_GENERIC")) ctx->force_generic = std::atoi(e) != 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
@@ -1659,6 +1756,49 @@ int rchem_jk_device_noalloc(rchem_ctx* ctx, const double* dD, double* dJ, double
   TRY(set_device(ctx));
   const JkBuffers bf = jk_buffers_in(ctx, base);
   return jk_device(ctx, dD, dJ, dK, &o, &bf);
+}
+
+int rchem_lds_variant_info(int variant, int* waves, int* min_waves_per_simd, int* prefetch) {
+  if (variant < 0 || variant >= kNumLdsVariants) return RCHEM_ERR_INVALID;
+  if (waves) *waves = kLdsVariants[variant].waves;
+  if (min_waves_per_simd) *min_waves_per_simd = kLdsVariants[variant].min_waves_per_simd;
+  if (prefetch) *prefetch = kLdsVariants[variant].prefetch;
+  return RCHEM_OK;
+}
+
+int rchem_set_class_tuning(rchem_ctx* ctx, int la, int lb, int lc, int ld, int fp32, int variant,
+                           int blocks, int chunk_tiles) {
+  if (!ctx) return RCHEM_ERR_INVALID;
+  if (la < 0 || la > kMaxL || lb < 0 || lb > kMaxL || lc < 0 || lc > kMaxL || ld < 0 ||
+      ld > kMaxL || (fp32 != 0 && fp32 != 1) || !tune_valid(variant, blocks, chunk_tiles))
+    return ctx->fail(RCHEM_ERR_INVALID, "rchem_set_class_tuning: argument out of range");
+  ClassTune t;
+  t.la = la; t.lb = lb; t.lc = lc; t.ld = ld; t.fp32 = fp32;
+  t.variant = variant;
+  t.blocks = blocks;
+  t.chunk_tiles = chunk_tiles;
+  set_tune(ctx, t);
+  ctx->plan_tau = -1.0;  // chunk sizes are part of the plan
+  return RCHEM_OK;
+}
+
+int rchem_get_class_tuning(const rchem_ctx* ctx, int la, int lb, int lc, int ld, int fp32,
+                           int* variant, int* blocks, int* chunk_tiles) {
+  if (!ctx || !variant || !blocks || !chunk_tiles) return RCHEM_ERR_INVALID;
+  const ClassTune t = tune_for(ctx, la, lb, lc, ld, fp32 ? 1 : 0);
+  *variant = t.variant;
+  *blocks = t.blocks;
+  *chunk_tiles = t.chunk_tiles;
+  return RCHEM_OK;
+}
+
+int rchem_load_tuning(rchem_ctx* ctx, const char* path, int* entries) {
+  if (!ctx) return RCHEM_ERR_INVALID;
+  ctx->tuning.clear();
+  ctx->plan_tau = -1.0;
+  const int n = (path && *path) ? load_tuning(ctx, path) : 0;
+  if (entries) *entries = n;
+  return RCHEM_OK;
 }
 
 int rchem_jk(rchem_ctx* ctx, const double* D, double* J, double* K, const rchem_opts* opts) {
@@ -2888,7 +3028,7 @@ int rchem_rhf_gradient(rchem_ctx* ctx, int natom, const double* Z, const double*
     const GenericProgram* gp = nullptr;
     TRY(find_program(ctx, hc.la, hc.lb, hc.lc, hc.ld, 1, &gp));
     if (gp) {
-      HIPC(eri_generic_launch(a, *gp, kModeGrad, kBoysExact, 0, ctx->stream));
+      HIPC(eri_generic_launch(a, *gp, kModeGrad, kBoysExact, 0, ctx->stream, ctx->lds_block));
       continue;
     }
     // one program per centre
@@ -2898,7 +3038,7 @@ int rchem_rhf_gradient(rchem_ctx* ctx, int natom, const double* Z, const double*
         return ctx->fail(RCHEM_ERR_UNSUPPORTED, "gradient of ERI class " +
                                                     class_name(hc.la, hc.lb, hc.lc, hc.ld) +
                                                     " not served by this library");
-      HIPC(eri_generic_launch(a, *gp, kModeGrad, kBoysExact, 0, ctx->stream));
+      HIPC(eri_generic_launch(a, *gp, kModeGrad, kBoysExact, 0, ctx->stream, ctx->lds_block));
     }
   }
   HIPC(grad_pairs_launch(dgp.as<double>(), dpa.as<int2>(), npairs, dgrad.as<double>(),

@@ -55,6 +55,18 @@ struct Chunk {
 
 enum EriMode { kModeJK = 0, kModeDiag = 1, kModeTensor = 2, kModeQuartet = 3, kModeGrad = 4 };
 
+// Launch shapes of the LDS-digestion kernel a class can be compiled in (the
+// per-class tuning table picks one): waves per workgroup (they share the bra's
+// K rows), the minimum waves per SIMD the register allocation must allow (0:
+// no bound), and whether a lane fetches its next ket primitive pair one
+// iteration ahead (latency hiding in registers instead of by other waves).
+struct LdsVariant {
+  int waves, min_waves_per_simd, prefetch;
+};
+constexpr LdsVariant kLdsVariants[] = {{4, 0, 1}, {4, 0, 0}, {8, 0, 1}, {8, 0, 0},
+                                       {4, 6, 0}, {8, 6, 0}, {8, 8, 0}, {16, 0, 0}};
+constexpr int kNumLdsVariants = sizeof(kLdsVariants) / sizeof(kLdsVariants[0]);
+
 // Pair-indexed Coulomb accumulators: component k of pair p (pair class X) is
 // Jp[base_X + k * n_X + (p - begin_X)], so the 64 lanes of a wave, which hold
 // consecutive ket pairs, update consecutive addresses.
@@ -133,6 +145,7 @@ struct PlanClass {
   int task_begin;          // first task; tasks = (bra - bra_begin) * nbk + bucket
   int same;                // X == Y: keep only ket <= bra
   int unit_begin;          // first bra unit; units = bra - bra_begin
+  int chunk64, chunk32;    // tiles per LDS-digestion chunk, FP64 and FP32 lists
   int pad;
 };
 
@@ -153,7 +166,6 @@ struct PlanArgs {
   int* hi;
   double* stats;           // per class: {quartets64, quartets32, prim64, prim32}
   int nunits;              // bra units (class, bra pair)
-  int chunk_tiles;         // tiles per chunk at most
   // several molecules in one basis: kets restricted to the bra's molecule,
   // segs[(bucket slot) * nmol + molecule] = its pairs [begin, end) in the bucket
   const int2* segs;        // null: one molecule

@@ -1,0 +1,84 @@
+"""Launch tuning of the Fock build: every compiled launch shape of the
+LDS-digestion kernel, workgroup count and chunk size gives the same J and K
+(to the order of floating-point atomics: 1e-12 relative), and the tuning-table
+reader takes well-formed lines and skips the rest."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from rchem_amd import basis as rb
+from rchem_amd._lib import PREC_MIXED, default_opts, lib
+from rchem_amd.inputs import water_cluster
+
+pytestmark = pytest.mark.gpu
+SP_CLASSES = [(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (1, 1, 1, 0), (1, 1, 1, 1)]
+
+
+def _nvariants():
+    n = 0
+    while lib().rchem_lds_variant_info(n, None, None, None) == 0:
+        n += 1
+    return n
+
+
+def test_every_launch_shape_gives_the_same_jk(gpu_ctx):
+    L = lib()
+    h = gpu_ctx.h
+    Z, xyz = water_cluster(dims=(2, 2, 2), seed=8)
+    B = rb.Basis.new(Z, xyz, "6-31G", ctx=gpu_ctx)
+    n = B.nbf
+    X = np.random.default_rng(4).standard_normal((n, n))
+    D = 0.05 * (X + X.T)
+    nv = _nvariants()
+    assert nv >= 2
+    try:
+        for prec in (0, PREC_MIXED):
+            o = default_opts(precision=prec, mixed_thresh=1e-5)
+            assert L.rchem_load_tuning(h, None, None) == 0
+            J0 = np.zeros((n, n)); K0 = np.zeros((n, n))
+            rb.JK_direct(J0, K0, B, D, o)
+            scale = max(np.abs(J0).max(), np.abs(K0).max())
+            for v in range(nv):
+                blocks, chunk = [(2048, 64), (300, 7), (4096, 128)][v % 3]
+                print(f"case: launch shape {v}, {blocks} workgroups, chunks of {chunk}, "
+                      f"precision {prec}", flush=True)
+                for c in SP_CLASSES:
+                    for fp32 in (0, 1):
+                        assert L.rchem_set_class_tuning(h, *c, fp32, v, blocks, chunk) == 0
+                J = np.zeros((n, n)); K = np.zeros((n, n))
+                rb.JK_direct(J, K, B, D, o)
+                assert np.max(np.abs(J - J0)) < 1e-12 * scale, v
+                assert np.max(np.abs(K - K0)) < 1e-12 * scale, v
+    finally:
+        L.rchem_load_tuning(h, None, None)
+
+
+def test_tuning_table_reader(gpu_ctx, tmp_path):
+    L = lib()
+    h = gpu_ctx.h
+    p = tmp_path / "t.txt"
+    p.write_text("# comment\n1 0 0 0 0 3 1024 32  # kept\n1 0 0 0 1 99 1024 32\n"
+                 "0 0 0 0 1 2 5000 128\nnot a line\n2 2 2 2 0 1 0 64\n")
+    cnt = ctypes.c_int()
+    assert L.rchem_load_tuning(h, str(p).encode(), ctypes.byref(cnt)) == 0
+    assert cnt.value == 2
+    v, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    assert L.rchem_get_class_tuning(h, 1, 0, 0, 0, 0, ctypes.byref(v), ctypes.byref(b),
+                                    ctypes.byref(c)) == 0
+    assert (v.value, b.value, c.value) == (3, 1024, 32)
+    assert L.rchem_get_class_tuning(h, 1, 0, 0, 0, 1, ctypes.byref(v), ctypes.byref(b),
+                                    ctypes.byref(c)) == 0
+    assert (v.value, b.value, c.value) == (0, 2048, 64)  # bad variant: skipped
+    assert L.rchem_get_class_tuning(h, 0, 0, 0, 0, 1, ctypes.byref(v), ctypes.byref(b),
+                                    ctypes.byref(c)) == 0
+    assert (v.value, b.value, c.value) == (2, 5000, 128)
+    assert L.rchem_set_class_tuning(h, 0, 0, 0, 0, 0, 1000, 1, 1) == 1
+    assert L.rchem_load_tuning(h, None, ctypes.byref(cnt)) == 0 and cnt.value == 0
+    shipped = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                           "rchem_amd", "data", "tuning_gfx950.txt")
+    if os.path.exists(shipped):
+        assert L.rchem_load_tuning(h, shipped.encode(), ctypes.byref(cnt)) == 0
+        assert cnt.value > 0
+        L.rchem_load_tuning(h, None, None)
