@@ -51,24 +51,61 @@ def core_guess_density(S, H, nocc):
     return C[:, :nocc] @ C[:, :nocc].T
 
 
-def cpu_baseline(Z, xyz, D, gpu_quartets, seconds):
-    """Reference JK_direct inner loop on one core over random function quartets."""
+def host_cores():
+    """Cores this process may use, capped at the 16 a one-GPU box gives it."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def _sample_rate(O, B, D, addr, seconds, threads):
+    """Function quartets/s of the JK_direct body over random quartets."""
+    rng = np.random.default_rng(1234)
+    done, t_total, batch = 0, 0.0, 256 * threads
+    while t_total < seconds:
+        q = rng.integers(0, B.n, (batch, 4))
+        t0 = time.perf_counter()
+        O.jk_direct_sample(B, D, q, boys=O.BOYS_COMPAT, cints_fn=addr, threads=threads)
+        t_total += time.perf_counter() - t0
+        done += batch
+        batch = min(batch * 2, 8192 * threads)
+    return done, t_total
+
+
+def cpu_baseline(Z, xyz, D, gpu_quartets, seconds, water_gpu_scf_s):
+    """The reference algorithm on the host: rchem's JK_direct loop (basis.rs:383-428)
+    over the reference's own cints coulomb_repulsion (oracle/_ref) -- on one core
+    (rchem is single-threaded) and on every core (OpenMP over mu, each row
+    private, basis.rs:422-423).  (H2O)_64: a bounded random sample of function
+    quartets, extrapolated to the n^4 quartets one build visits.  Water STO-3G
+    (7 functions): rchem's whole SCF end to end (rchem.rs:40-90)."""
     import oracle as O
+    from rchem_amd.inputs import WATER_CRAWFORD, WATER_Z
     bse = O.load_bse_json(os.path.join(ROOT, "rchem_amd", "data", "6-31g.json"))
     B = O.cgtos_from_bse(bse, Z, xyz)
     addr = O.cints_coulomb_repulsion_address()
     kind = "reference" if addr else "port"
-    rng = np.random.default_rng(1234)
-    done, t_total, batch = 0, 0.0, 256
-    while t_total < seconds:
-        q = rng.integers(0, B.n, (batch, 4))
-        t0 = time.perf_counter()
-        O.jk_direct_sample(B, D, q, boys=O.BOYS_COMPAT, cints_fn=addr)
-        t_total += time.perf_counter() - t0
-        done += batch
-        batch = min(batch * 2, 8192)
+    src = "the reference cints.c coulomb_repulsion" if addr else "the C restatement"
+    cores = host_cores()
+    done, t_total = _sample_rate(O, B, D, addr, seconds, 1)
     rate = done / t_total                       # function quartets / s (each: J and K ERI)
     fock_s = B.n ** 4 / rate                    # JK_direct visits n^4 function quartets
+    done_mt, t_mt = _sample_rate(O, B, D, addr, seconds, cores)
+    rate_mt = done_mt / t_mt
+    fock_mt = B.n ** 4 / rate_mt
+    # water STO-3G, rchem's SCF end to end on the reference integrals
+    sto = O.load_bse_json(os.path.join(ROOT, "rchem_amd", "data", "sto-3g.json"))
+    wxyz = np.array(WATER_CRAWFORD, dtype=float).reshape(-1, 3)
+    Bw = O.cgtos_from_bse(sto, WATER_Z, wxyz)
+    water = {}
+    for th in (1, cores):
+        t0 = time.perf_counter()
+        S, T, V = O.one_electron(Bw, WATER_Z, wxyz)
+        r = O.scf(S, T + V, lambda Dm: O.jk_direct(Bw, Dm, boys=O.BOYS_COMPAT, cints_fn=addr,
+                                                   threads=th), nocc=5)
+        water[th] = (time.perf_counter() - t0, r)
     return {
         "value": gpu_quartets / fock_s,
         "unit": "shell quartets/s (same Fock build)",
@@ -76,9 +113,19 @@ def cpu_baseline(Z, xyz, D, gpu_quartets, seconds):
         "kind": kind,
         "sample": (f"{done} random (mu,nu,la,si) function quartets of the (H2O)_64 6-31G "
                    f"JK_direct loop in {t_total:.1f} s ({rate:.0f}/s, 2 contracted ERIs each via "
-                   f"{'the reference cints.c coulomb_repulsion' if addr else 'the C restatement'}); "
-                   f"extrapolated Fock build {fock_s:.3g} s"),
+                   f"{src}); extrapolated Fock build {fock_s:.3g} s"),
         "fock_build_s_extrapolated": fock_s,
+        "all_cores": {
+            "value": gpu_quartets / fock_mt, "cores": cores,
+            "sample": (f"{done_mt} random function quartets in {t_mt:.1f} s on {cores} threads "
+                       f"({rate_mt:.0f}/s); extrapolated Fock build {fock_mt:.3g} s"),
+            "fock_build_s_extrapolated": fock_mt},
+        "water_sto3g_scf": {
+            "what": "rchem's whole SCF (plain iterations, |dE| < 1e-11) on water STO-3G, "
+                    "JK_direct over the reference integrals every iteration",
+            "iterations": water[1][1].iterations, "e_elec": water[1][1].e_elec,
+            "seconds_1_core": water[1][0], "seconds_all_cores": water[cores][0],
+            "cores": cores, "gpu_seconds": water_gpu_scf_s},
     }
 
 
@@ -244,6 +291,10 @@ def main():
                          "RHF gradient of (H2O)_32 6-31G against the forward SCF")
     ap.add_argument("--shells", type=int, default=2000,
                     help="eri2000 workload: number of shells (the cube grows to keep the density)")
+    ap.add_argument("--no-tuning-compare", dest="tuning_compare", action="store_false",
+                    help="skip timing the builds again without the tuning table")
+    ap.add_argument("--mixed-sweep", type=float, nargs="*", default=[1e-7, 1e-6, 1e-5, 1e-4, 1e-3],
+                    help="h2o64: mixed-precision thresholds to sweep after the SCF")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (for HBM counter scaling)")
     args = ap.parse_args()
@@ -350,6 +401,22 @@ def main():
     if args.workload == "eri2000":
         precisions = ["fp64"]
     runs = {p: measure(p) for p in precisions}
+    # the same builds with every class at its default launch choices (no
+    # tuning table), against the table the context loaded
+    tuning = None
+    table = os.environ.get("RCHEM_TUNING_FILE",
+                           os.path.join(ROOT, "rchem_amd", "data", "tuning_gfx950.txt"))
+    if args.tuning_compare and table and os.path.exists(table):
+        ctx.check(L.rchem_load_tuning(ctx.h, None, None))
+        base = {p: measure(p) for p in precisions}
+        ctx.check(L.rchem_load_tuning(ctx.h, table.encode(), None))
+        tuning = {"table": os.path.relpath(table, ROOT)}
+        for p in precisions:
+            tc = {c["class"]: c["avg_ms"] for c in runs[p]["classes"]}
+            tuning[p] = {"default_ms": base[p]["ms"], "tuned_ms": runs[p]["ms"],
+                         "classes": [{"class": c["class"], "default_ms": c["avg_ms"],
+                                      "tuned_ms": tc.get(c["class"])}
+                                     for c in base[p]["classes"]]}
     main_run = runs["mixed"] if "mixed" in runs else runs["fp64"]
     st = main_run["st"]
     classes = main_run["classes"]
@@ -398,6 +465,8 @@ def main():
         for _ in range(3):
             rb.JK_direct(Jh, Kh, B, D, oh)
         out["host_buffers_ms_per_step"] = (time.perf_counter() - t0) / 3 * 1e3
+    if tuning is not None:
+        out["tuning"] = tuning
     if "mixed" in runs and "fp64" in runs:
         f = runs["fp64"]
         out["fp64"] = {"ms_per_step": f["ms"], "value": f["quartets_per_s"],
@@ -442,6 +511,39 @@ def main():
                 precision=PREC_FP64 if p == "fp64" else PREC_MIXED, schwarz_thresh=1e-12))
             e_build[p] = float(np.sum(D_conv * (2.0 * Hc + 2.0 * Jm - Km)))
         scf["delta_e_fock_build"] = abs(e_build["mixed"] - e_build["fp64"])
+        # mixed-precision threshold sweep: the build's energy error at the
+        # converged density against the all-FP64 build, and its time on the
+        # bench density
+        sweep = []
+        for thr in args.mixed_sweep:
+            om = default_opts(precision=PREC_MIXED, schwarz_thresh=1e-12, mixed_thresh=thr)
+            Jm, Km = np.zeros((n, n)), np.zeros((n, n))
+            rb.JK_direct(Jm, Km, B, D_conv, om)
+            de = abs(float(np.sum(D_conv * (2.0 * Hc + 2.0 * Jm - Km))) - e_build["fp64"])
+            st_m = rb.jk_last_stats(B)
+
+            def step_m():
+                ctx.check(L.rchem_jk_device(ctx.h, ctypes.c_void_p(dD.data_ptr()),
+                                            ctypes.c_void_p(dJ.data_ptr()),
+                                            ctypes.c_void_p(dK.data_ptr()), ctypes.byref(om)))
+            step_m()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                step_m()
+            torch.cuda.synchronize()
+            ms_m = (time.perf_counter() - t0) / args.steps * 1e3
+            sweep.append({"mixed_thresh": thr, "ms": ms_m,
+                          "speedup_vs_fp64": runs["fp64"]["ms"] / ms_m if "fp64" in runs else None,
+                          "fp32_quartet_fraction": st_m.quartets_fp32 / max(st_m.quartets, 1.0),
+                          "delta_e_fock_build": de})
+            say(f"mixed threshold {thr:g}: {ms_m:.2f} ms, |dE| {de:.2e} Eh")
+        scf["mixed_sweep"] = sweep
+        ok = [x for x in sweep if x["delta_e_fock_build"] < 1e-6]
+        if ok:
+            best = max(ok, key=lambda x: x["mixed_thresh"])
+            scf["largest_threshold_within_1e-6"] = best["mixed_thresh"]
+            scf["speedup_at_largest_threshold"] = best["speedup_vs_fp64"]
         scf["note"] = ("DIIS(8); mixed switches to all-FP64 J/K once |dE| < 1e-7 Eh; "
                        "convergence |dE| < 1e-10 Eh and max|FDS-SDF| < 1e-8; 'seconds' "
                        "replays the captured Fock build (hipGraph) on every iteration and "
@@ -449,7 +551,16 @@ def main():
                        "instead of the eigensolver")
         out["scf"] = scf
     if not args.no_cpu_baseline and args.workload == "h2o64":
-        out["cpu_baseline"] = cpu_baseline(Z, xyz, D, st.quartets, args.cpu_seconds)
+        from rchem_amd.inputs import WATER_CRAWFORD, WATER_Z
+        Bw = rb.Basis.new(WATER_Z, WATER_CRAWFORD, "STO-3G", ctx=ctx)
+        ow = default_opts(boys_mode=1, schwarz_thresh=0.0)  # compat Boys, as the reference
+        rb.scf(Bw, WATER_CRAWFORD, WATER_Z, nocc=5, opts=ow)
+        t0 = time.perf_counter()
+        rw, _ = rb.scf(Bw, WATER_CRAWFORD, WATER_Z, nocc=5, opts=ow)
+        water_gpu = time.perf_counter() - t0
+        out["cpu_baseline"] = cpu_baseline(Z, xyz, D, st.quartets, args.cpu_seconds, water_gpu)
+        out["cpu_baseline"]["water_sto3g_scf"]["gpu_e_elec"] = rw.e_elec
+        say("CPU baseline done")
     print(json.dumps(out), flush=True)
     ctx.close()
 

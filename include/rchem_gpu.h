@@ -197,7 +197,8 @@ int rchem_jk_class_timings(rchem_ctx *ctx, rchem_class_timing *out, int max, int
  * "la lb lc ld fp32 variant blocks chunk_tiles", '#' comments.  A shape a class
  * is not compiled in falls back to shape 0.  Setting an entry re-plans the
  * next build (which allocates). */
-int rchem_lds_variant_info(int variant, int *waves, int *min_waves_per_simd, int *prefetch);
+int rchem_lds_variant_info(int variant, int *waves, int *min_waves_per_simd, int *prefetch,
+                           int *unroll, int *branch_free);
 int rchem_set_class_tuning(rchem_ctx *ctx, int la, int lb, int lc, int ld, int fp32, int variant,
                            int blocks, int chunk_tiles);
 int rchem_get_class_tuning(const rchem_ctx *ctx, int la, int lb, int lc, int ld, int fp32,

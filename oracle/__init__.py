@@ -68,6 +68,14 @@ def lib():
         L.orc_jk_direct_rows.restype = None
         L.orc_jk_direct_rows.argtypes = basis_args + [_dp, ctypes.c_int, ctypes.c_void_p,
                                                       ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.orc_jk_direct_rows_mt.restype = None
+        L.orc_jk_direct_rows_mt.argtypes = basis_args + [_dp, ctypes.c_int, ctypes.c_void_p,
+                                                          ctypes.c_int, ctypes.c_int,
+                                                          ctypes.c_int, _dp, _dp]
+        L.orc_jk_direct_sample_mt.restype = None
+        L.orc_jk_direct_sample_mt.argtypes = basis_args + [_dp, ctypes.c_int, ctypes.c_void_p,
+                                                            ctypes.c_int, _ip, ctypes.c_int,
+                                                            _dp, _dp]
         L.orc_jk_direct_sample.restype = None
         L.orc_jk_direct_sample.argtypes = basis_args + [_dp, ctypes.c_int, ctypes.c_void_p,
                                                         ctypes.c_int, _ip, _dp, _dp]
@@ -278,21 +286,26 @@ def jk_inmem(I, D):
     return J, K
 
 
-def jk_direct(basis: CGTOBasis, D, boys=BOYS_COMPAT, rows=None, cints_fn=None):
+def jk_direct(basis: CGTOBasis, D, boys=BOYS_COMPAT, rows=None, cints_fn=None, threads=1):
     """JK_direct (basis.rs:383-428), literally; ``rows`` limits mu to a range so
     a bounded sample can be timed.  ``cints_fn`` = address of the reference's
-    compiled coulomb_repulsion (oracle/_ref) to use instead of the restatement."""
+    compiled coulomb_repulsion (oracle/_ref) to use instead of the restatement.
+    ``threads`` > 1 splits the mu loop over threads (each row is private)."""
     n = basis.n
     args, keep = basis.args()
     Dd, Dp = _d(D)
     J = np.zeros((n, n)); K = np.zeros((n, n))
     r0, r1 = rows if rows is not None else (0, n)
-    lib().orc_jk_direct_rows(*args, Dp, boys, cints_fn, r0, r1, J.ctypes.data_as(_dp),
-                             K.ctypes.data_as(_dp))
+    if threads > 1:
+        lib().orc_jk_direct_rows_mt(*args, Dp, boys, cints_fn, r0, r1, int(threads),
+                                    J.ctypes.data_as(_dp), K.ctypes.data_as(_dp))
+    else:
+        lib().orc_jk_direct_rows(*args, Dp, boys, cints_fn, r0, r1, J.ctypes.data_as(_dp),
+                                 K.ctypes.data_as(_dp))
     return J, K
 
 
-def jk_direct_sample(basis: CGTOBasis, D, quartets, boys=BOYS_COMPAT, cints_fn=None):
+def jk_direct_sample(basis: CGTOBasis, D, quartets, boys=BOYS_COMPAT, cints_fn=None, threads=1):
     """JK_direct's inner body over an explicit list of function quartets
     (rows of ``quartets`` = mu, nu, la, si); returns partial (J, K)."""
     n = basis.n
@@ -300,8 +313,12 @@ def jk_direct_sample(basis: CGTOBasis, D, quartets, boys=BOYS_COMPAT, cints_fn=N
     Dd, Dp = _d(D)
     q, qp = _i(np.asarray(quartets).reshape(-1, 4))
     J = np.zeros((n, n)); K = np.zeros((n, n))
-    lib().orc_jk_direct_sample(*args, Dp, boys, cints_fn, len(q), qp, J.ctypes.data_as(_dp),
-                               K.ctypes.data_as(_dp))
+    if threads > 1:
+        lib().orc_jk_direct_sample_mt(*args, Dp, boys, cints_fn, len(q), qp, int(threads),
+                                      J.ctypes.data_as(_dp), K.ctypes.data_as(_dp))
+    else:
+        lib().orc_jk_direct_sample(*args, Dp, boys, cints_fn, len(q), qp, J.ctypes.data_as(_dp),
+                                   K.ctypes.data_as(_dp))
     return J, K
 
 

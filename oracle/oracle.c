@@ -597,6 +597,74 @@ void orc_jk_direct_rows(int n, const double *centre, const int *powers, const in
   free_basis(&b);
 }
 
+/* orc_jk_direct_rows on `nthreads` threads (OpenMP over mu): row mu of J and K
+ * is written by one thread only, as each iteration of JK_direct's outer loop
+ * writes only row mu (basis.rs:392-426), so the threads never share an
+ * output element.  The CPU baseline's all-core leg. */
+void orc_jk_direct_rows_mt(int n, const double *centre, const int *powers, const int *prim_off,
+                           const double *exps, const double *coefs, const double *D,
+                           int boys_mode, void *cints_coulomb_repulsion, int mu_begin, int mu_end,
+                           int nthreads, double *J, double *K) {
+  orc_basis b;
+  make_basis(&b, n, centre, powers, prim_off, exps, coefs);
+  orc_eri_fn prim = orc_coulomb_repulsion;
+  if (cints_coulomb_repulsion) {
+    g_cints_eri = (orc_cints_eri_fn)cints_coulomb_repulsion;
+    prim = prim_via_cints;
+  }
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+  for (int mu = mu_begin; mu < mu_end; ++mu) {
+    for (int nu = 0; nu < n; ++nu) {
+      double j = 0.0, k = 0.0;
+      for (int la = 0; la < n; ++la)
+        for (int si = 0; si < n; ++si) {
+          const double d = D[(size_t)la * n + si];
+          j += contracted_eri(&b, mu, nu, la, si, boys_mode, prim) * d;
+          k += contracted_eri(&b, mu, la, nu, si, boys_mode, prim) * d;
+        }
+      J[(size_t)mu * n + nu] = j;
+      K[(size_t)mu * n + nu] = k;
+    }
+  }
+  free_basis(&b);
+}
+
+/* orc_jk_direct_sample on `nthreads` threads: each thread accumulates its
+ * share of the quartets into private J, K, summed at the end.  Used to time a
+ * bounded sample of the reference loop on every core. */
+void orc_jk_direct_sample_mt(int n, const double *centre, const int *powers, const int *prim_off,
+                             const double *exps, const double *coefs, const double *D,
+                             int boys_mode, void *cints_coulomb_repulsion, int nq, const int *q,
+                             int nthreads, double *J, double *K) {
+  orc_basis b;
+  make_basis(&b, n, centre, powers, prim_off, exps, coefs);
+  orc_eri_fn prim = orc_coulomb_repulsion;
+  if (cints_coulomb_repulsion) {
+    g_cints_eri = (orc_cints_eri_fn)cints_coulomb_repulsion;
+    prim = prim_via_cints;
+  }
+  const size_t n2 = (size_t)n * n;
+#pragma omp parallel num_threads(nthreads)
+  {
+    double *Jt = calloc(n2, sizeof(double)), *Kt = calloc(n2, sizeof(double));
+#pragma omp for schedule(dynamic, 64)
+    for (int t = 0; t < nq; ++t) {
+      int mu = q[4 * t], nu = q[4 * t + 1], la = q[4 * t + 2], si = q[4 * t + 3];
+      double d = D[(size_t)la * n + si];
+      Jt[(size_t)mu * n + nu] += contracted_eri(&b, mu, nu, la, si, boys_mode, prim) * d;
+      Kt[(size_t)mu * n + nu] += contracted_eri(&b, mu, la, nu, si, boys_mode, prim) * d;
+    }
+#pragma omp critical
+    for (size_t i = 0; i < n2; ++i) {
+      J[i] += Jt[i];
+      K[i] += Kt[i];
+    }
+    free(Jt);
+    free(Kt);
+  }
+  free_basis(&b);
+}
+
 /* One-electron matrices S, T, V (basis.rs:214-338): lower triangle computed and
  * mirrored; V = sum_C Z_C <mu| -1/r_C |nu>.  V uses the exact Boys function,
  * as rchem's os86 path does (os86.rs:770). */

@@ -9,7 +9,9 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "librchem_gpu.so")
+# RCHEM_LIB selects another build of the same library (an experimental build
+# kept out of the tree); the default is the in-tree one.
+LIB_PATH = os.environ.get("RCHEM_LIB") or os.path.join(HERE, "librchem_gpu.so")
 
 RCHEM_OK = 0
 STATUS_NAMES = {0: "OK", 1: "ERR_INVALID", 2: "ERR_HIP", 3: "ERR_NO_DEVICE",
@@ -87,7 +89,7 @@ SIGNATURES = {
     "rchem_jk_class_timings": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ClassTiming),
                                               ctypes.c_int, _ip]),
     "rchem_scf_opts_default": (ctypes.c_int, [ctypes.POINTER(SCFOpts)]),
-    "rchem_lds_variant_info": (ctypes.c_int, [ctypes.c_int, _ip, _ip, _ip]),
+    "rchem_lds_variant_info": (ctypes.c_int, [ctypes.c_int, _ip, _ip, _ip, _ip, _ip]),
     "rchem_set_class_tuning": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 8),
     "rchem_get_class_tuning": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 5 +
                                [_ip, _ip, _ip]),

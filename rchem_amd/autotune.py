@@ -4,9 +4,10 @@ precision, on the device it runs on.
 For every class kernel the build launches it measures, with the library's own
 per-class HIP-event timings:
   1. every compiled launch shape of the LDS-digestion kernel (waves per
-     workgroup, register bound for occupancy, one-ahead ket fetch;
-     rchem_lds_variant_info), all classes switched together, one timed set of
-     builds per shape;
+     workgroup, register bound for occupancy, one-ahead ket fetch, bra
+     primitive pairs per inner iteration, branch-free Boys function;
+     rchem_lds_variant_info) at chunks of 64 and of 256 tiles, all classes
+     switched together, one timed set of builds per setting;
   2. persistent workgroup counts (512 ... 8192) at each class's best shape;
   3. chunk sizes (16 ... 256 tiles) at the best shape and count.
 A choice is kept for a class only where it beats the class's best so far by
@@ -66,7 +67,7 @@ class Tuner:
         self.opts = {"fp64": default_opts(precision=PREC_FP64, schwarz_thresh=1e-12),
                      "mixed": default_opts(precision=PREC_MIXED, schwarz_thresh=1e-12)}
         nv = 0
-        while self.L.rchem_lds_variant_info(nv, None, None, None) == 0:
+        while self.L.rchem_lds_variant_info(nv, None, None, None, None, None) == 0:
             nv += 1
         self.nvariants = nv
 
@@ -130,7 +131,11 @@ def tune(tuner):
         for k in keys:
             tuner.set(k, best[k]["variant"], best[k]["blocks"], best[k]["chunk"])
 
-    sweep("shape", range(1, tuner.nvariants), lambda cur, v: (v, cur["blocks"], cur["chunk"]))
+    # shape and chunk size interact (the chunk is the workgroup's share of one
+    # bra pair): every shape at a short and at a long chunk first
+    for chunk in (64, 256):
+        sweep(f"shape at chunk {chunk}:", range(tuner.nvariants),
+              lambda cur, v, c=chunk: (v, cur["blocks"], c))
     sweep("workgroups", BLOCKS, lambda cur, b: (cur["variant"], b, cur["chunk"]))
     sweep("chunk", CHUNKS, lambda cur, c: (cur["variant"], cur["blocks"], c))
     final = tuner.time_all()

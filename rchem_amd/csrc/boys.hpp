@@ -2,8 +2,8 @@
 //
 // Two evaluations:
 //   * exact (production): F_{M}(T) from a Taylor expansion about the nearest
-//     grid point T0 of a table (spacing 1/2, 13 terms in FP64, |T - T0| <= 1/4,
-//     truncation < 3e-18 relative; 7 terms in FP32), then downward recursion
+//     grid point T0 of a table (spacing 1/8, |T - T0| <= 1/16; 9 terms in FP64,
+//     truncation (1/16)^9/9! < 5e-17 relative, 6 in FP32, < 3e-10), then downward recursion
 //     F_m = (2T F_{m+1} + e^{-T}) / (2m+1), which is stable, with
 //     e^{-T} = e^{-T0} e^{T0 - T} from the same row and a Taylor series.  For
 //     T >= 36 the asymptote F_0 = sqrt(pi/T)/2 (error < e^{-T}) and the upward
@@ -23,19 +23,27 @@
 namespace rchem {
 
 constexpr int kBoysMaxM = 9;                          // highest m served
-constexpr int kBoysTerms = 13;                        // Taylor terms, FP64
-constexpr int kBoysTermsF = 7;                        // Taylor terms, FP32
+// The finer the grid, the fewer Taylor terms a lookup reads from LDS: at
+// spacing 1/8 a lookup reads 9 doubles (72 B) where spacing 1/2 needed 13
+// (104 B); the Horner chains are shorter by the same count.
+constexpr int kBoysTerms = 9;                         // Taylor terms, FP64
+constexpr int kBoysTermsF = 6;                        // Taylor terms, FP32
 constexpr double kBoysTMax = 36.0;
-constexpr double kBoysInvH = 2.0;                     // grid spacing 1/2
-constexpr int kBoysRows = 73;                         // T0 = 0, 0.5, ..., 36
+constexpr double kBoysInvH = 8.0;                     // grid spacing 1/8
+constexpr int kBoysRows = 289;                        // T0 = 0, 1/8, ..., 36
 // Device layout: for M = 0..kBoysMaxM a block of kBoysRows rows holding
 // F_{M+j}(T0) / j!, j = 0..kBoysTerms-1 (the Taylor coefficients of F_M about
 // T0 in powers of T0 - T), then one row of exp(-T0).  A kernel of order M
-// stages its own block and the exp row (8 KB in FP64).
+// stages its own block (20.8 KB in FP64) and, for M >= 2 (downward
+// recursion), the exp row (2.3 KB).
 constexpr int kBoysBlock = kBoysRows * kBoysTerms;
 constexpr int kBoysExpOffset = (kBoysMaxM + 1) * kBoysBlock;
 constexpr int kBoysTableSize = kBoysExpOffset + kBoysRows;
 constexpr int kBoysStageSize = kBoysBlock + kBoysRows;
+// Staged entries a kernel of highest order M needs (no exp row for M <= 1).
+__host__ __device__ constexpr int boys_stage_size(int M) {
+  return M <= 1 ? kBoysBlock : kBoysStageSize;
+}
 
 enum BoysMode { kBoysExact = 0, kBoysCompat = 1 };
 
@@ -93,6 +101,7 @@ __device__ __forceinline__ void boys_exact(R T, const R* __restrict__ table, R* 
       for (int m = M - 1; m >= 0; --m) F[m] = (t2 * F[m + 1] + e) * R(1.0 / (2 * m + 1));
     } else if constexpr (M > 0) {
       // e^{d} by its Taylor series with constant coefficients 1/j!
+      static_assert(K <= 14, "inv_fact holds 14 terms");
       constexpr double inv_fact[14] = {1.0, 1.0, 1.0 / 2, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720,
                                        1.0 / 5040, 1.0 / 40320, 1.0 / 362880, 1.0 / 3628800,
                                        1.0 / 39916800, 1.0 / 479001600, 1.0 / 6227020800.0};
@@ -107,13 +116,53 @@ __device__ __forceinline__ void boys_exact(R T, const R* __restrict__ table, R* 
   } else {
     const R rt = fast_rsqrt(T);
     F[0] = R(0.88622692545275801365) * rt;  // sqrt(pi)/2 / sqrt(T); erf(sqrt T) = 1 - O(e^-T)
-    if constexpr (M > 0) {
+    if constexpr (M == 1) {
+      // F_1 = (F_0 - e^{-T}) / (2T): e^{-T} / F_0 < 2e-15 for T >= 36, so the
+      // exponential is dropped (the branch costs a few operations per lane)
+      F[1] = F[0] * (R(0.5) * rt * rt);
+    } else if constexpr (M > 1) {
       // e^{-T} < 2.4e-16 here: it only corrects the upward recursion, so a
       // single-precision exponential is accurate enough
       const R e = static_cast<R>(__expf(-static_cast<float>(T)));
       const R h = R(0.5) * rt * rt;  // 1/(2T)
 #pragma unroll
       for (int m = 0; m < M; ++m) F[m + 1] = (R(2 * m + 1) * F[m] - e) * h;
+    }
+  }
+}
+
+// Orders 0 and 1 without a branch: the table form (at the clamped grid point
+// for T beyond the table) and the asymptotic form are both evaluated and the
+// one that applies is selected, so the caller's loop body stays one basic
+// block.  Higher orders take boys_exact.
+template <int M, typename R>
+__device__ __forceinline__ void boys_exact_select(R T, const R* __restrict__ table,
+                                                  R* __restrict__ F) {
+  if constexpr (M > 1) {
+    boys_exact<M, R>(T, table, F);
+  } else {
+    constexpr int K = sizeof(R) == 8 ? kBoysTerms : kBoysTermsF;
+    const bool near = T < R(kBoysTMax);
+    const R Tn = near ? T : R(0);
+    const int k = static_cast<int>(Tn * R(kBoysInvH) + R(0.5));
+    const R d = R(k) * (R(1) / R(kBoysInvH)) - Tn;
+    const R* __restrict__ c = table + k * kBoysTerms;
+    const R rt = fast_rsqrt(near ? R(1) : T);
+    const R f0 = R(0.88622692545275801365) * rt;
+    if constexpr (M == 1) {
+      R p = c[K - 1], q = R(0);
+#pragma unroll
+      for (int j = K - 2; j >= 0; --j) {
+        q = q * d + p;
+        p = p * d + c[j];
+      }
+      F[0] = near ? p : f0;
+      F[1] = near ? q : f0 * (R(0.5) * rt * rt);
+    } else {
+      R p = c[K - 1];
+#pragma unroll
+      for (int j = K - 2; j >= 0; --j) p = p * d + c[j];
+      F[0] = near ? p : f0;
     }
   }
 }
@@ -139,14 +188,16 @@ __device__ __forceinline__ void boys_exact_rt(int M, double T, const double* __r
   }
 }
 
-// Copies the block of order boys_block_order(M) and the exp row into LDS (every thread of the
-// workgroup calls it; a barrier follows).
+// Copies the block of order boys_block_order(M) and, where the recursion needs
+// it (M >= 2), the exp row into LDS: boys_stage_size(M) entries (every thread of
+// the workgroup calls it; a barrier follows).
 template <int M, typename R>
 __device__ __forceinline__ void stage_boys(R* __restrict__ dst, const R* __restrict__ src) {
   const int m = boys_block_order(M <= kBoysMaxM ? M : kBoysMaxM);
   for (int k = threadIdx.x; k < kBoysBlock; k += blockDim.x) dst[k] = src[m * kBoysBlock + k];
-  for (int k = threadIdx.x; k < kBoysRows; k += blockDim.x)
-    dst[kBoysBlock + k] = src[kBoysExpOffset + k];
+  if constexpr (boys_stage_size(M) > kBoysBlock)
+    for (int k = threadIdx.x; k < kBoysRows; k += blockDim.x)
+      dst[kBoysBlock + k] = src[kBoysExpOffset + k];
 }
 
 // Compat: the reference's Fgamma(m, x) (cints.c:302-373), NR gser/gcf.
@@ -195,10 +246,12 @@ __device__ __forceinline__ void boys_compat(double T, double* __restrict__ F) {
   for (int m = 0; m <= M; ++m) F[m] = fgamma_compat(double(m), T);
 }
 
-template <int M, int MODE, typename R>
+template <int M, int MODE, typename R, bool SELECT = false>
 __device__ __forceinline__ void boys(R T, const R* __restrict__ table, R* __restrict__ F) {
   if constexpr (MODE == kBoysCompat) {
     boys_compat<M>(T, F);
+  } else if constexpr (SELECT) {
+    boys_exact_select<M, R>(T, table, F);
   } else {
     boys_exact<M, R>(T, table, F);
   }

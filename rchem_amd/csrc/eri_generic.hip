@@ -101,17 +101,20 @@ __device__ void setup_record(const EriArgs& a, const PairRec& bp, const PairRec&
   const double rs = fast_rsqrt(zb + zk);
   const double ooze = rs * rs;
   const double roz = zk * ooze, roe = zb * ooze;
-  const double PQx = pb.px - pk.px, PQy = pb.py - pk.py, PQz = pb.pz - pk.pz;
+  // P - Q = (P - A) - (Q - C) + (A - C)
+  const double PQx = pb.pax - pk.pax + (bp.A[0] - kp.A[0]);
+  const double PQy = pb.pay - pk.pay + (bp.A[1] - kp.A[1]);
+  const double PQz = pb.paz - pk.paz + (bp.A[2] - kp.A[2]);
   const double T = zb * roz * (PQx * PQx + PQy * PQy + PQz * PQz);
-  rec[(kPA + 0) * P + q] = pb.px - bp.A[0];
-  rec[(kPA + 1) * P + q] = pb.py - bp.A[1];
-  rec[(kPA + 2) * P + q] = pb.pz - bp.A[2];
+  rec[(kPA + 0) * P + q] = pb.pax;
+  rec[(kPA + 1) * P + q] = pb.pay;
+  rec[(kPA + 2) * P + q] = pb.paz;
   rec[(kWP + 0) * P + q] = -roz * PQx;
   rec[(kWP + 1) * P + q] = -roz * PQy;
   rec[(kWP + 2) * P + q] = -roz * PQz;
-  rec[(kQC + 0) * P + q] = pk.px - kp.A[0];
-  rec[(kQC + 1) * P + q] = pk.py - kp.A[1];
-  rec[(kQC + 2) * P + q] = pk.pz - kp.A[2];
+  rec[(kQC + 0) * P + q] = pk.pax;
+  rec[(kQC + 1) * P + q] = pk.pay;
+  rec[(kQC + 2) * P + q] = pk.paz;
   rec[(kWQ + 0) * P + q] = roe * PQx;
   rec[(kWQ + 1) * P + q] = roe * PQy;
   rec[(kWQ + 2) * P + q] = roe * PQz;

@@ -63,7 +63,10 @@ __global__ __launch_bounds__(256) void eri_grad_kernel(const EriArgs a) {
     const int ik = kp.p0 + j * kp.ps;
     const PrimRec pk = a.prims[ik];
     const double wC = 2.0 * a.prim_alpha[ik];
-    const double QCx = pk.px - kp.A[0], QCy = pk.py - kp.A[1], QCz = pk.pz - kp.A[2];
+    const double QCx = pk.pax, QCy = pk.pay, QCz = pk.paz;
+    // Q - A = (Q - C) - (A - C)
+    const double QAx = QCx - (bp.A[0] - kp.A[0]), QAy = QCy - (bp.A[1] - kp.A[1]),
+                 QAz = QCz - (bp.A[2] - kp.A[2]);
     for (int i = 0; i < bp.np; ++i) {
       const int ib = bp.p0 + i * bp.ps;
       const PrimRec pb = a.prims[ib];
@@ -72,7 +75,7 @@ __global__ __launch_bounds__(256) void eri_grad_kernel(const EriArgs a) {
       const double rs = fast_rsqrt(zb + zk);
       const double ooze = rs * rs;
       const double roz = zk * ooze, roe = zb * ooze;
-      const double PQx = pb.px - pk.px, PQy = pb.py - pk.py, PQz = pb.pz - pk.pz;
+      const double PQx = pb.pax - QAx, PQy = pb.pay - QAy, PQz = pb.paz - QAz;
       const double T = zb * roz * (PQx * PQx + PQy * PQy + PQz * PQz);
       double F[L + 1];
       boys_exact<L, double>(T, boys_tab, F);
@@ -80,7 +83,7 @@ __global__ __launch_bounds__(256) void eri_grad_kernel(const EriArgs a) {
       double Gm[L + 1];
 #pragma unroll
       for (int m = 0; m <= L; ++m) Gm[m] = pref * F[m];
-      G::template vrr<double>(Gm, pb.px - bp.A[0], pb.py - bp.A[1], pb.pz - bp.A[2], -roz * PQx,
+      G::template vrr<double>(Gm, pb.pax, pb.pay, pb.paz, -roz * PQx,
                               -roz * PQy, -roz * PQz, QCx, QCy, QCz, roe * PQx, roe * PQy,
                               roe * PQz, pb.oo2z, pk.oo2z, 0.5 * ooze, roz, roe, 2.0 * aa,
                               2.0 * (zb - aa), wC, acc);

@@ -30,7 +30,7 @@ import math
 import sys
 
 AX = "xyz"
-N_LDS_VARIANTS = 8  # entries of kLdsVariants (types.hpp)
+N_LDS_VARIANTS = 10  # entries of kLdsVariants (types.hpp)
 
 
 def cart(l):

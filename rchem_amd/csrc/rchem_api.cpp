@@ -279,7 +279,7 @@ struct rchem_ctx {
   size_t nprims = 0;
   std::vector<Bucket> buckets[kNumPairClasses];
   int class_begin[kNumPairClasses + 1] = {0};
-  DevBuf pairs_d, prims_d;
+  DevBuf pairs_d, prims_d, prims_f_d;
   DevBuf alpha_d;  // exponent of the first shell of every primitive pair (prims layout)
 
   // tiles for a threshold
@@ -531,9 +531,10 @@ void make_prims(const rchem_ctx* ctx, int a, int b, double eps, std::vector<Prim
       const double z = al + be;
       PrimRec pr;
       pr.zeta = z;
-      pr.px = (al * sa.xyz[0] + be * sb.xyz[0]) / z;
-      pr.py = (al * sa.xyz[1] + be * sb.xyz[1]) / z;
-      pr.pz = (al * sa.xyz[2] + be * sb.xyz[2]) / z;
+      // P - A = (alpha A + beta B) / zeta - A = -(beta / zeta) (A - B)
+      pr.pax = -be / z * (sa.xyz[0] - sb.xyz[0]);
+      pr.pay = -be / z * (sa.xyz[1] - sb.xyz[1]);
+      pr.paz = -be / z * (sa.xyz[2] - sb.xyz[2]);
       pr.kab = ctx->ccoef[sa.p0 + ip] * ctx->ccoef[sb.p0 + jp] * std::sqrt(2.0) *
                std::pow(M_PI, 1.25) / z * std::exp(-al * be / z * AB2);
       pr.oo2z = 0.5 / z;
@@ -671,7 +672,18 @@ int layout_pairs(rchem_ctx* ctx, const std::vector<PairHost>& sel, double prim_e
   if (!prims.empty())
     HIPC(hipMemcpyAsync(ctx->prims_d.p, prims.data(), sizeof(PrimRec) * prims.size(),
                         hipMemcpyHostToDevice, ctx->stream));
-  HIPC(hipStreamSynchronize(ctx->stream));
+  std::vector<PrimRecF> primsf(prims.size());
+  for (size_t k = 0; k < prims.size(); ++k) {
+    const PrimRec& r = prims[k];
+    primsf[k] = PrimRecF{static_cast<float>(r.zeta), static_cast<float>(r.pax),
+                         static_cast<float>(r.pay), static_cast<float>(r.paz),
+                         static_cast<float>(r.kab), static_cast<float>(r.oo2z)};
+  }
+  HIPC(ctx->prims_f_d.ensure(sizeof(PrimRecF) * std::max<size_t>(primsf.size(), 1)));
+  if (!primsf.empty())
+    HIPC(hipMemcpyAsync(ctx->prims_f_d.p, primsf.data(), sizeof(PrimRecF) * primsf.size(),
+                        hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));  // primsf is a local
   return RCHEM_OK;
 }
 
@@ -828,6 +840,7 @@ int compute_schwarz(rchem_ctx* ctx) {
     EriArgs a{};
     a.pairs = ctx->pairs_d.as<PairRec>();
     a.prims = ctx->prims_d.as<PrimRec>();
+  a.prims_f = ctx->prims_f_d.as<PrimRecF>();
     a.tiles = ctx->tiles_d.as<Tile>() + ranges[X].first;
     a.ntiles = ranges[X].second;
     a.nbf = ctx->nbf;
@@ -1227,6 +1240,7 @@ EriArgs base_args(const rchem_ctx* ctx) {
   EriArgs a{};
   a.pairs = ctx->pairs_d.as<PairRec>();
   a.prims = ctx->prims_d.as<PrimRec>();
+  a.prims_f = ctx->prims_f_d.as<PrimRecF>();
   a.nbf = ctx->nbf;
   a.boys_table = ctx->boys_d.as<double>();
   a.boys_table_f = ctx->boys_f_d.as<float>();
@@ -1250,9 +1264,9 @@ int run_eri_host(rchem_ctx* ctx, int mode, int bm, double* out) {
   return RCHEM_OK;
 }
 
-// LDS an LDS-digestion workgroup needs beside the K rows (Boys block, bra
-// primitive pairs; 11 KB in FP64)
-constexpr size_t kLdsStaticReserve = 12 * 1024;
+// LDS an LDS-digestion workgroup needs beside the K rows (its Boys block and
+// exp row: at most 23.1 KB in FP64)
+constexpr size_t kLdsStaticReserve = 24 * 1024;
 
 int launch_timed(rchem_ctx* ctx, const QClass& qc, int fp32, const ClassKernel& fn,
                  const EriArgs& a, int blocks, bool may_time = true) {
@@ -1758,11 +1772,15 @@ int rchem_jk_device_noalloc(rchem_ctx* ctx, const double* dD, double* dJ, double
   return jk_device(ctx, dD, dJ, dK, &o, &bf);
 }
 
-int rchem_lds_variant_info(int variant, int* waves, int* min_waves_per_simd, int* prefetch) {
+int rchem_lds_variant_info(int variant, int* waves, int* min_waves_per_simd, int* prefetch,
+                           int* unroll, int* branch_free) {
   if (variant < 0 || variant >= kNumLdsVariants) return RCHEM_ERR_INVALID;
-  if (waves) *waves = kLdsVariants[variant].waves;
-  if (min_waves_per_simd) *min_waves_per_simd = kLdsVariants[variant].min_waves_per_simd;
-  if (prefetch) *prefetch = kLdsVariants[variant].prefetch;
+  const LdsVariant& v = kLdsVariants[variant];
+  if (waves) *waves = v.waves;
+  if (min_waves_per_simd) *min_waves_per_simd = v.min_waves_per_simd;
+  if (prefetch) *prefetch = v.prefetch;
+  if (unroll) *unroll = v.unroll;
+  if (branch_free) *branch_free = v.branch_free;
   return RCHEM_OK;
 }
 

@@ -13,8 +13,8 @@
 
 namespace rchem {
 
-// Primitives per shell accepted by the library (so a pair has at most 64
-// primitive pairs, which the LDS-digestion kernel stages in LDS).
+// Primitives per shell accepted by the library (a pair has at most 64
+// primitive pairs).
 constexpr int kMaxPrimPerShell = 8;
 constexpr int kMaxPrimPairs = kMaxPrimPerShell * kMaxPrimPerShell;
 
@@ -29,12 +29,20 @@ struct PairRec {
   int flags;     // bit 0: a and b are different shells
 };
 
-// One primitive pair (i of a, j of b).
+// One primitive pair (i of a, j of b).  The product centre is stored relative
+// to shell a's centre, P - A = -(alpha_j / zeta) (A - B), so that no lane
+// forms it by cancellation, and P - Q = (P - A) - (Q - C) + (A - C).
 struct PrimRec {
   double zeta;          // alpha_i + alpha_j
-  double px, py, pz;    // Gaussian product centre
+  double pax, pay, paz; // P - A
   double kab;           // c_i N_i c_j N_j sqrt(2) pi^(5/4) / zeta exp(-alpha_i alpha_j/zeta |AB|^2)
   double oo2z;          // 1 / (2 zeta)
+};
+
+// The same record rounded to FP32 (same index), for the FP32 kernels: half
+// the bytes per primitive pair.
+struct PrimRecF {
+  float zeta, pax, pay, paz, kab, oo2z;
 };
 
 // A wavefront's work: one bra pair against up to 64 consecutive ket pairs of
@@ -58,13 +66,18 @@ enum EriMode { kModeJK = 0, kModeDiag = 1, kModeTensor = 2, kModeQuartet = 3, kM
 // Launch shapes of the LDS-digestion kernel a class can be compiled in (the
 // per-class tuning table picks one): waves per workgroup (they share the bra's
 // K rows), the minimum waves per SIMD the register allocation must allow (0:
-// no bound), and whether a lane fetches its next ket primitive pair one
-// iteration ahead (latency hiding in registers instead of by other waves).
+// no bound), whether a lane fetches its next ket primitive pair one iteration
+// ahead (latency hiding in registers instead of by other waves), how many bra
+// primitive pairs one iteration of the inner loop takes (2: two independent
+// dependency chains to interleave), and whether the Boys function of orders
+// 0 and 1 is evaluated without a branch (both the table and the asymptotic
+// form, then a select: one basic block, so the unrolled chains interleave).
 struct LdsVariant {
-  int waves, min_waves_per_simd, prefetch;
+  int waves, min_waves_per_simd, prefetch, unroll, branch_free;
 };
-constexpr LdsVariant kLdsVariants[] = {{4, 0, 1}, {4, 0, 0}, {8, 0, 1}, {8, 0, 0},
-                                       {4, 6, 0}, {8, 6, 0}, {8, 8, 0}, {16, 0, 0}};
+constexpr LdsVariant kLdsVariants[] = {
+    {4, 0, 1, 1, 0}, {4, 0, 0, 1, 0}, {8, 0, 1, 1, 0}, {8, 0, 0, 1, 0}, {8, 6, 0, 1, 0},
+    {8, 8, 0, 1, 0}, {16, 0, 0, 1, 0}, {8, 0, 0, 2, 1}, {4, 0, 0, 2, 1}, {8, 0, 0, 1, 1}};
 constexpr int kNumLdsVariants = sizeof(kLdsVariants) / sizeof(kLdsVariants[0]);
 
 // Pair-indexed Coulomb accumulators: component k of pair p (pair class X) is
@@ -77,6 +90,7 @@ struct JpClass {
 struct EriArgs {
   const PairRec* pairs;
   const PrimRec* prims;
+  const PrimRecF* prims_f;  // prims rounded to FP32
   const Tile* tiles;
   int ntiles;              // host-sized launch: tiles[0 .. ntiles)
   const int* tile_range;   // if set (persistent launch): tiles[range[0] .. range[1]),

@@ -1,7 +1,10 @@
 """Launch tuning of the Fock build: every compiled launch shape of the
-LDS-digestion kernel, workgroup count and chunk size gives the same J and K
-(to the order of floating-point atomics: 1e-12 relative), and the tuning-table
-reader takes well-formed lines and skips the rest."""
+LDS-digestion kernel, workgroup count and chunk size gives the same J and K --
+in FP64 to the order of floating-point atomics (1e-12 relative); in a mixed
+build the shapes also order the FP32 arithmetic differently, so there the bar
+is the FP32 quartets' share (threshold 1e-5 times FP32 rounding: 1e-9
+relative) -- and the tuning-table reader takes well-formed lines and skips
+the rest."""
 import ctypes
 import os
 
@@ -18,7 +21,7 @@ SP_CLASSES = [(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (1, 1, 1, 
 
 def _nvariants():
     n = 0
-    while lib().rchem_lds_variant_info(n, None, None, None) == 0:
+    while lib().rchem_lds_variant_info(n, None, None, None, None, None) == 0:
         n += 1
     return n
 
@@ -40,6 +43,7 @@ def test_every_launch_shape_gives_the_same_jk(gpu_ctx):
             J0 = np.zeros((n, n)); K0 = np.zeros((n, n))
             rb.JK_direct(J0, K0, B, D, o)
             scale = max(np.abs(J0).max(), np.abs(K0).max())
+            bar = (1e-9 if prec else 1e-12) * scale
             for v in range(nv):
                 blocks, chunk = [(2048, 64), (300, 7), (4096, 128)][v % 3]
                 print(f"case: launch shape {v}, {blocks} workgroups, chunks of {chunk}, "
@@ -49,8 +53,8 @@ def test_every_launch_shape_gives_the_same_jk(gpu_ctx):
                         assert L.rchem_set_class_tuning(h, *c, fp32, v, blocks, chunk) == 0
                 J = np.zeros((n, n)); K = np.zeros((n, n))
                 rb.JK_direct(J, K, B, D, o)
-                assert np.max(np.abs(J - J0)) < 1e-12 * scale, v
-                assert np.max(np.abs(K - K0)) < 1e-12 * scale, v
+                assert np.max(np.abs(J - J0)) < bar, (v, prec, np.max(np.abs(J - J0)) / scale)
+                assert np.max(np.abs(K - K0)) < bar, (v, prec, np.max(np.abs(K - K0)) / scale)
     finally:
         L.rchem_load_tuning(h, None, None)
 
