@@ -231,6 +231,13 @@ typedef struct rchem_scf_opts {
                           an iteration whose F' has no gap.  Single-molecule SCF
                           and gradient; the batched SCF ignores it.  0 (default):
                           rchem's eigh every iteration (rchem.rs:74-78) */
+  double damping;      /* a in [0, 1): each new density is mixed with the previous one,
+                          D <- (1 - a) D_new + a D_old (a convergence aid for hard
+                          cases; the converged density is the same stationary point);
+                          0 (default): rchem's undamped update */
+  double damping_until; /* damping applies while the orthogonalised commutator
+                          max |X^T (FDS - SDF) X| (with DIIS) or |dE| (without) is at
+                          least this, per molecule in a batch; 0: every iteration */
 } rchem_scf_opts;
 
 typedef struct rchem_scf_result {

@@ -49,7 +49,8 @@ class SCFOpts(ctypes.Structure):
     _fields_ = [("nocc", ctypes.c_int), ("max_iter", ctypes.c_int),
                 ("thresh_e", ctypes.c_double), ("jk", Opts), ("verbose", ctypes.c_int),
                 ("diis", ctypes.c_int), ("graph", ctypes.c_int),
-                ("purify", ctypes.c_int)]
+                ("purify", ctypes.c_int), ("damping", ctypes.c_double),
+                ("damping_until", ctypes.c_double)]
 
 
 class SCFResult(ctypes.Structure):

@@ -201,7 +201,8 @@ def schwarz(basis: Basis):
 
 
 def scf(basis: Basis, atomcoords, atomnos, nocc, thresh_e=1e-11, max_iter=1024,
-        opts: Opts | None = None, verbose=False, diis=0, graph=False, purify=False):
+        opts: Opts | None = None, verbose=False, diis=0, graph=False, purify=False,
+        damping=0.0, damping_until=0.0):
     """rchem.rs:10-90 on the device: returns (SCFResult, D).  diis = k > 0
     replaces rchem's plain iterations by DIIS over k Fock matrices; graph
     replays the Fock build as a hipGraph; purify forms each iteration's
@@ -216,6 +217,8 @@ def scf(basis: Basis, atomcoords, atomnos, nocc, thresh_e=1e-11, max_iter=1024,
     o.thresh_e = float(thresh_e)
     o.jk = opts or default_opts()
     o.verbose = int(bool(verbose))
+    o.damping = float(damping)
+    o.damping_until = float(damping_until)
     xyz = np.ascontiguousarray(atomcoords, dtype=np.float64).reshape(-1, 3)
     Z = np.ascontiguousarray(atomnos, dtype=np.float64)
     res = SCFResult()
@@ -228,7 +231,7 @@ def scf(basis: Basis, atomcoords, atomnos, nocc, thresh_e=1e-11, max_iter=1024,
 
 
 def batch_scf(molecules, basis_set_name, thresh_e=1e-11, max_iter=1024, diis=0,
-              opts: Opts | None = None, ctx=None, graph=False):
+              opts: Opts | None = None, ctx=None, graph=False, damping=0.0, damping_until=0.0):
     """Many independent molecules [(atomnos, atomcoords), ...] solved together
     on the device; returns a list of SCFResult (the context's basis is cleared)."""
     ctx = ctx or default_context()
@@ -243,6 +246,8 @@ def batch_scf(molecules, basis_set_name, thresh_e=1e-11, max_iter=1024, diis=0,
     o.diis = int(diis)
     o.graph = int(bool(graph))
     o.jk = opts or default_opts()
+    o.damping = float(damping)
+    o.damping_until = float(damping_until)
     res = (SCFResult * len(molecules))()
     rc = lib().rchem_batch_scf(ctx.h, len(molecules), _iptr(natom), _iptr(Z), _dptr(xyz),
                                basis_set_name.encode(), ctypes.byref(o), res)

@@ -149,7 +149,29 @@ unsigned blocks_for(size_t n) {
   return static_cast<unsigned>(g ? g : 1);
 }
 
+// D <- (1 - a_b) D + a_b D_old on molecule b's diagonal block (a_b = 0: kept).
+__global__ void blend_kernel(double* __restrict__ D, const double* __restrict__ Dold, size_t n,
+                             const int* __restrict__ f0, const int* __restrict__ nb,
+                             const double* __restrict__ alpha) {
+  const int b = blockIdx.y;
+  const double a = alpha[b];
+  if (a == 0.0) return;
+  const int nn = nb[b];
+  for (size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+       k < static_cast<size_t>(nn) * nn; k += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const size_t i = f0[b] + k / nn, j = f0[b] + k % nn;
+    D[i * n + j] = (1.0 - a) * D[i * n + j] + a * Dold[i * n + j];
+  }
+}
+
 }  // namespace
+
+hipError_t batch_blend(double* D, const double* Dold, size_t n, const int* f0, const int* nb,
+                       const double* alpha, int nmax, int nbatch, hipStream_t s) {
+  hipLaunchKernelGGL(blend_kernel, dim3(blocks_for(static_cast<size_t>(nmax) * nmax), nbatch),
+                     dim3(256), 0, s, D, Dold, n, f0, nb, alpha);
+  return hipGetLastError();
+}
 
 hipError_t batch_pack(const double* A, size_t n, const int* f0, const int* nb, int nmax, int nbatch,
                       double pad, double* out, hipStream_t s) {

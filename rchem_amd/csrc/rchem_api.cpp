@@ -87,6 +87,8 @@ hipError_t batch_antisym(const double* A, double* E, int nmax, int nbatch, hipSt
 hipError_t batch_dots(const double* base, const long long* pairs, int npairs, size_t len,
                       double* out, hipStream_t s);
 hipError_t batch_maxabs(const double* E, int nmax, int nbatch, double* out, hipStream_t s);
+hipError_t batch_blend(double* D, const double* Dold, size_t n, const int* f0, const int* nb,
+                       const double* alpha, int nmax, int nbatch, hipStream_t s);
 hipError_t batch_combine(const double* Fs, size_t batch_stride, int m, int k, const double* c,
                          int nmax, int nbatch, double* out, hipStream_t s);
 hipError_t scale_columns_launch(const double* U, const double* w, double* out, int n,
@@ -1197,6 +1199,14 @@ int check_opts(rchem_ctx* ctx, const rchem_opts* o) {
   return RCHEM_OK;
 }
 
+int check_scf_opts(rchem_ctx* ctx, const rchem_scf_opts* o) {
+  if (!(o->damping >= 0.0 && o->damping < 1.0))
+    return ctx->fail(RCHEM_ERR_INVALID, "damping must lie in [0, 1)");
+  if (!(o->damping_until >= 0.0) || !std::isfinite(o->damping_until))
+    return ctx->fail(RCHEM_ERR_INVALID, "damping_until must be finite and >= 0");
+  return RCHEM_OK;
+}
+
 ClassTimer* timer_for(rchem_ctx* ctx, const QClass& qc, int fp32) {
   for (auto& t : ctx->timers)
     if (t.la == qc.la && t.lb == qc.lb && t.lc == qc.lc && t.ld == qc.ld && t.fp32 == fp32)
@@ -2182,6 +2192,8 @@ int rchem_scf_opts_default(rchem_scf_opts* o) {
   o->diis = 0;
   o->graph = 0;
   o->purify = 0;
+  o->damping = 0.0;
+  o->damping_until = 0.0;
   return RCHEM_OK;
 }
 
@@ -2256,6 +2268,7 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
   if (opts->nocc <= 0 || opts->nocc > n)
     return ctx->fail(RCHEM_ERR_INVALID, "nocc out of range");
   TRY(check_opts(ctx, &opts->jk));
+  TRY(check_scf_opts(ctx, opts));
   TRY(set_device(ctx));
   const size_t n2 = static_cast<size_t>(n) * n;
   // one-electron matrices on the host path of this library, then resident on device
@@ -2356,6 +2369,19 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
     HIPC(energy_launch(dD.as<double>(), dH.as<double>(), Fm, dE.as<double>(), n2, ctx->stream));
     HIPC(hipMemcpyAsync(e, dE.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPC(hipStreamSynchronize(ctx->stream));
+    return RCHEM_OK;
+  };
+  // D <- (1 - a) D + a D_old (opts->damping), D_old saved by save_old()
+  auto save_old = [&]() -> int {
+    HIPC(hipMemcpyAsync(dDo.p, dD.p, n2 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    return RCHEM_OK;
+  };
+  auto damp = [&](double measure) -> int {
+    if (opts->damping <= 0.0 || measure < opts->damping_until) return RCHEM_OK;
+    const double keep = 1.0 - opts->damping, mix = opts->damping;
+    const rocblas_int nn2 = static_cast<rocblas_int>(n2);
+    BLASC(rocblas_dscal(hb, nn2, &keep, dD.as<double>(), 1));
+    BLASC(rocblas_daxpy(hb, nn2, &mix, dDo.as<double>(), 1, dD.as<double>(), 1));
     return RCHEM_OK;
   };
   TRY(density(dH.as<double>(), false));
@@ -2478,21 +2504,27 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
           BLASC(rocblas_daxpy(hb, nn, &ci, dFs.as<double>() + static_cast<size_t>(i) * n2, 1,
                               dFx.as<double>(), 1));
         }
+        TRY(save_old());
         TRY(density(dFx.as<double>(), false));
+        TRY(damp(errmax));
       } else {
         count = 0;  // singular subspace: restart from the current Fock matrix
+        TRY(save_old());
         TRY(density(dF.as<double>(), false));
+        TRY(damp(errmax));
       }
     }
   }
+  double e_prev_plain = e_new - 1.0;  // |dE| of the previous iteration (damping measure)
   for (; opts->diis <= 0 && it < opts->max_iter; ++it) {
     TRY(fock_graph.run(ctx, jk, dD.as<double>(), dJ.as<double>(), dK.as<double>(),
                        dH.as<double>(), dF.as<double>(), n2));
-    if (dconv > 0.0)
-      HIPC(hipMemcpyAsync(dDo.p, dD.p, n2 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    if (dconv > 0.0 || opts->damping > 0.0) TRY(save_old());
     TRY(density(dF.as<double>(), false));
+    TRY(damp(std::fabs(e_new - e_prev_plain)));
     const double e_old = e_new;
     TRY(energy(dF.as<double>(), &e_new));
+    e_prev_plain = e_old;
     if (!std::isfinite(e_new)) return ctx->fail(RCHEM_ERR_NONFINITE, "non-finite SCF energy");
     const double de_ = e_new - e_old;
     if (opts->verbose) std::printf("%4d %20.12f %20.12f\n", it, e_new, de_);
@@ -2738,7 +2770,7 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
     return RCHEM_OK;
   };
   TRY(density_dense(dH.as<double>()));  // core-Hamiltonian guess (rchem.rs:56-61)
-  std::vector<double> e_cur(B, 0.0), e_prev(B, 0.0), errmax(B, 0.0);
+  std::vector<double> e_cur(B, 0.0), e_prev(B, 0.0), errmax(B, 0.0), de_last(B, 1.0);
   TRY(energies(dH.as<double>(), &e_cur));
   std::vector<int> conv_at(B, -1);
   std::vector<double> e_final(B, 0.0);
@@ -2773,6 +2805,24 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
     HIPC(derr.ensure(sizeof(double) * B));
   }
   std::vector<double> Bm(static_cast<size_t>(B) * k * k, 0.0);
+  // density damping (opts->damping): previous density and per-molecule factors
+  DevBuf dDold, dalpha;
+  std::vector<double> alpha(B, 0.0);
+  const bool damping = opts->damping > 0.0;
+  if (damping) {
+    HIPC(dDold.ensure(n2 * sizeof(double)));
+    HIPC(dalpha.ensure(sizeof(double) * B));
+  }
+  auto damp_b = [&](const std::vector<double>& measure) -> int {
+    if (!damping) return RCHEM_OK;
+    for (int b = 0; b < B; ++b)
+      alpha[b] = (conv_at[b] < 0 && measure[b] >= opts->damping_until) ? opts->damping : 0.0;
+    HIPC(hipMemcpyAsync(dalpha.p, alpha.data(), sizeof(double) * B, hipMemcpyHostToDevice, st));
+    HIPC(batch_blend(dD.as<double>(), dDold.as<double>(), n, df0.as<int>(), dnb.as<int>(),
+                     dalpha.as<double>(), nmax, B, st));
+    HIPC(hipStreamSynchronize(st));  // alpha is reused next iteration
+    return RCHEM_OK;
+  };
   int count = 0, it = 0;
   bool all_conv = false;
   FockGraph fock_graph;
@@ -2815,8 +2865,12 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
           Bb[slot * k + i] = Bb[i * k + slot] = dots[static_cast<size_t>(b) * m + i];
         }
     } else {
+      if (damping)
+        HIPC(hipMemcpyAsync(dDold.p, dD.p, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
       TRY(density_dense(dF.as<double>()));
+      TRY(damp_b(de_last));  // |dE| of the previous iteration
       TRY(energies(dF.as<double>(), &e_cur));  // rchem's sum D_new o (H + F(D_old))
+      for (int b = 0; b < B; ++b) de_last[b] = std::fabs(e_cur[b] - e_prev[b]);
     }
     double demax = 0.0;
     bool changed = false;
@@ -2896,7 +2950,10 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
       }
       HIPC(hipMemcpyAsync(dcoef.p, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, st));
       HIPC(batch_combine(dFs.as<double>(), m2 * B, m, k, dcoef.as<double>(), nmax, B, dFb.as<double>(), st));
+      if (damping)
+        HIPC(hipMemcpyAsync(dDold.p, dD.p, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
       TRY(density_b(dFb.as<double>()));
+      TRY(damp_b(errmax));
     }
   }
   HIPC(hipStreamSynchronize(st));
@@ -2929,6 +2986,7 @@ int rchem_batch_scf(rchem_ctx* ctx, int nmol, const int* natom, const int* Z, co
   if (nmol <= 0 || !natom || !Z || !xyz || !basis_name || !opts || !res)
     return ctx->fail(RCHEM_ERR_INVALID, "rchem_batch_scf: bad arguments");
   TRY(check_opts(ctx, &opts->jk));
+  TRY(check_scf_opts(ctx, opts));
   TRY(set_device(ctx));
   auto lower = [](std::string s) {
     for (auto& c : s) c = static_cast<char>(std::tolower(static_cast<unsigned char>(c)));

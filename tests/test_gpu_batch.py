@@ -53,18 +53,24 @@ def test_batch_plain_iterations_water_copies(gpu_ctx):
 
 
 def test_batch_several_chunks_and_mixed(gpu_ctx):
-    """Random H/C/N/O random-walk molecules are often hard for RHF (some never
-    converge, alone or in a batch); the bar is on those that do."""
+    """Random H/C/N/O random-walk molecules are often hard for RHF: under plain
+    DIIS many oscillate.  With density damping (30 % of the previous density
+    while the commutator exceeds 1e-2) at least 90 % converge; the energy bars
+    are on those that do."""
     mols = random_molecules(count=120, seed=77, nmin=10, nmax=30)
-    _say(f"case: batch of {len(mols)} molecules over several chunks, FP64 and mixed")
-    r64 = rb.batch_scf(mols, "STO-3G", thresh_e=1e-10, diis=8, max_iter=128, ctx=gpu_ctx)
+    damp = dict(damping=0.3, damping_until=1e-2)
+    _say(f"case: batch of {len(mols)} molecules over several chunks, FP64 and mixed, damped DIIS")
+    r64 = rb.batch_scf(mols, "STO-3G", thresh_e=1e-10, diis=8, max_iter=128, ctx=gpu_ctx,
+                       **damp)
     rmx = rb.batch_scf(mols, "STO-3G", thresh_e=1e-10, diis=8, max_iter=128, ctx=gpu_ctx,
-                       opts=default_opts(precision=PREC_MIXED))
+                       opts=default_opts(precision=PREC_MIXED), **damp)
     ok = [i for i in range(len(mols)) if r64[i].converged and rmx[i].converged]
     _say(f"converged in both: {len(ok)} of {len(mols)}")
-    assert len(ok) >= 0.5 * len(mols)
+    assert len(ok) >= 0.9 * len(mols)
     assert max(abs(r64[i].e_total - rmx[i].e_total) for i in ok) < 1e-6
     for i in ok[:: max(1, len(ok) // 8)]:
-        s = _single(gpu_ctx, mols[i][0], mols[i][1], diis=8)
+        B = rb.Basis.new(mols[i][0], mols[i][1], "STO-3G", ctx=gpu_ctx)
+        s, _ = rb.scf(B, mols[i][1], mols[i][0], nocc=nocc_of(mols[i][0]), thresh_e=1e-10,
+                      diis=8, max_iter=128, opts=default_opts(boys_mode=BOYS_EXACT), **damp)
         if s.converged:
             assert abs(r64[i].e_total - s.e_total) < 1e-8

@@ -25,6 +25,9 @@ from rchem_amd.inputs import nocc_of, random_molecules, water_cluster
 pytestmark = pytest.mark.gpu
 H = 1e-4
 STO3G_NBF = {1: 1, 6: 5, 7: 5, 8: 5}
+# random-walk molecules often oscillate under plain DIIS; mixing 30 % of the
+# previous density while the commutator is large converges ~97 % of them
+DAMP = dict(damping=0.3, damping_until=1e-2)
 O_STO3G = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "rchem_amd",
                        "data", "sto-3g.json")
 
@@ -67,8 +70,10 @@ def test_h2o32_gradient_vs_fd(gpu_ctx):
 @pytest.fixture(scope="module")
 def batch512(gpu_ctx):
     mols = random_molecules(count=512, seed=512, nmin=3, nmax=30)
-    _say(f"case: 512 molecules (seed 512), one batched SCF call, DIIS(8), max 128 iterations")
-    res = rb.batch_scf(mols, "STO-3G", thresh_e=1e-10, diis=8, max_iter=128, ctx=gpu_ctx)
+    _say("case: 512 molecules (seed 512), one batched SCF call, DIIS(8) with density damping "
+         "0.3 while the commutator exceeds 1e-2, max 128 iterations")
+    res = rb.batch_scf(mols, "STO-3G", thresh_e=1e-10, diis=8, max_iter=128, ctx=gpu_ctx,
+                       **DAMP)
     return mols, res
 
 
@@ -76,14 +81,14 @@ def test_batch512_vs_single_molecule_scf(gpu_ctx, batch512):
     mols, res = batch512
     conv = sum(r.converged for r in res)
     _say(f"  converged: {conv} of {len(res)}")
-    assert conv >= 0.7 * len(res)
+    assert conv >= 0.9 * len(res)
     sample = np.random.default_rng(512).choice(len(mols), 16, replace=False)
     compared = 0
     for i in sample:
         Z, xyz = mols[i]
         B = rb.Basis.new(Z, xyz, "STO-3G", ctx=gpu_ctx)
         s, _ = rb.scf(B, xyz, Z, nocc=nocc_of(Z), thresh_e=1e-10, diis=8, max_iter=128,
-                      opts=default_opts(boys_mode=BOYS_EXACT))
+                      opts=default_opts(boys_mode=BOYS_EXACT), **DAMP)
         if not (s.converged and res[i].converged):
             continue
         assert abs(res[i].e_total - s.e_total) < 1e-8, (int(i), res[i].e_total, s.e_total)
@@ -112,7 +117,7 @@ def test_batch512_small_molecules_vs_oracle(gpu_ctx, batch512):
             continue
         B = rb.Basis.new(Z, xyz, "STO-3G", ctx=gpu_ctx)
         s, D = rb.scf(B, xyz, Z, nocc=nocc_of(Z), thresh_e=1e-10, diis=8, max_iter=128,
-                      opts=default_opts(boys_mode=BOYS_EXACT))
+                      opts=default_opts(boys_mode=BOYS_EXACT), **DAMP)
         if not s.converged or abs(s.e_total - res[i].e_total) > 1e-6:
             continue  # the single SCF found another solution: nothing to compare
         assert abs(s.e_total - res[i].e_total) < 1e-8
