@@ -219,18 +219,25 @@ typedef struct rchem_scf_opts {
                           the last k Fock matrices with the orthogonalised commutator
                           X^T (FDS - SDF) X as error, converged when also
                           max |error| < 1e-8 (rchem has no DIIS, rchem.rs:72-90) */
-  int graph;           /* 1: capture an iteration's Fock build (planner, ERI classes,
-                          digestion, F = H + 2J - K) once as a hipGraph and replay it on
-                          every later iteration (recaptured at the precision switch);
-                          0 (default): launch it kernel by kernel */
+  int graph;           /* 1: replay iterations from hipGraphs.  With diis > 0 and
+                          purify = 1 the whole iteration -- Fock build, energy,
+                          commutator, DIIS products and solve, extrapolation,
+                          purification, density, damping -- runs on the device with
+                          no host step and is captured once (after one eager
+                          iteration) and replayed, one host read of a few status
+                          scalars per iteration; otherwise the Fock build (planner,
+                          ERI classes, digestion, F = H + 2J - K) alone.  Recaptured
+                          at the precision switch; a capture the runtime refuses
+                          falls back to kernel-by-kernel launches.  0 (default):
+                          kernel by kernel */
   int purify;          /* 1: each iteration's density D = X P X^T with P, the projector
                           onto the nocc lowest eigenvectors of F' = X^T F X, from
                           canonical purification (GEMMs only) instead of the
-                          eigensolver; the converged orbitals and orbital energies
-                          still come from the eigensolver, which also takes over
-                          an iteration whose F' has no gap.  Single-molecule SCF
-                          and gradient; the batched SCF ignores it.  0 (default):
-                          rchem's eigh every iteration (rchem.rs:74-78) */
+                          eigensolver, which takes over only an iteration whose F'
+                          has no gap.  Single-molecule SCF and gradient (whose
+                          energy-weighted density is W = D F D either way); the
+                          batched SCF ignores it.  0 (default): rchem's eigh every
+                          iteration (rchem.rs:74-78) */
   double damping;      /* a in [0, 1): each new density is mixed with the previous one,
                           D <- (1 - a) D_new + a D_old (a convergence aid for hard
                           cases; the converged density is the same stationary point);
@@ -249,6 +256,8 @@ typedef struct rchem_scf_result {
   int fp64_from;   /* mixed precision: first iteration with all-FP64 J/K (the switch
                       comes when |dE| < 1e-7 Eh; in a batched SCF per molecule, at
                       iteration 64 at the latest); 0 for FP64 runs, -1 if never */
+  int graph_iterations; /* iterations replayed from a whole-iteration hipGraph
+                      (opts->graph with DIIS and purification); 0 otherwise */
 } rchem_scf_result;
 
 int rchem_scf_opts_default(rchem_scf_opts *o);

@@ -56,7 +56,8 @@ class SCFOpts(ctypes.Structure):
 class SCFResult(ctypes.Structure):
     _fields_ = [("e_elec", ctypes.c_double), ("e_nuc", ctypes.c_double),
                 ("e_total", ctypes.c_double), ("iterations", ctypes.c_int),
-                ("converged", ctypes.c_int), ("fp64_from", ctypes.c_int)]
+                ("converged", ctypes.c_int), ("fp64_from", ctypes.c_int),
+                ("graph_iterations", ctypes.c_int)]
 
 
 # name -> (restype, argtypes); every symbol include/rchem_gpu.h declares.

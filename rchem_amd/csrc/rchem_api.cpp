@@ -74,6 +74,15 @@ hipError_t purify_start_launch(const double* F, int n, int nocc, double* lo, dou
                                double* par, double* P, hipStream_t s);
 hipError_t purify_step_launch(double* P, const double* P2, const double* P3, int n,
                               double* coef, hipStream_t s);
+hipError_t diis_push_launch(const double* E, const double* F, double* Es, double* Fs, size_t n2,
+                            const ScfDevState* st, int k, double* part, hipStream_t s);
+hipError_t diis_solve_launch(ScfDevState* st, const double* part, const double* energy,
+                             const ScfDevParams& p, hipStream_t s);
+hipError_t diis_extrapolate_launch(const double* Fs, size_t n2, const ScfDevState* st, double* Fx,
+                                   hipStream_t s);
+hipError_t scf_commit_launch(double* D, const double* Dn, size_t n2, ScfDevState* st,
+                             const double* coef, int steps, double tol, double damping,
+                             hipStream_t s);
 hipError_t batch_pack(const double* A, size_t n, const int* f0, const int* nb, int nmax, int nbatch,
                       double pad, double* out, hipStream_t s);
 hipError_t batch_unpack(const double* in, int nmax, const int* f0, const int* nb, int nbatch,
@@ -318,9 +327,10 @@ struct rchem_ctx {
   DevBuf jp_d, jp_cls_d, jp_ncomp_d;
 
   DevBuf boys_d, boys_f_d;
-  // state of the last SCF: density, MO coefficients and orbital energies
-  // (column-major, as rocSOLVER leaves them), kept for the gradient
-  DevBuf scf_D, scf_C, scf_w, scf_Dold;
+  // state of the last SCF, kept for the gradient: the density and the Fock
+  // matrix it is the density of (W = D F D)
+  DevBuf scf_D, scf_F, scf_Dold;
+  ScfDevState* scf_status_h = nullptr;  // pinned: the scalars read after each iteration
   DevBuf work_Jh, work_Kh, work_D, work_J, work_K;
   rchem_jk_stats last_stats{};
   const double* last_stats_dev = nullptr;  // per-class planner statistics of the last build
@@ -1615,6 +1625,7 @@ int rchem_ctx_destroy(rchem_ctx* ctx) {
   }
   if (ctx->blas) rocblas_destroy_handle(ctx->blas);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->scf_status_h) (void)hipHostFree(ctx->scf_status_h);
   delete ctx;
   return RCHEM_OK;
 }
@@ -2275,10 +2286,9 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
   std::vector<double> S(n2), T(n2), V(n2), H(n2);
   TRY(rchem_one_electron(ctx, natom, Z, xyz, S.data(), T.data(), V.data()));
   for (size_t i = 0; i < n2; ++i) H[i] = T[i] + V[i];
-  DevBuf dS, dH, dX, dW, dF, dFp, dJ, dK, dtmp, de, dinfo, dE;
-  DevBuf& dC = ctx->scf_C;
+  DevBuf dS, dH, dX, dW, dFp, dJ, dK, dtmp, dC, dw, de, dinfo, dE;
+  DevBuf& dF = ctx->scf_F;
   DevBuf& dD = ctx->scf_D;
-  DevBuf& dw = ctx->scf_w;
   DevBuf& dDo = ctx->scf_Dold;
   for (DevBuf* b : {&dS, &dH, &dX, &dW, &dF, &dFp, &dC, &dD, &dJ, &dK, &dtmp, &dDo})
     HIPC(b->ensure(n2 * sizeof(double)));
@@ -2300,29 +2310,33 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
                       dW.as<double>(), n, dtmp.as<double>(), n, &zero, dX.as<double>(), n));
   // Canonical purification of F' (dFp) into P (dP): GEMMs until tr(P - P^2)
   // falls below kPurifyTol, then two more steps (each squares the distance of
-  // every eigenvalue from 0 or 1); false if that does not happen in
+  // every eigenvalue from 0 or 1); *steps = 0 if that does not happen in
   // kPurifyMax steps (no gap at the Fermi level), and the eigensolver takes over.
+  constexpr int kPurifyMax = 200;
+  constexpr double kPurifyTol = 1e-9;
   DevBuf dP, dP2, dP3, dpl, dph, dpar;
   if (opts->purify) {
     for (DevBuf* b : {&dP, &dP2, &dP3}) HIPC(b->ensure(n2 * sizeof(double)));
     HIPC(dpl.ensure(n * sizeof(double)));
     HIPC(dph.ensure(n * sizeof(double)));
-    HIPC(dpar.ensure(8 * sizeof(double)));
+    HIPC(dpar.ensure((4 + 2 * kPurifyMax) * sizeof(double)));
   }
-  constexpr int kPurifyMax = 200;
-  constexpr double kPurifyTol = 1e-9;
-  auto purify = [&](bool* ok) -> int {
-    *ok = false;
+  auto purify_step = [&](double* coef) -> int {
+    BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
+                        dP.as<double>(), n, dP.as<double>(), n, &zero, dP2.as<double>(), n));
+    BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
+                        dP2.as<double>(), n, dP.as<double>(), n, &zero, dP3.as<double>(), n));
+    HIPC(purify_step_launch(dP.as<double>(), dP2.as<double>(), dP3.as<double>(), n, coef,
+                            ctx->stream));
+    return RCHEM_OK;
+  };
+  auto purify = [&](int* steps) -> int {
+    *steps = 0;
     HIPC(purify_start_launch(dFp.as<double>(), n, opts->nocc, dpl.as<double>(), dph.as<double>(),
                              dpar.as<double>(), dP.as<double>(), ctx->stream));
     int extra = -1;
     for (int k = 0; k < kPurifyMax; ++k) {
-      BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
-                          dP.as<double>(), n, dP.as<double>(), n, &zero, dP2.as<double>(), n));
-      BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
-                          dP2.as<double>(), n, dP.as<double>(), n, &zero, dP3.as<double>(), n));
-      HIPC(purify_step_launch(dP.as<double>(), dP2.as<double>(), dP3.as<double>(), n,
-                              dpar.as<double>() + 4, ctx->stream));
+      TRY(purify_step(dpar.as<double>() + 4));
       double t1 = 0.0;
       HIPC(hipMemcpyAsync(&t1, dpar.as<double>() + 5, sizeof(double), hipMemcpyDeviceToHost,
                           ctx->stream));
@@ -2331,29 +2345,41 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
       if (!std::isfinite(t1) || t1 < -kPurifyTol) return RCHEM_OK;
       if (extra < 0 && t1 < kPurifyTol) extra = 2;
       if (extra >= 0 && extra-- == 0) {
-        *ok = true;
+        *steps = k + 1;
         return RCHEM_OK;
       }
     }
     return RCHEM_OK;
   };
-  auto density = [&](const double* Fm, bool exact) -> int {
-    // F' = X^T F X, eigh, C = X C', D = C_occ C_occ^T (rchem.rs:56-59, 74-78, 101-104)
+  // F' = X^T Fm X (rchem.rs:56, 74)
+  auto orthogonalise = [&](const double* Fm) -> int {
     BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, Fm, n,
                         dX.as<double>(), n, &zero, dtmp.as<double>(), n));
     BLASC(rocblas_dgemm(hb, rocblas_operation_transpose, rocblas_operation_none, n, n, n, &one,
                         dX.as<double>(), n, dtmp.as<double>(), n, &zero, dFp.as<double>(), n));
-    if (opts->purify && !exact) {
-      // the same density as X P X^T, P the projector onto F''s nocc lowest eigenvectors
-      bool ok = false;
-      TRY(purify(&ok));
-      if (ok) {
-        BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
-                            dX.as<double>(), n, dP.as<double>(), n, &zero, dtmp.as<double>(), n));
-        BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_transpose, n, n, n,
-                            &one, dtmp.as<double>(), n, dX.as<double>(), n, &zero,
-                            dD.as<double>(), n));
-        return RCHEM_OK;
+    return RCHEM_OK;
+  };
+  // Dout = X P X^T from the purified P in dP
+  auto density_from_p = [&](double* Dout) -> int {
+    BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
+                        dX.as<double>(), n, dP.as<double>(), n, &zero, dtmp.as<double>(), n));
+    BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_transpose, n, n, n, &one,
+                        dtmp.as<double>(), n, dX.as<double>(), n, &zero, Dout, n));
+    return RCHEM_OK;
+  };
+  // Dout = the density of Fm: F' = X^T F X, eigh, C = X C', D = C_occ C_occ^T
+  // (rchem.rs:56-59, 74-78, 101-104), or X P X^T by purification; *psteps
+  // (may be null) receives the purification's step count, 0 if the
+  // eigensolver formed it
+  auto density = [&](const double* Fm, double* Dout, int* psteps) -> int {
+    if (psteps) *psteps = 0;
+    TRY(orthogonalise(Fm));
+    if (opts->purify) {
+      int steps = 0;
+      TRY(purify(&steps));
+      if (steps > 0) {
+        if (psteps) *psteps = steps;
+        return density_from_p(Dout);
       }
     }
     BLASC(rocsolver_dsyevd(hb, rocblas_evect_original, rocblas_fill_upper, n, dFp.as<double>(), n,
@@ -2361,8 +2387,8 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
     BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
                         dX.as<double>(), n, dFp.as<double>(), n, &zero, dC.as<double>(), n));
     BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_transpose, n, n,
-                        opts->nocc, &one, dC.as<double>(), n, dC.as<double>(), n, &zero,
-                        dD.as<double>(), n));
+                        opts->nocc, &one, dC.as<double>(), n, dC.as<double>(), n, &zero, Dout,
+                        n));
     return RCHEM_OK;
   };
   auto energy = [&](const double* Fm, double* e) -> int {
@@ -2384,11 +2410,13 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
     BLASC(rocblas_daxpy(hb, nn2, &mix, dDo.as<double>(), 1, dD.as<double>(), 1));
     return RCHEM_OK;
   };
-  TRY(density(dH.as<double>(), false));
+  int seen_steps = 0;  // purification steps the last eager density needed
+  TRY(density(dH.as<double>(), dD.as<double>(), &seen_steps));
   double e_new = 0.0;
   TRY(energy(dH.as<double>(), &e_new));
   res->converged = 0;
   res->iterations = 0;
+  res->graph_iterations = 0;
   int it = 0;
   // Mixed precision: FP32 quartets until the energy changes by less than
   // kSwitch per iteration, FP64 for every quartet from then on, so the last
@@ -2396,131 +2424,202 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
   constexpr double kSwitch = 1e-7;
   rchem_opts jk = opts->jk;
   FockGraph fock_graph;
-  fock_graph.enabled = opts->graph != 0;
   res->fp64_from = jk.precision == RCHEM_PREC_MIXED ? -1 : 0;
-  auto maybe_switch = [&](double de_) {
-    if (jk.precision == RCHEM_PREC_MIXED && std::fabs(de_) < kSwitch) {
-      jk.precision = RCHEM_PREC_FP64;
-      res->fp64_from = it + 1;
-      return true;
-    }
-    return false;
-  };
   if (opts->diis > 0) {
-    // Pulay DIIS: extrapolate F from the last k Fock matrices so that the
-    // orthogonalised commutators X^T (FDS - SDF) X combine to the smallest norm.
-    const int k = std::min(opts->diis, 32);
+    // Pulay DIIS on the device: extrapolate F from the last k Fock matrices so
+    // that the orthogonalised commutators X^T (FDS - SDF) X combine to the
+    // smallest norm.  With purified densities an iteration has no host step and
+    // is replayed from one captured graph.
+    const int k = std::min(opts->diis, kDiisMax);
     const double errtol = dconv > 0.0 ? dconv : 1e-8;
-    DevBuf dFs, dEs, dE1, dFx;
+    DevBuf dFs, dEs, dEc, dFx, dDn, dpart, dst;
     HIPC(dFs.ensure(n2 * sizeof(double) * k));
     HIPC(dEs.ensure(n2 * sizeof(double) * k));
-    HIPC(dE1.ensure(n2 * sizeof(double)));
-    HIPC(dFx.ensure(n2 * sizeof(double)));
-    std::vector<double> B(static_cast<size_t>(k) * k, 0.0);
-    int count = 0;
-    double e_prev = 0.0;
-    const rocblas_int nn = static_cast<rocblas_int>(n2);
-    for (; it < opts->max_iter; ++it) {
-      TRY(fock_graph.run(ctx, jk, dD.as<double>(), dJ.as<double>(), dK.as<double>(),
-                         dH.as<double>(), dF.as<double>(), n2));
-      double e_cur = 0.0;
-      TRY(energy(dF.as<double>(), &e_cur));  // sum D o (H + F(D)) of the current density
-      if (!std::isfinite(e_cur)) return ctx->fail(RCHEM_ERR_NONFINITE, "non-finite SCF energy");
-      // commutator FDS - SDF, then X^T (.) X
+    for (DevBuf* b : {&dEc, &dFx, &dDn}) HIPC(b->ensure(n2 * sizeof(double)));
+    HIPC(dpart.ensure(sizeof(double) * kDiisBlocks * (kDiisMax + 1)));
+    HIPC(dst.ensure(sizeof(ScfDevState)));
+    if (!ctx->scf_status_h) HIPC(hipHostMalloc(&ctx->scf_status_h, sizeof(ScfDevState)));
+    ScfDevState* hst = ctx->scf_status_h;
+    ScfDevState* st = dst.as<ScfDevState>();
+    HIPC(hipMemsetAsync(dst.p, 0, sizeof(ScfDevState), ctx->stream));
+    auto params = [&]() {
+      ScfDevParams p{};
+      p.thresh_e = opts->thresh_e;
+      p.errtol = errtol;
+      p.switch_de = kSwitch;
+      p.damping = opts->damping;
+      p.damping_until = opts->damping_until;
+      p.k = k;
+      p.mixed = jk.precision == RCHEM_PREC_MIXED;
+      p.fp64 = jk.precision == RCHEM_PREC_FP64;
+      return p;
+    };
+    // energy of the current D, commutator, DIIS push and solve, F_x
+    auto front = [&]() -> int {
+      HIPC(energy_launch(dD.as<double>(), dH.as<double>(), dF.as<double>(), dE.as<double>(), n2,
+                         ctx->stream));
+      // FDS - SDF, then X^T (.) X
       BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
                           dF.as<double>(), n, dD.as<double>(), n, &zero, dtmp.as<double>(), n));
       BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
                           dtmp.as<double>(), n, dS.as<double>(), n, &zero, dFp.as<double>(), n));
-      HIPC(antisym_launch(dFp.as<double>(), dE1.as<double>(), n, ctx->stream));
-      const int slot = count % k;
-      double* Fslot = dFs.as<double>() + static_cast<size_t>(slot) * n2;
-      double* Eslot = dEs.as<double>() + static_cast<size_t>(slot) * n2;
+      HIPC(antisym_launch(dFp.as<double>(), dFx.as<double>(), n, ctx->stream));
       BLASC(rocblas_dgemm(hb, rocblas_operation_transpose, rocblas_operation_none, n, n, n, &one,
-                          dX.as<double>(), n, dE1.as<double>(), n, &zero, dtmp.as<double>(), n));
+                          dX.as<double>(), n, dFx.as<double>(), n, &zero, dtmp.as<double>(), n));
       BLASC(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
-                          dtmp.as<double>(), n, dX.as<double>(), n, &zero, Eslot, n));
-      HIPC(hipMemcpyAsync(Fslot, dF.p, n2 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-      ++count;
-      const int m = std::min(count, k);
-      for (int i = 0; i < m; ++i) {
-        double dot = 0.0;
-        BLASC(rocblas_ddot(hb, nn, Eslot, 1, dEs.as<double>() + static_cast<size_t>(i) * n2, 1,
-                           &dot));
-        B[static_cast<size_t>(slot) * k + i] = B[static_cast<size_t>(i) * k + slot] = dot;
+                          dtmp.as<double>(), n, dX.as<double>(), n, &zero, dEc.as<double>(), n));
+      HIPC(diis_push_launch(dEc.as<double>(), dF.as<double>(), dEs.as<double>(),
+                            dFs.as<double>(), n2, st, k, dpart.as<double>(), ctx->stream));
+      HIPC(diis_solve_launch(st, dpart.as<double>(), dE.as<double>(), params(), ctx->stream));
+      HIPC(diis_extrapolate_launch(dFs.as<double>(), n2, st, dFx.as<double>(), ctx->stream));
+      return RCHEM_OK;
+    };
+    // the density of F_x by a fixed number of purification steps, committed
+    // by the device if tr(P - P^2) was below kPurifyTol two steps before the end
+    auto back_fixed = [&](int steps) -> int {
+      TRY(orthogonalise(dFx.as<double>()));
+      HIPC(purify_start_launch(dFp.as<double>(), n, opts->nocc, dpl.as<double>(),
+                               dph.as<double>(), dpar.as<double>(), dP.as<double>(), ctx->stream));
+      double* coef = dpar.as<double>() + 4;
+      for (int s = 0; s < steps; ++s) TRY(purify_step(coef + 2 * s));
+      TRY(density_from_p(dDn.as<double>()));
+      HIPC(scf_commit_launch(dD.as<double>(), dDn.as<double>(), n2, st, coef, steps, kPurifyTol,
+                             opts->damping, ctx->stream));
+      return RCHEM_OK;
+    };
+    // the density of F_x on the host's schedule (adaptive purification or the
+    // eigensolver), committed unconditionally
+    auto back_eager = [&](int* steps) -> int {
+      TRY(density(dFx.as<double>(), dDn.as<double>(), steps));
+      HIPC(scf_commit_launch(dD.as<double>(), dDn.as<double>(), n2, st, nullptr, 0, kPurifyTol,
+                             opts->damping, ctx->stream));
+      return RCHEM_OK;
+    };
+    auto read_status = [&]() -> int {
+      HIPC(hipMemcpyAsync(hst, st, offsetof(ScfDevState, e_prev), hipMemcpyDeviceToHost,
+                          ctx->stream));
+      HIPC(hipStreamSynchronize(ctx->stream));
+      return RCHEM_OK;
+    };
+    // whole-iteration graph: purified densities, graphs on, not profiling
+    bool whole = opts->graph != 0 && opts->purify != 0 && !ctx->profiling && seen_steps > 0;
+    fock_graph.enabled = opts->graph != 0 && !whole;
+    hipGraph_t ig = nullptr;
+    hipGraphExec_t ix = nullptr;
+    // The graph holds a fixed number of purification steps: what the last
+    // eager density or replay needed plus kStepMargin; a replay that needs more
+    // raises purify_failed (its density is then formed eagerly and the graph
+    // recaptured longer), one that needs kStepSlack fewer is recaptured shorter.
+    constexpr int kStepMargin = 3, kStepSlack = 4;
+    int ig_steps = 0;
+    bool planned = false;  // a build ran at this precision (host planning done)
+    auto drop_graph = [&](bool replan) {
+      if (ix) (void)hipGraphExecDestroy(ix);
+      if (ig) (void)hipGraphDestroy(ig);
+      ix = nullptr;
+      ig = nullptr;
+      if (replan) planned = false;
+    };
+    auto iteration_body = [&](int steps) -> int {
+      TRY(jk_device(ctx, dD.as<double>(), dJ.as<double>(), dK.as<double>(), &jk));
+      HIPC(fock_launch(dH.as<double>(), dJ.as<double>(), dK.as<double>(), dF.as<double>(), n2,
+                       ctx->stream));
+      TRY(front());
+      return back_fixed(steps);
+    };
+    int rc_loop = RCHEM_OK;
+    for (; it < opts->max_iter; ++it) {
+      bool replayed = false;
+      if (whole && !ix && planned) {
+        // capture (nothing runs), then replay; a refused capture falls back to
+        // launching kernel by kernel for the rest of the SCF
+        ig_steps = std::min(std::max(seen_steps + kStepMargin, 8), kPurifyMax);
+        HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+        const int rc = iteration_body(ig_steps);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+        hipError_t ei = hipErrorUnknown;
+        if (rc == RCHEM_OK && ec == hipSuccess && g) ei = hipGraphInstantiateWithFlags(&ix, g, 0);
+        if (ei == hipSuccess) {
+          ig = g;
+        } else {
+          if (g) (void)hipGraphDestroy(g);
+          ix = nullptr;
+          (void)hipGetLastError();
+          whole = false;
+          fock_graph.enabled = opts->graph != 0;
+        }
       }
-      rocblas_int imax = 0;
-      BLASC(rocblas_idamax(hb, nn, Eslot, 1, &imax));
-      double errmax = 0.0;
-      HIPC(hipMemcpy(&errmax, Eslot + (imax - 1), sizeof(double), hipMemcpyDeviceToHost));
-      errmax = std::fabs(errmax);
-      const double de_ = e_cur - e_prev;
-      if (opts->verbose) std::printf("%4d %20.12f %20.12f %12.3e\n", it, e_cur, de_, errmax);
+      if (whole && ix) {
+        HIPC(hipGraphLaunch(ix, ctx->stream));
+        replayed = true;
+        ++res->graph_iterations;
+      } else {
+        TRY(fock_graph.run(ctx, jk, dD.as<double>(), dJ.as<double>(), dK.as<double>(),
+                           dH.as<double>(), dF.as<double>(), n2));
+        planned = true;
+        TRY(front());
+        TRY(read_status());
+        if (!hst->switched) {
+          int steps = 0;
+          TRY(back_eager(&steps));
+          if (steps > 0) seen_steps = steps;
+          if (opts->purify && steps == 0 && whole) {
+            whole = false;  // no gap: the eigensolver's turn, the Fock build alone replayed
+            fock_graph.enabled = opts->graph != 0;
+          }
+        }
+      }
+      TRY(read_status());
+      if (replayed && hst->purify_failed) {
+        // more steps than the graph holds (or no gap): this iteration's
+        // density eagerly, then a recapture with the steps it needed
+        int steps = 0;
+        TRY(back_eager(&steps));
+        TRY(read_status());
+        drop_graph(false);
+        if (steps > 0) {
+          seen_steps = steps;
+        } else {
+          whole = false;
+          fock_graph.enabled = opts->graph != 0;
+        }
+      } else if (replayed && hst->purify_steps + kStepMargin + kStepSlack <= ig_steps) {
+        seen_steps = hst->purify_steps;
+        drop_graph(false);
+      }
+      const double e_cur = hst->e_cur;
+      if (!std::isfinite(e_cur)) {
+        rc_loop = ctx->fail(RCHEM_ERR_NONFINITE, "non-finite SCF energy");
+        break;
+      }
+      if (opts->verbose) std::printf("%4d %20.12f %20.12f %12.3e\n", it, e_cur, hst->de, hst->errmax);
       e_new = e_cur;
-      if (it > 0 && maybe_switch(de_)) {
-        e_prev = e_cur;
-        continue;  // the next Fock matrix is all-FP64; keep the DIIS history
+      if (hst->switched) {
+        // the next Fock matrix is all-FP64 (D kept, the DIIS history too)
+        jk.precision = RCHEM_PREC_FP64;
+        res->fp64_from = it + 1;
+        drop_graph(true);
+        continue;
       }
-      if (it > 0 && jk.precision == RCHEM_PREC_FP64 && std::fabs(de_) < opts->thresh_e &&
-          errmax < errtol) {
+      if (hst->converged) {
+        // D is already the density of this iteration's F (no extrapolation)
         if (opts->verbose) std::printf("Convergence achieved!\n");
         res->converged = 1;
         ++it;
-        TRY(density(dF.as<double>(), true));  // C, eps and D of the converged Fock matrix
         break;
       }
-      e_prev = e_cur;
-      // coefficients: [B 1; 1 0] (c, -lambda) = (0, 1), Gaussian elimination
-      const int sz = m + 1;
-      std::vector<double> A(static_cast<size_t>(sz) * sz, 0.0), rhs(sz, 0.0);
-      for (int i = 0; i < m; ++i) {
-        for (int j = 0; j < m; ++j) A[i * sz + j] = B[static_cast<size_t>(i) * k + j];
-        A[i * sz + m] = A[m * sz + i] = 1.0;
-      }
-      rhs[m] = 1.0;
-      bool ok = true;
-      for (int c = 0; c < sz && ok; ++c) {
-        int piv = c;
-        for (int r = c + 1; r < sz; ++r)
-          if (std::fabs(A[r * sz + c]) > std::fabs(A[piv * sz + c])) piv = r;
-        if (std::fabs(A[piv * sz + c]) < 1e-300) {
-          ok = false;
-          break;
-        }
-        if (piv != c) {
-          for (int j = 0; j < sz; ++j) std::swap(A[c * sz + j], A[piv * sz + j]);
-          std::swap(rhs[c], rhs[piv]);
-        }
-        for (int r = 0; r < sz; ++r) {
-          if (r == c) continue;
-          const double f = A[r * sz + c] / A[c * sz + c];
-          for (int j = c; j < sz; ++j) A[r * sz + j] -= f * A[c * sz + j];
-          rhs[r] -= f * rhs[c];
-        }
-      }
-      if (ok) {
-        HIPC(fill_launch(dFx.as<double>(), n2, 0.0, ctx->stream));
-        for (int i = 0; i < m; ++i) {
-          const double ci = rhs[i] / A[i * sz + i];
-          BLASC(rocblas_daxpy(hb, nn, &ci, dFs.as<double>() + static_cast<size_t>(i) * n2, 1,
-                              dFx.as<double>(), 1));
-        }
-        TRY(save_old());
-        TRY(density(dFx.as<double>(), false));
-        TRY(damp(errmax));
-      } else {
-        count = 0;  // singular subspace: restart from the current Fock matrix
-        TRY(save_old());
-        TRY(density(dF.as<double>(), false));
-        TRY(damp(errmax));
-      }
     }
+    drop_graph(true);
+    if (rc_loop != RCHEM_OK) return rc_loop;
   }
   double e_prev_plain = e_new - 1.0;  // |dE| of the previous iteration (damping measure)
+  fock_graph.enabled = opts->graph != 0;
   for (; opts->diis <= 0 && it < opts->max_iter; ++it) {
     TRY(fock_graph.run(ctx, jk, dD.as<double>(), dJ.as<double>(), dK.as<double>(),
                        dH.as<double>(), dF.as<double>(), n2));
     if (dconv > 0.0 || opts->damping > 0.0) TRY(save_old());
-    TRY(density(dF.as<double>(), false));
+    TRY(density(dF.as<double>(), dD.as<double>(), nullptr));
     TRY(damp(std::fabs(e_new - e_prev_plain)));
     const double e_old = e_new;
     TRY(energy(dF.as<double>(), &e_new));
@@ -2540,13 +2639,15 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
                      hipMemcpyDeviceToHost));
       ddmax = std::fabs(ddmax);
     }
-    if (maybe_switch(de_)) continue;
+    if (jk.precision == RCHEM_PREC_MIXED && std::fabs(de_) < kSwitch) {
+      jk.precision = RCHEM_PREC_FP64;
+      res->fp64_from = it + 1;
+      continue;
+    }
     if (std::fabs(de_) < opts->thresh_e && ddmax <= dconv) {
       if (opts->verbose) std::printf("Convergence achieved!\n");
       res->converged = 1;
       ++it;
-      // orbitals and orbital energies (the gradient's W) from the eigensolver
-      if (opts->purify) TRY(density(dF.as<double>(), true));
       break;
     }
   }
@@ -3037,23 +3138,24 @@ int rchem_rhf_gradient(rchem_ctx* ctx, int natom, const double* Z, const double*
   if (!grad) return ctx->fail(RCHEM_ERR_INVALID, "null gradient");
   if (natom <= 0 || !Z || !xyz) return ctx->fail(RCHEM_ERR_INVALID, "bad nuclei");
   const int rc = scf_run(ctx, natom, Z, xyz, opts, res, 1e-9);
-  // The gradient formula holds at a stationary density only, and W needs the
-  // orbitals and orbital energies of the converged Fock matrix (with purified
-  // densities the eigensolver runs only on convergence): without convergence
-  // no gradient is formed and grad is left as the caller passed it.
+  // The gradient formula holds at a stationary density only: without
+  // convergence no gradient is formed and grad is left as the caller passed it.
   if (rc != RCHEM_OK) return rc;
   const int n = ctx->nbf, ns = static_cast<int>(ctx->shells.size());
   const size_t n2 = static_cast<size_t>(n) * n;
-  // energy-weighted density W = C_occ diag(eps_occ) C_occ^T (column-major C)
-  DevBuf dCe, dW, dgrad, dpa, dsat, dsh, dex, dcc, dpr, dax, daz;
-  HIPC(dCe.ensure(n2 * sizeof(double)));
+  // energy-weighted density W = C_occ diag(eps_occ) C_occ^T = D F D: the SCF
+  // leaves D = C_occ C_occ^T, the density of F (F C = S C eps, C^T S C = 1,
+  // so C_occ^T F C_occ = diag(eps_occ)); no orbitals needed
+  DevBuf dDF, dW, dgrad, dpa, dsat, dsh, dex, dcc, dpr, dax, daz;
+  HIPC(dDF.ensure(n2 * sizeof(double)));
   HIPC(dW.ensure(n2 * sizeof(double)));
   const double one = 1.0, zero = 0.0;
-  BLASC(rocblas_ddgmm(ctx->blas, rocblas_side_right, n, opts->nocc, ctx->scf_C.as<double>(), n,
-                      ctx->scf_w.as<double>(), 1, dCe.as<double>(), n));
-  BLASC(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose, n, n,
-                      opts->nocc, &one, dCe.as<double>(), n, ctx->scf_C.as<double>(), n, &zero,
-                      dW.as<double>(), n));
+  BLASC(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
+                      ctx->scf_D.as<double>(), n, ctx->scf_F.as<double>(), n, &zero,
+                      dDF.as<double>(), n));
+  BLASC(rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
+                      dDF.as<double>(), n, ctx->scf_D.as<double>(), n, &zero, dW.as<double>(),
+                      n));
   // shell -> nucleus
   std::vector<int> satom(ns, -1);
   for (int s = 0; s < ns; ++s)

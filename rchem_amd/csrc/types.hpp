@@ -207,4 +207,38 @@ struct PlanBuffers {
   size_t scan_bytes;
 };
 
+// Device-side state of one DIIS SCF (Pulay extrapolation over a ring of at most
+// kDiisMax Fock matrices and their orthogonalised commutators), updated by the
+// SCF step's kernels so that a whole iteration can be replayed as one graph.
+// The leading scalars are what the host reads back after an iteration.
+constexpr int kDiisMax = 32;
+struct ScfDevState {
+  double e_cur;       // sum D o (H + F(D)) of the density this iteration's F was built from
+  double de;          // e_cur - e_prev
+  double errmax;      // max |X^T (FDS - SDF) X|
+  int converged;      // |de| < thresh_e, errmax < errtol, all-FP64 build, not the first iteration
+  int switched;       // mixed build switches to all-FP64: D kept, nothing extrapolated
+  int singular;       // DIIS system singular: history restarted, next D from F
+  int purify_failed;  // fixed-step purification ended above its tolerance: D kept
+  int it;             // iterations done
+  int count;          // commutators pushed since the last restart
+  int m;              // subspace size of this iteration
+  int slot;           // ring slot of this iteration's push
+  int damp;           // this iteration's density is damped
+  int purify_steps;   // steps the fixed-step purification needed (first below tolerance + 3)
+  double e_prev;
+  double c[kDiisMax];               // extrapolation coefficients (ring order)
+  double B[kDiisMax * kDiisMax];    // commutator dot products (ring order)
+};
+struct ScfDevParams {
+  double thresh_e, errtol;     // convergence: |dE| and max |commutator|
+  double switch_de;            // mixed -> all-FP64 once |dE| falls below this
+  double damping, damping_until;
+  int k;                       // ring length (<= kDiisMax)
+  int mixed;                   // the build being replayed is the mixed one
+  int fp64;                    // the build being replayed is the all-FP64 one
+  int pad;
+};
+constexpr int kDiisBlocks = 240;  // fixed grid of the dot-product pass: fixed summation order
+
 }  // namespace rchem
