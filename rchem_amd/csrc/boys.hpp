@@ -77,7 +77,7 @@ __device__ __forceinline__ void boys_exact(R T, const R* __restrict__ table, R* 
   constexpr int K = sizeof(R) == 8 ? kBoysTerms : kBoysTermsF;
   if (T < R(kBoysTMax)) {
     const int k = static_cast<int>(T * R(kBoysInvH) + R(0.5));
-    const R d = R(k) * (R(1) / R(kBoysInvH)) - T;  // T0 - T, |d| <= 1/4
+    const R d = R(k) * (R(1) / R(kBoysInvH)) - T;  // T0 - T, |d| <= 1/16
     const R* __restrict__ c = table + k * kBoysTerms;
     if constexpr (M == 1) {
       // F_0(T0 - d) = sum_j c_j d^j (c_j = F_j(T0) / j!); F_1 = d/dd of it
