@@ -3,9 +3,13 @@
 // for any class on request, and the derivative integrals of the nuclear
 // gradient.
 //
-// One wavefront (a 64-thread workgroup) evaluates one contracted shell
-// quartet at a time, P primitive quartets per batch (P a power of two, the
-// class's table value or less for short contractions):
+// One wavefront evaluates one contracted shell quartet at a time, P primitive
+// quartets per batch (P a power of two, at most the class's table value, less
+// for short contractions).  A workgroup holds W = 1, 2 or 4 such waves, each on
+// its own quartets with its own share of LDS; with W > 1 (or on request) the
+// VRR operation and accumulation tables are staged in LDS once per workgroup,
+// so the operation stream is read from LDS instead of L2, and the waves
+// synchronise with wave-level barriers only.  Per wave:
 //   1. lanes 0..P-1 set up the batch's primitive-quartet records (exponent
 //      factors, PA, WP, QC, WQ, 2 alpha of three centres and
 //      G_m = K_ab K_cd (zeta+eta)^(-1/2) F_m(T), m = 0..L) field-major in LDS;
@@ -14,7 +18,7 @@
 //      q = l mod P and takes operations l / P, l / P + 64 / P, ... of a group,
 //      a group being operations of one dependency level and one (side, axis)
 //      whose recurrence coefficients the lane loads once; a level boundary is a
-//      workgroup barrier (one wave);
+//      barrier of the wave;
 //   3. the accumulation list adds each target [e0|f0]^(0), summed over the
 //      batch with its set's weight (1 or 2 alpha of a centre), into the
 //      contracted accumulators, one lane per accumulator;
@@ -30,6 +34,7 @@
 #include <algorithm>
 
 #include "boys.hpp"
+#include "eri_coop.hpp"
 #include "types.hpp"
 
 namespace rchem {
@@ -40,19 +45,8 @@ namespace {
 constexpr int kPA = 0, kWP = 3, kQC = 6, kWQ = 9, kOo2z = 12, kOo2e = 13, kOo2ze = 14,
               kRoz = 15, kRoe = 16, kWA = 17, kWB = 18, kWC = 19, kG = 20;
 
-__device__ __forceinline__ int rec_fields(int L) { return kG + L + 1; }
+__host__ __device__ __forceinline__ int rec_fields(int L) { return kG + L + 1; }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  return v;
-}
 
 // pref * F_m(T), m = 0..M, into out[m * stride]: the exact Boys function of
 // boys_exact_rt (Taylor row of F_M on the T0 grid, exp row, downward
@@ -143,14 +137,15 @@ __device__ void setup_record(const EriArgs& a, const PairRec& bp, const PairRec&
 }
 
 // One contracted quartet (bra pair bidx, ket pair kidx): the HRR results end in
-// H = R; the outputs are read by the epilogue.
-template <int BM>
-__device__ void quartet(const EriArgs& a, const GenericProgram& p, int bidx, int kidx,
+// H = R; the outputs are read by the epilogue.  vops, vgroups, accs: the
+// program's VRR and accumulation tables (device memory or LDS).
+template <int BM, int W>
+__device__ void quartet(const EriArgs& a, const GenericProgram& p, const uint4* __restrict__ vops,
+                        const uint4* __restrict__ vgroups, const uint2* __restrict__ accs,
+                        const CoopTables& tbl, const PairRec& bp, const PairRec& kp,
                         double* __restrict__ rec, double* __restrict__ R, int lane) {
-  const PairRec bp = a.pairs[bidx];
-  const PairRec kp = a.pairs[kidx];
   const int L = p.L;
-  __syncthreads();  // the previous quartet's outputs have been read
+  wsync<W>();  // the previous quartet's outputs have been read
   double* __restrict__ acc = R;
   for (int k = lane; k < p.nacc; k += 64) acc[k] = 0.0;
   const int nk = kp.np, npq = bp.np * kp.np;
@@ -161,188 +156,123 @@ __device__ void quartet(const EriArgs& a, const GenericProgram& p, int bidx, int
   }
   double* __restrict__ V = R + p.nacc;
   const int q = lane & (P - 1);
+  double* __restrict__ Vq = V + q;  // slot s of this lane's primitive quartet: Vq[s << LP]
   const int ostep = 64 >> LP;
   for (int b0 = 0; b0 < npq; b0 += P) {
-    __syncthreads();  // the previous batch's records and slots are consumed
+    wsync<W>();  // the previous batch's records and slots are consumed
     if (lane < P) {
       const int j = b0 + lane;
       const bool valid = j < npq;
       setup_record<BM>(a, bp, kp, valid ? j / nk : 0, valid ? j % nk : 0, valid, L, P, lane, rec);
     }
-    __syncthreads();
+    wsync<W>();
     for (int idx = lane; idx < (L + 2) * P; idx += 64) {
       const int s = idx >> LP, qq = idx & (P - 1);
-      V[idx] = s == 0 ? 0.0 : rec[(kG + s - 1) * P + qq];
+      V[idx] = s == 0 ? 0.0 : rec[((kG + s - 1) << LP) + qq];
     }
-    __syncthreads();
+    wsync<W>();
     for (int g = 0; g < p.nvgroups; ++g) {
-      const uint4 gr = p.vgroups[g];
+      const uint4 gr = vgroups[g];
       const int side = gr.z & 1, ax = (gr.z >> 1) & 3;
-      const double c0 = rec[((side ? kQC : kPA) + ax) * P + q];
-      const double c1 = rec[((side ? kWQ : kWP) + ax) * P + q];
-      const double o = rec[(side ? kOo2e : kOo2z) * P + q];
-      const double r = rec[(side ? kRoe : kRoz) * P + q];
-      const double oz = rec[kOo2ze * P + q];
-      for (int op = static_cast<int>(gr.x) + (lane >> LP); op < static_cast<int>(gr.y);
-           op += ostep) {
-        const uint4 w = p.vops[op];
+      const double c0 = rec[(((side ? kQC : kPA) + ax) << LP) + q];
+      const double c1 = rec[(((side ? kWQ : kWP) + ax) << LP) + q];
+      const double o = rec[((side ? kOo2e : kOo2z) << LP) + q];
+      const double r = rec[((side ? kRoe : kRoz) << LP) + q];
+      const double oz = rec[(kOo2ze << LP) + q];
+      // the operations of a group are independent (one dependency level):
+      // two per iteration, both operation words and all ten operands in
+      // flight before either result is formed
+      auto vrr_op = [&](const uint4 w) __attribute__((always_inline)) {
         const double n1 = static_cast<double>(w.w & 15u);
         const double n2 = static_cast<double>((w.w >> 4) & 15u);
-        double v = c0 * V[(w.x >> 16) * P + q] + c1 * V[(w.y & 0xffffu) * P + q];
-        v += (n1 * o) * (V[(w.y >> 16) * P + q] - r * V[(w.z & 0xffffu) * P + q]);
-        v += (n2 * oz) * V[(w.z >> 16) * P + q];
-        V[(w.x & 0xffffu) * P + q] = v;
+        double v = c0 * Vq[(w.x >> 16) << LP] + c1 * Vq[(w.y & 0xffffu) << LP];
+        v += (n1 * o) * (Vq[(w.y >> 16) << LP] - r * Vq[(w.z & 0xffffu) << LP]);
+        v += (n2 * oz) * Vq[(w.z >> 16) << LP];
+        return v;
+      };
+      for (int op = static_cast<int>(gr.x) + (lane >> LP); op < static_cast<int>(gr.y);
+           op += 2 * ostep) {
+        const int op2 = op + ostep;
+        const bool two = op2 < static_cast<int>(gr.y);
+        const uint4 w = vops[op];
+        const uint4 w2 = vops[two ? op2 : op];
+        const double v = vrr_op(w);
+        const double v2 = vrr_op(w2);
+        Vq[(w.x & 0xffffu) << LP] = v;
+        if (two) Vq[(w2.x & 0xffffu) << LP] = v2;
       }
-      if (gr.z & 8u) __syncthreads();
+      if (gr.z & 8u) wsync<W>();
     }
     for (int t = lane; t < p.naccs; t += 64) {
-      const uint2 e = p.accs[t];
-      const double* src = V + (e.x & 0xffffu) * P;
+      const uint2 e = accs[t];
+      const double* src = V + ((e.x & 0xffffu) << LP);
       const int set = static_cast<int>(e.x >> 16);
       double s = 0.0;
       if (set == 0) {
         for (int qq = 0; qq < P; ++qq) s += src[qq];
       } else {
-        const double* wt = rec + (kWA + set - 1) * P;
+        const double* wt = rec + ((kWA + set - 1) << LP);
         for (int qq = 0; qq < P; ++qq) s += wt[qq] * src[qq];
       }
       acc[e.y] += s;
     }
   }
-  __syncthreads();
-  // horizontal recurrence on the contracted accumulators
-  double* __restrict__ H = R;
-  for (int g = 0; g < p.nhgroups; ++g) {
-    const uint4 gr = p.hgroups[g];
-    const int side = gr.z & 1, ax = (gr.z >> 1) & 3;
-    const double xb = ax == 0 ? bp.AB[0] : (ax == 1 ? bp.AB[1] : bp.AB[2]);
-    const double xk = ax == 0 ? kp.AB[0] : (ax == 1 ? kp.AB[1] : kp.AB[2]);
-    const double X = side ? xk : xb;
-    for (int op = static_cast<int>(gr.x) + lane; op < static_cast<int>(gr.y); op += 64) {
-      const uint2 w = p.hops[op];
-      H[w.x & 0xffffu] = H[w.x >> 16] + X * H[w.y];
-    }
-    if (gr.z & 8u) __syncthreads();
-  }
-  __syncthreads();
+  wsync<W>();
+  coop_hrr<W>(tbl, p.nhgroups, R, bp.AB[0], bp.AB[1], bp.AB[2], kp.AB[0], kp.AB[1], kp.AB[2],
+              lane);
 }
 
-__device__ __forceinline__ int ncart_rt(int l) { return (l + 1) * (l + 2) / 2; }
 
-__device__ __forceinline__ double output(const GenericProgram& p, const double* __restrict__ H,
-                                         int k) {
-  const int4 o = p.outs[k];
-  return p.out_norm[k] * (H[o.x] - static_cast<double>(o.z) * H[o.y]);
+// Byte size of the VRR and accumulation tables when staged in LDS.
+__host__ __device__ inline size_t table_bytes(const GenericProgram& p) {
+  return ((16 * static_cast<size_t>(p.nvops) + 16 * static_cast<size_t>(p.nvgroups) +
+           8 * static_cast<size_t>(p.naccs)) + 15) & ~static_cast<size_t>(15);
 }
 
-// 8-fold weight deg/8 of a unique quartet of pairs (bidx, kidx)
-__device__ __forceinline__ double sym_weight(const PairRec& bp, const PairRec& kp, int bidx,
-                                             int kidx) {
-  double s = 0.125;
-  if (bp.flags & 1) s *= 2.0;
-  if (kp.flags & 1) s *= 2.0;
-  if (bidx != kidx) s *= 2.0;
-  return s;
+// LDS doubles of one wave for program p at batch size P.
+__host__ __device__ inline size_t wave_doubles(const GenericProgram& p, int P) {
+  const size_t nf = static_cast<size_t>(rec_fields(p.L));
+  const size_t buf = std::max(static_cast<size_t>(p.nacc) + static_cast<size_t>(p.nv) * P,
+                              static_cast<size_t>(p.nh));
+  return static_cast<size_t>(P) * nf + buf + kCoopDBlock;
 }
 
-template <int MODE>
-__device__ void epilogue(const EriArgs& a, const GenericProgram& p, const Tile& t, int bidx,
-                         int kidx, const double* __restrict__ H, int lane) {
-  const PairRec bp = a.pairs[bidx];
-  const PairRec kp = a.pairs[kidx];
-  const int NA = ncart_rt(p.la), NB = ncart_rt(p.lb), NC = ncart_rt(p.lc), ND = ncart_rt(p.ld);
-  const size_t n = static_cast<size_t>(a.nbf);
-  const int fa = bp.fa, fb = bp.fb, fc = kp.fa, fd = kp.fb;
-  if constexpr (MODE == kModeQuartet) {
-    for (int k = lane; k < p.nout; k += 64)
-      a.out[static_cast<size_t>(t.pad) + k] = output(p, H, k);
-  } else if constexpr (MODE == kModeDiag) {
-    double m = 0.0;
-    for (int k = lane; k < NA * NB; k += 64) {
-      const int ia = k / NB, ib = k % NB;
-      m = fmax(m, fabs(output(p, H, ((ia * NB + ib) * NC + ia) * ND + ib)));
-    }
-    m = wave_max(m);
-    if (lane == 0) a.out[kidx] = sqrt(m);
-  } else if constexpr (MODE == kModeTensor) {
-    for (int k = lane; k < p.nout; k += 64) {
-      const double v = output(p, H, k);
-      const int id = k % ND, ic = (k / ND) % NC, ib = (k / (ND * NC)) % NB, ia = k / (ND * NC * NB);
-      const size_t P = fa + ia, Q = fb + ib, Rr = fc + ic, S = fd + id;
-      double* I = a.out;
-      I[((P * n + Q) * n + Rr) * n + S] = v;
-      I[((Q * n + P) * n + Rr) * n + S] = v;
-      I[((P * n + Q) * n + S) * n + Rr] = v;
-      I[((Q * n + P) * n + S) * n + Rr] = v;
-      I[((Rr * n + S) * n + P) * n + Q] = v;
-      I[((S * n + Rr) * n + P) * n + Q] = v;
-      I[((Rr * n + S) * n + Q) * n + P] = v;
-      I[((S * n + Rr) * n + Q) * n + P] = v;
-    }
-  } else if constexpr (MODE == kModeJK) {
-    // the lower-half accumulators get deg/8 of every permutation's
-    // contribution; J + J^T, K + K^T restore the rest
-    const double s = sym_weight(bp, kp, bidx, kidx);
-    const double* __restrict__ D = a.D;
-    for (int k = lane; k < p.nout; k += 64) {
-      const double v = s * output(p, H, k);
-      const int id = k % ND, ic = (k / ND) % NC, ib = (k / (ND * NC)) % NB, ia = k / (ND * NC * NB);
-      const size_t A = fa + ia, B = fb + ib, C = fc + ic, Dd = fd + id;
-      unsafeAtomicAdd(&a.J[A * n + B], 2.0 * v * D[C * n + Dd]);
-      unsafeAtomicAdd(&a.J[C * n + Dd], 2.0 * v * D[A * n + B]);
-      unsafeAtomicAdd(&a.K[A * n + C], v * D[B * n + Dd]);
-      unsafeAtomicAdd(&a.K[A * n + Dd], v * D[B * n + C]);
-      unsafeAtomicAdd(&a.K[B * n + C], v * D[A * n + Dd]);
-      unsafeAtomicAdd(&a.K[B * n + Dd], v * D[A * n + C]);
-    }
-  } else {  // kModeGrad
-    // dE/dX = sum over ordered quartets of Gamma d(mu nu|la si)/dX with
-    // Gamma = 2 D_mn D_ls - (D_ml D_ns + D_ms D_nl) / 2; a unique quartet
-    // stands for deg = 8 s ordered ones.  Outputs: (centre A, B, C) x (x, y, z)
-    // x components; centre D by translational invariance.
-    const int nc = NA * NB * NC * ND;
-    const double deg = 8.0 * sym_weight(bp, kp, bidx, kidx);
-    const double* __restrict__ D = a.D;
-    // a per-centre program holds that centre's 3 x nc outputs: map them onto
-    // rows 3 centre .. 3 centre + 2 of g, the others stay zero
-    const int c0 = p.centre < 0 ? 0 : 3 * p.centre, nrow = p.centre < 0 ? 9 : 3;
-    double g[9];
-#pragma unroll
-    for (int c = 0; c < 9; ++c) g[c] = 0.0;
-    for (int k = lane; k < nc; k += 64) {
-      const int id = k % ND, ic = (k / ND) % NC, ib = (k / (ND * NC)) % NB, ia = k / (ND * NC * NB);
-      const size_t A = fa + ia, B = fb + ib, C = fc + ic, Dd = fd + id;
-      const double gam = 2.0 * D[A * n + B] * D[C * n + Dd] -
-                         0.5 * (D[A * n + C] * D[B * n + Dd] + D[A * n + Dd] * D[B * n + C]);
-#pragma unroll
-      for (int c = 0; c < 9; ++c)
-        if (c >= c0 && c < c0 + nrow) g[c] += gam * output(p, H, (c - c0) * nc + k);
-    }
-    // six accumulators per pair (its two atoms' x, y, z), added onto the
-    // atoms once every class is done: per-atom atomics would contend
-    double* __restrict__ gb = a.grad_pair + 6 * static_cast<size_t>(bidx);
-    double* __restrict__ gk = a.grad_pair + 6 * static_cast<size_t>(kidx);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      const double gA = deg * wave_sum(g[i]);
-      const double gB = deg * wave_sum(g[3 + i]);
-      const double gC = deg * wave_sum(g[6 + i]);
-      if (lane == 0) {
-        unsafeAtomicAdd(&gb[i], gA);
-        unsafeAtomicAdd(&gb[3 + i], gB);
-        unsafeAtomicAdd(&gk[i], gC);
-        unsafeAtomicAdd(&gk[3 + i], -(gA + gB + gC));
-      }
-    }
-  }
-}
-
-template <int MODE, int BM>
-__global__ __launch_bounds__(64) void eri_generic_kernel(const EriArgs a, const GenericProgram p) {
+// W waves per workgroup, each on its own quartets; TL: VRR and accumulation
+// tables in LDS, and with stage_coop the HRR and output tables too.
+template <int MODE, int BM, int W, bool TL>
+__global__ __launch_bounds__(64 * W) void eri_generic_kernel(const EriArgs a,
+                                                             const GenericProgram p,
+                                                             int stage_coop) {
   extern __shared__ double lds[];
-  const int lane = threadIdx.x;
-  double* rec = lds;
-  double* R = lds + p.P * rec_fields(p.L);
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const uint4* vops = p.vops;
+  const uint4* vgroups = p.vgroups;
+  const uint2* accs = p.accs;
+  size_t tb = 0;
+  if constexpr (TL) {
+    uint4* tv = reinterpret_cast<uint4*>(lds);
+    uint4* tg = tv + p.nvops;
+    uint2* ta = reinterpret_cast<uint2*>(tg + p.nvgroups);
+    for (int i = threadIdx.x; i < p.nvops; i += 64 * W) tv[i] = p.vops[i];
+    for (int i = threadIdx.x; i < p.nvgroups; i += 64 * W) tg[i] = p.vgroups[i];
+    for (int i = threadIdx.x; i < p.naccs; i += 64 * W) ta[i] = p.accs[i];
+    __syncthreads();
+    vops = tv;
+    vgroups = tg;
+    accs = ta;
+    tb = table_bytes(p);
+  }
+  const bool sc = TL && stage_coop != 0;
+  const CoopTables tbl = coop_tables(p, reinterpret_cast<char*>(lds) + tb, sc);
+  if (sc) {
+    tb += coop_table_bytes(p);
+    __syncthreads();
+  }
+  double* rec = reinterpret_cast<double*>(reinterpret_cast<char*>(lds) + tb) +
+                static_cast<size_t>(wave) * wave_doubles(p, p.P);
+  double* R = rec + p.P * rec_fields(p.L);
+  double* dl = rec + wave_doubles(p, p.P) - kCoopDBlock;
   int begin, end;
   if (a.tile_range) {
     begin = a.tile_range[0];
@@ -351,67 +281,120 @@ __global__ __launch_bounds__(64) void eri_generic_kernel(const EriArgs a, const 
     begin = 0;
     end = a.ntiles;
   }
-  // every workgroup leaves once its next tile index passes the end
-  for (int ti = begin + static_cast<int>(blockIdx.x); ti < end; ti += gridDim.x) {
+  // every wave leaves once its next tile index passes the end; the waves of a
+  // workgroup share nothing after the table load, so they may leave apart
+  for (int ti = begin + static_cast<int>(blockIdx.x) * W + wave; ti < end;
+       ti += static_cast<int>(gridDim.x) * W) {
     const Tile t = a.tiles[ti];
     for (int i = 0; i < t.count; ++i) {
       const int kidx = t.ket0 + i;
       const int bidx = (MODE == kModeDiag) ? kidx : t.bra;
-      quartet<BM>(a, p, bidx, kidx, rec, R, lane);
-      epilogue<MODE>(a, p, t, bidx, kidx, R, lane);
+      const PairRec bp = a.pairs[bidx];
+      const PairRec kp = a.pairs[kidx];
+      quartet<BM, W>(a, p, vops, vgroups, accs, tbl, bp, kp, rec, R, lane);
+      epilogue<W, MODE>(a, p, tbl, t, bidx, kidx, bp, kp, R, dl, lane);
     }
   }
 }
 
-template <int MODE, int BM>
-hipError_t launch_mode(const EriArgs& a, const GenericProgram& p, unsigned grid, size_t lds,
-                       hipStream_t s) {
-  hipLaunchKernelGGL((eri_generic_kernel<MODE, BM>), dim3(grid), dim3(64), lds, s, a, p);
+template <int MODE, int BM, int W, bool TL>
+hipError_t launch_shape(const EriArgs& a, const GenericProgram& p, int sc, unsigned grid,
+                        size_t lds, hipStream_t s) {
+  hipLaunchKernelGGL((eri_generic_kernel<MODE, BM, W, TL>), dim3(grid), dim3(64 * W), lds, s, a,
+                     p, sc);
   return hipGetLastError();
+}
+
+template <int MODE, int BM>
+hipError_t launch_mode(const EriArgs& a, const GenericProgram& p, int W, bool tl, int sc,
+                       unsigned grid, size_t lds, hipStream_t s) {
+  if (W == 4) return launch_shape<MODE, BM, 4, true>(a, p, sc, grid, lds, s);
+  if (W == 2) return launch_shape<MODE, BM, 2, true>(a, p, sc, grid, lds, s);
+  return tl ? launch_shape<MODE, BM, 1, true>(a, p, sc, grid, lds, s)
+            : launch_shape<MODE, BM, 1, false>(a, p, 0, grid, lds, s);
 }
 
 }  // namespace
 
-// LDS bytes of one workgroup for program p.
+// LDS bytes of one one-wave workgroup for program p with its tables in
+// device memory (the smallest footprint: whether the class fits a workgroup).
 size_t eri_generic_lds_bytes(const GenericProgram& p) {
-  const size_t nf = 20 + p.L + 1;
-  const size_t buf = std::max(static_cast<size_t>(p.nacc) + static_cast<size_t>(p.nv) * p.P,
-                              static_cast<size_t>(p.nh));
-  return sizeof(double) * (static_cast<size_t>(p.P) * nf + buf);
+  return sizeof(double) * wave_doubles(p, p.P);
 }
 
+// Launch shapes (the tuning table's variant for a class on this kernel):
+// {waves per workgroup, tables in LDS, batch divisor}; shape 0 picks the
+// largest W whose workgroup fits the device's LDS with the tables staged.
+struct GenericShape {
+  int waves, tables_in_lds, pdiv;
+};
+constexpr GenericShape kGenericShapes[] = {{0, 1, 1}, {1, 0, 1}, {1, 1, 1}, {2, 1, 1},
+                                           {4, 1, 1}, {2, 1, 2}, {4, 1, 2}, {4, 1, 4}};
+constexpr int kNumGenericShapes = sizeof(kGenericShapes) / sizeof(kGenericShapes[0]);
+
+int eri_generic_shapes() { return kNumGenericShapes; }
+
 // blocks: with a device tile range, the number of 4-wave blocks the per-lane
-// kernels would use (this kernel launches 4 one-wave workgroups per block);
-// without one, one workgroup per tile.
-hipError_t eri_generic_launch(const EriArgs& a, const GenericProgram& p, int mode, int bm,
-                              int blocks, hipStream_t s, size_t lds_limit) {
-  unsigned grid;
+// kernels would use (this kernel launches as many waves, W per workgroup);
+// without one, one wave per tile.
+hipError_t eri_generic_launch(const EriArgs& a, const GenericProgram& prog, int mode, int bm,
+                              int blocks, hipStream_t s, size_t lds_limit, int shape) {
+  GenericProgram p = prog;
+  if (p.P < 1 || p.P > 64 || (p.P & (p.P - 1)) != 0 || p.L > kBoysMaxM) return hipErrorInvalidValue;
+  if (shape < 0 || shape >= kNumGenericShapes) shape = 0;
+  GenericShape sh = kGenericShapes[shape];
+  auto bytes = [&](int W, bool tl, int P) {
+    return (tl ? table_bytes(p) : 0) + sizeof(double) * W * wave_doubles(p, P);
+  };
+  if (sh.waves == 0) {
+    sh = {1, 0, 1};
+    for (int W : {4, 2, 1})
+      if (bytes(W, true, p.P) <= lds_limit) {
+        sh = {W, 1, 1};
+        break;
+      }
+  }
+  int P = std::max(1, p.P / sh.pdiv);
+  if (bytes(sh.waves, sh.tables_in_lds != 0, P) > lds_limit) {  // shape too big: one wave
+    sh = {1, 0, 1};
+    P = p.P;
+  }
+  p.P = P;
+  size_t lds = bytes(sh.waves, sh.tables_in_lds != 0, P);
+  if (lds > lds_limit) return hipErrorInvalidValue;
+  // the HRR and output tables too, where they fit beside the rest
+  int sc = 0;
+  if (sh.tables_in_lds && lds + coop_table_bytes(p) <= lds_limit) {
+    sc = 1;
+    lds += coop_table_bytes(p);
+  }
+  unsigned waves;
   if (a.tile_range) {
     if (blocks <= 0) return hipSuccess;
-    grid = static_cast<unsigned>(blocks) * 4u;
+    waves = static_cast<unsigned>(blocks) * 4u;
   } else {
     if (a.ntiles <= 0) return hipSuccess;
-    grid = static_cast<unsigned>(a.ntiles);
+    waves = static_cast<unsigned>(a.ntiles);
   }
-  if (p.P < 1 || p.P > 64 || (p.P & (p.P - 1)) != 0 || p.L > kBoysMaxM) return hipErrorInvalidValue;
-  const size_t lds = eri_generic_lds_bytes(p);
-  if (lds > lds_limit) return hipErrorInvalidValue;
+  const unsigned grid = (waves + sh.waves - 1) / sh.waves;
+  const bool tl = sh.tables_in_lds != 0;
+  const int W = sh.waves;
   const bool compat = bm == kBoysCompat;
   switch (mode) {
     case kModeJK:
-      return compat ? launch_mode<kModeJK, kBoysCompat>(a, p, grid, lds, s)
-                    : launch_mode<kModeJK, kBoysExact>(a, p, grid, lds, s);
+      return compat ? launch_mode<kModeJK, kBoysCompat>(a, p, W, tl, sc, grid, lds, s)
+                    : launch_mode<kModeJK, kBoysExact>(a, p, W, tl, sc, grid, lds, s);
     case kModeDiag:
-      return compat ? launch_mode<kModeDiag, kBoysCompat>(a, p, grid, lds, s)
-                    : launch_mode<kModeDiag, kBoysExact>(a, p, grid, lds, s);
+      return compat ? launch_mode<kModeDiag, kBoysCompat>(a, p, W, tl, sc, grid, lds, s)
+                    : launch_mode<kModeDiag, kBoysExact>(a, p, W, tl, sc, grid, lds, s);
     case kModeTensor:
-      return compat ? launch_mode<kModeTensor, kBoysCompat>(a, p, grid, lds, s)
-                    : launch_mode<kModeTensor, kBoysExact>(a, p, grid, lds, s);
+      return compat ? launch_mode<kModeTensor, kBoysCompat>(a, p, W, tl, sc, grid, lds, s)
+                    : launch_mode<kModeTensor, kBoysExact>(a, p, W, tl, sc, grid, lds, s);
     case kModeQuartet:
-      return compat ? launch_mode<kModeQuartet, kBoysCompat>(a, p, grid, lds, s)
-                    : launch_mode<kModeQuartet, kBoysExact>(a, p, grid, lds, s);
+      return compat ? launch_mode<kModeQuartet, kBoysCompat>(a, p, W, tl, sc, grid, lds, s)
+                    : launch_mode<kModeQuartet, kBoysExact>(a, p, W, tl, sc, grid, lds, s);
     case kModeGrad:
-      return launch_mode<kModeGrad, kBoysExact>(a, p, grid, lds, s);
+      return launch_mode<kModeGrad, kBoysExact>(a, p, W, tl, sc, grid, lds, s);
     default:
       return hipErrorInvalidValue;
   }

@@ -227,6 +227,32 @@ def gen_class(la, lb, lc, ld):
                 vrr_flops=vrr_flops, hrr_flops=hflops[0])
 
 
+def stream_vrr(la, lb, lc, ld):
+    """The VRR of the streaming kernel (eri_stream.hpp): one lane holds one
+    primitive quartet and hands each target [e0|f0]^(0) to acc.put<k>() as
+    soon as it is formed, k being its accumulator index (the wave-cooperative
+    program's layout, ie * len(F) + jf), in increasing k: the lane keeps no
+    accumulators, only the VRR values still needed (~50-300 doubles for
+    L = 5..8)."""
+    E = [t for l in range(la, la + lb + 1) for t in cart(l)]
+    F = [t for l in range(lc, lc + ld + 1) for t in cart(l)]
+    em = Emitter()
+    v = VRR(em)
+    body = []
+    k = 0
+    for e in E:
+        for f in F:
+            n0 = len(em.lines)
+            name = v.node(e, f, 0)
+            body.extend(em.lines[n0:])
+            body.append(f"acc.template put<{k}>({name});")
+            k += 1
+    return list(em.consts.values()) + body
+
+
+STREAM_L = 5  # classes from this total angular momentum get a streaming kernel
+
+
 def pair_class(l1, l2):
     return l1 * (l1 + 1) // 2 + l2
 
@@ -287,6 +313,21 @@ def tu_source(la, lb, lc, ld):
         L += ["  (void)bm;", "  (void)fp32;", "  (void)v;", "  return nullptr;"]
     L += ["}", "", "}  // namespace rchem", ""]
     return "\n".join(L)
+
+
+def stream_tu_source(la, lb, lc, ld):
+    """Streaming-kernel launchers of one class (L >= STREAM_L)."""
+    A = f"{la}, {lb}, {lc}, {ld}"
+    key = f"{la}{lb}{lc}{ld}"
+    return "\n".join([
+        "// Generated by gen_eri.py -- do not edit.", '#include "../eri_stream.hpp"', "",
+        "namespace rchem {", "",
+        f"EriStreamFn eri_stream_launcher_c{key}(int mode) {{",
+        f"  if (mode == kModeJK) return eri_stream_launch<{A}, kModeJK>;",
+        f"  if (mode == kModeDiag) return eri_stream_launch<{A}, kModeDiag>;",
+        f"  if (mode == kModeTensor) return eri_stream_launch<{A}, kModeTensor>;",
+        f"  if (mode == kModeQuartet) return eri_stream_launch<{A}, kModeQuartet>;",
+        "  return nullptr;", "}", "", "}  // namespace rchem", ""])
 
 
 def write_programs(path, lmax):
@@ -365,6 +406,15 @@ def main():
         parts.append("    (void)oo2z; (void)oo2e; (void)oo2ze; (void)roz; (void)roe;")
         parts.extend("    " + s for s in g["vrr"])
         parts.append("  }")
+        if g["L"] >= STREAM_L:
+            parts.append("  template <typename SA>")
+            parts.append("  __device__ static __forceinline__ void vrr_stream(const double* __restrict__ G,"
+                         " double PAx, double PAy, double PAz, double WPx, double WPy, double WPz,")
+            parts.append("      double QCx, double QCy, double QCz, double WQx, double WQy, double WQz,"
+                         " double oo2z, double oo2e, double oo2ze, double roz, double roe, SA& acc) {")
+            parts.append("    using R = double;")
+            parts.extend("    " + x for x in stream_vrr(la, lb, lc, ld))
+            parts.append("  }")
         parts.append("  template <typename R>")
         parts.append("  __device__ static __forceinline__ void hrr(const R* __restrict__ acc,"
                      " R ABx, R ABy, R ABz, R CDx, R CDy, R CDz, R* __restrict__ out) {")
@@ -379,6 +429,13 @@ def main():
         if old != tu:  # keep timestamps of unchanged units
             with open(path, "w") as fh:
                 fh.write(tu)
+        if g["L"] >= STREAM_L:
+            su = stream_tu_source(la, lb, lc, ld)
+            path = os.path.join(out_dir, f"eri_s_{key}.hip")
+            old = open(path).read() if os.path.exists(path) else None
+            if old != su:
+                with open(path, "w") as fh:
+                    fh.write(su)
     with open(os.path.join(out_dir, "eri_classes.inc"), "w") as fh:
         fh.write("\n".join(parts))
     write_programs(os.path.join(out_dir, "eri_programs.inc"), lmax)
@@ -400,6 +457,14 @@ def main():
     for k in keys:
         disp.append(f"  if (la == {k[0]} && lb == {k[1]} && lc == {k[2]} && ld == {k[3]}) "
                     f"return eri_jk_lds_launcher_c{k}(bm, fp32, v);")
+    disp += ["  return nullptr;", "}", ""]
+    skeys = [f"{a}{b}{c}{d}" for a, b, c, d in classes(lmax) if a + b + c + d >= STREAM_L]
+    disp += [f"EriStreamFn eri_stream_launcher_c{k}(int mode);" for k in skeys]
+    disp += ["", "// Streaming kernel of the class in `mode` (exact Boys); nullptr if none.",
+             "EriStreamFn eri_stream_launcher(int la, int lb, int lc, int ld, int mode) {"]
+    for k in skeys:
+        disp.append(f"  if (la == {k[0]} && lb == {k[1]} && lc == {k[2]} && ld == {k[3]}) "
+                    f"return eri_stream_launcher_c{k}(mode);")
     disp += ["  return nullptr;", "}", ""]
     with open(os.path.join(out_dir, "eri_dispatch.inc"), "w") as fh:
         fh.write("\n".join(disp))

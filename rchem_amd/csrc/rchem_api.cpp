@@ -35,9 +35,10 @@ hipError_t dmax_launch(const double* D, size_t n2, unsigned long long* out, hipS
 size_t plan_scan_bytes(int ntasks);
 hipError_t plan_launch(const PlanArgs& a, const PlanBuffers& b, hipStream_t s);
 EriLaunchFn eri_jk_lds_launcher(int la, int lb, int lc, int ld, int bm, int fp32, int v);
+EriStreamFn eri_stream_launcher(int la, int lb, int lc, int ld, int mode);
 hipError_t eri_generic_launch(const EriArgs& a, const GenericProgram& p, int mode, int bm,
-                              int blocks, hipStream_t s,
-                              size_t lds_limit);
+                              int blocks, hipStream_t s, size_t lds_limit, int shape);
+int eri_generic_shapes();
 
 size_t eri_generic_lds_bytes(const GenericProgram& p);
 
@@ -346,6 +347,7 @@ struct rchem_ctx {
   bool profiling = false;
   bool lds_digestion = true;  // RCHEM_LDS_DIGESTION=0 selects global-atomic digestion
   bool force_generic = false;  // RCHEM_ERI_GENERIC=1: every class on the wave-cooperative kernel
+  int eri_stream_mode = 1;     // streaming kernel: 0 never, 1 where faster, 2 wherever it serves
   // wave-cooperative kernel: operation tables in device memory (uploaded on first use)
   std::vector<GenericProgram> gprog;
   DevBuf gprog_d;
@@ -705,12 +707,16 @@ int layout_pairs(rchem_ctx* ctx, const std::vector<PairHost>& sel, double prim_e
 // FP32 tile list of a mixed build).
 struct ClassKernel {
   EriLaunchFn fn = nullptr;
+  EriStreamFn sfn = nullptr;  // streaming kernel (uses gp's HRR and output tables)
   const GenericProgram* gp = nullptr;
   int mode = 0, bm = 0;
   size_t lds_limit = 65536;  // the device's LDS per workgroup (wave-cooperative kernel)
+  int shape = 0;             // wave-cooperative kernel's launch shape (0: chosen from the LDS)
   bool valid() const { return fn || gp; }
   hipError_t operator()(const EriArgs& a, int blocks, hipStream_t s) const {
-    return fn ? fn(a, blocks, s) : eri_generic_launch(a, *gp, mode, bm, blocks, s, lds_limit);
+    if (fn) return fn(a, blocks, s);
+    if (sfn) return sfn(a, *gp, blocks, s, lds_limit);
+    return eri_generic_launch(a, *gp, mode, bm, blocks, s, lds_limit, shape);
   }
 };
 
@@ -740,7 +746,9 @@ int upload_programs(rchem_ctx* ctx) {
     GenericProgram& g = progs[i];
     g.la = h.la; g.lb = h.lb; g.lc = h.lc; g.ld = h.ld;
     g.L = h.L; g.nacc = h.nacc; g.nv = h.nv; g.nh = h.nh; g.nout = h.nout;
-    g.nvgroups = h.nvgroups; g.nhgroups = h.nhgroups; g.naccs = h.naccs; g.P = h.P;
+    g.nvops = h.nvops; g.nvgroups = h.nvgroups; g.nhops = h.nhops; g.nhgroups = h.nhgroups;
+    g.naccs = h.naccs;
+    g.P = h.P;
     g.centre = h.grad >= 2 ? h.grad - 2 : -1;
     g.vops = static_cast<const uint4*>(put(h.vops, h.nvops, 16));
     g.vgroups = static_cast<const uint4*>(put(h.vgroups, h.nvgroups, 16));
@@ -780,11 +788,32 @@ int class_kernel(rchem_ctx* ctx, int la, int lb, int lc, int ld, int mode, int b
   out->mode = mode;
   out->bm = bm;
   out->lds_limit = ctx->lds_block;
+  if (la + lb + lc + ld > kBoysMaxM) return RCHEM_OK;
   if (!ctx->force_generic) {
+    // the streaming kernel (no per-lane accumulators; it takes the HRR and
+    // output tables from the class's wave-cooperative program) where it is the
+    // faster engine, measured on the 300-shell s/p/d microbenchmark (ms per
+    // FP64 build, streaming vs per-lane or wave-cooperative): (dp|dp) 138 vs
+    // 246, (dd|pp) 78 vs 150, (dd|dp) 84 vs 470, (dd|dd) 37 vs 113; the
+    // per-lane kernels keep (dp|pp) 125 vs 174, (dp|ds) 52 vs 142, (dd|ps) 17
+    // vs 72, (dd|ds) 53 vs 62.  RCHEM_ERI_STREAM=0: never, 2: every class it
+    // serves.
+    const int key = ((la * 10 + lb) * 10 + lc) * 10 + ld;
+    const bool prefer = ctx->eri_stream_mode == 2 ||
+                        (ctx->eri_stream_mode == 1 &&
+                         (key == 2121 || key == 2211 || key == 2221 || key == 2222));
+    const EriStreamFn sfn =
+        prefer && bm == kBoysExact ? eri_stream_launcher(la, lb, lc, ld, mode) : nullptr;
+    if (sfn) {
+      TRY(find_program(ctx, la, lb, lc, ld, 0, &out->gp));
+      if (out->gp) {
+        out->sfn = sfn;
+        return RCHEM_OK;
+      }
+    }
     out->fn = eri_launcher(la, lb, lc, ld, mode, bm, fp32);
     if (out->fn) return RCHEM_OK;
   }
-  if (la + lb + lc + ld > kBoysMaxM) return RCHEM_OK;
   return find_program(ctx, la, lb, lc, ld, 0, &out->gp);
 }
 
@@ -1476,6 +1505,7 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
             return ctx->fail(RCHEM_ERR_UNSUPPORTED, "ERI class " +
                                                         class_name(qc.la, qc.lb, qc.lc, qc.ld) +
                                                         " not served by this library");
+          fn.shape = tn.variant;  // the class's launch shape on the wave-cooperative kernel
           a.tile_range = ranges + (prec ? 2 : 0);
           TRY(launch_timed(ctx, qc, prec, fn, a, std::min(tn.blocks, (cap + 3) / 4), timed_ok));
         }
@@ -1582,6 +1612,7 @@ int rchem_ctx_create(int device, rchem_ctx** out) {
   }
   if (const char* e = std::getenv("RCHEM_LDS_DIGESTION")) ctx->lds_digestion = std::atoi(e) != 0;
   if (const char* e = std::getenv("RCHEM_ERI_GENERIC")) ctx->force_generic = std::atoi(e) != 0;
+  if (const char* e = std::getenv("RCHEM_ERI_STREAM")) ctx->eri_stream_mode = std::atoi(e);
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
     g_create_error = "stream creation failed";
     delete ctx;
@@ -3206,7 +3237,7 @@ int rchem_rhf_gradient(rchem_ctx* ctx, int natom, const double* Z, const double*
     const GenericProgram* gp = nullptr;
     TRY(find_program(ctx, hc.la, hc.lb, hc.lc, hc.ld, 1, &gp));
     if (gp) {
-      HIPC(eri_generic_launch(a, *gp, kModeGrad, kBoysExact, 0, ctx->stream, ctx->lds_block));
+      HIPC(eri_generic_launch(a, *gp, kModeGrad, kBoysExact, 0, ctx->stream, ctx->lds_block, 0));
       continue;
     }
     // one program per centre
@@ -3216,7 +3247,7 @@ int rchem_rhf_gradient(rchem_ctx* ctx, int natom, const double* Z, const double*
         return ctx->fail(RCHEM_ERR_UNSUPPORTED, "gradient of ERI class " +
                                                     class_name(hc.la, hc.lb, hc.lc, hc.ld) +
                                                     " not served by this library");
-      HIPC(eri_generic_launch(a, *gp, kModeGrad, kBoysExact, 0, ctx->stream, ctx->lds_block));
+      HIPC(eri_generic_launch(a, *gp, kModeGrad, kBoysExact, 0, ctx->stream, ctx->lds_block, 0));
     }
   }
   HIPC(grad_pairs_launch(dgp.as<double>(), dpa.as<int2>(), npairs, dgrad.as<double>(),

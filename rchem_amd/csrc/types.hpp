@@ -128,10 +128,11 @@ struct EriArgs {
 
 // A class served by the wave-cooperative kernel: its recurrences as operation
 // tables in device memory (layouts in gen_eri_prog.py).  One wavefront
-// evaluates one contracted quartet; its LDS holds the records of a batch of P
-// primitive quartets and one buffer of max(nacc + nv P, nh) doubles that is
-// the contracted accumulators + VRR slots during the primitive loop and the
-// HRR workspace afterwards.
+// evaluates one contracted quartet; its share of LDS holds the records of a
+// batch of P primitive quartets and one buffer of max(nacc + nv P, nh) doubles
+// that is the contracted accumulators + VRR slots during the primitive loop
+// and the HRR workspace afterwards.  A workgroup of several waves (each on its
+// own quartets) can stage the VRR and accumulation tables in LDS once.
 struct GenericProgram {
   int la, lb, lc, ld;
   int L;          // highest Boys order (la + lb + lc + ld, + 1 for the gradient)
@@ -139,7 +140,7 @@ struct GenericProgram {
   int nv;         // VRR slots per primitive quartet (slot 0 = zero, 1..L+1 = G_m)
   int nh;         // HRR slots (the first nacc are the accumulators)
   int nout;       // outputs (integrals, or 9 x integrals for the gradient)
-  int nvgroups, nhgroups, naccs;
+  int nvops, nvgroups, nhops, nhgroups, naccs;
   int P;          // primitive quartets per batch (power of two)
   int centre;     // gradient program of one centre (0 A, 1 B, 2 C; outputs 3 x components),
                   // -1: all three (outputs 9 x components) or not a gradient program
@@ -151,6 +152,11 @@ struct GenericProgram {
   const int4* outs;       // (plus slot, minus slot, n, 0): norm (H[plus] - n H[minus])
   const double* out_norm;
 };
+
+// Launcher of a class's streaming kernel (eri_stream.hpp) in one mode: the
+// class's wave-cooperative program supplies the HRR and output tables.
+using EriStreamFn = hipError_t (*)(const EriArgs&, const GenericProgram&, int blocks,
+                                   hipStream_t, size_t lds_limit);
 
 // One quartet class as the device planner sees it.
 struct PlanClass {

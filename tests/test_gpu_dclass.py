@@ -7,11 +7,17 @@
 * J and K of a basis with d shells on several centres from the direct build
   against J, K contracted from the oracle's ERI tensor (THO, exact Boys),
   JK_inmem (basis.rs:462-484).  Bar 1e-11 absolute.
+* The same J and K with every class from total angular momentum 5 on the
+  streaming kernel (RCHEM_ERI_STREAM=2: one lane per primitive quartet,
+  targets summed over the wave) and with none on it (RCHEM_ERI_STREAM=0:
+  per-lane and wave-cooperative kernels), and the Schwarz bounds of both.
 * The analytic gradient with d shells on two atoms, so (dd|dp) and (dd|dd)
   quartets span two centres and their per-centre derivative programs cannot
   cancel on one atom, against central differences of the GPU energy.
   Bar 1e-6 Eh/bohr.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -71,6 +77,46 @@ def test_dbasis_jk_vs_oracle_tensor(gpu_ctx):
     assert np.max(np.abs(J - Jo)) < 1e-11
     assert np.max(np.abs(K - Ko)) < 1e-11
     assert np.max(np.abs(Jo)) > 1e-2
+
+
+def _ctx_with(var, value):
+    os.environ[var] = value
+    try:
+        return rb.Context(0)
+    finally:
+        del os.environ[var]
+
+
+@pytest.mark.parametrize("stream", ["0", "2"])
+def test_dbasis_jk_engines_vs_oracle_tensor(stream):
+    ctx = _ctx_with("RCHEM_ERI_STREAM", stream)
+    try:
+        sh = _dbasis(nshell=8, seed=11, box=3.5)
+        B = rb.Basis.from_shells(*sh, ctx=ctx)
+        n = B.nbf
+        _say(f"case: d-basis J/K vs oracle tensor, RCHEM_ERI_STREAM={stream}, nbf={n}")
+        rng = np.random.default_rng(17)
+        X = rng.standard_normal((n, n))
+        D = 0.1 * (X + X.T)
+        J = np.zeros((n, n)); K = np.zeros((n, n))
+        rb.JK_direct(J, K, B, D, default_opts(boys_mode=BOYS_EXACT, schwarz_thresh=1e-14))
+        Bo = O.cgtos_from_shells(*sh)
+        Jo, Ko = O.jk_inmem(O.eri_tensor(Bo, boys=O.BOYS_EXACT), D)
+        assert np.max(np.abs(J - Jo)) < 1e-11
+        assert np.max(np.abs(K - Ko)) < 1e-11
+        assert np.max(np.abs(Jo)) > 1e-2
+        # the Schwarz bounds sqrt(max |(ab|ab)|) of the diagonal mode
+        Q = rb.schwarz(B)
+        c, l, npr, ex, cf = sh
+        f0 = np.concatenate([[0], np.cumsum((l + 1) * (l + 2) // 2)])
+        I = O.eri_tensor(Bo, boys=O.BOYS_EXACT)
+        for a in range(len(l)):
+            for b in range(a + 1):
+                blk = I[f0[a]:f0[a + 1], f0[b]:f0[b + 1], f0[a]:f0[a + 1], f0[b]:f0[b + 1]]
+                q = np.sqrt(np.max(np.abs(np.einsum("ijij->ij", blk))))
+                assert abs(Q[a, b] - q) < 1e-12 * max(1.0, q)
+    finally:
+        ctx.close()
 
 
 def test_dshell_gradient_two_centres_vs_fd(gpu_ctx):
