@@ -74,6 +74,22 @@ def _sample_rate(O, B, D, addr, seconds, threads):
     return done, t_total
 
 
+def jk_rates(Bo, D, seconds):
+    """rchem's JK_direct loop (basis.rs:383-428) over the reference's cints
+    coulomb_repulsion (oracle/_ref; the C restatement where that is not
+    built): function quartets/s on one core and on every core, each from a
+    bounded random sample."""
+    import oracle as O
+    addr = O.cints_coulomb_repulsion_address()
+    cores = host_cores()
+    d1, t1 = _sample_rate(O, Bo, D, addr, seconds, 1)
+    dm, tm = _sample_rate(O, Bo, D, addr, seconds, cores)
+    return {"kind": "reference" if addr else "port",
+            "src": "the reference cints.c coulomb_repulsion" if addr else "the C restatement",
+            "rate_1": d1 / t1, "done_1": d1, "t_1": t1,
+            "rate_all": dm / tm, "done_all": dm, "t_all": tm, "cores": cores}
+
+
 def cpu_baseline(Z, xyz, D, gpu_quartets, seconds, water_gpu_scf_s):
     """The reference algorithm on the host: rchem's JK_direct loop (basis.rs:383-428)
     over the reference's own cints coulomb_repulsion (oracle/_ref) -- on one core
@@ -195,6 +211,35 @@ def batch_workload(args, ctx, t_start):
         if any(r.converged for r in r64) else None,
         "total_functions": int(sum(nbf)),
     }
+    if not args.no_cpu_baseline:
+        # rchem solves one molecule at a time: per molecule, its SCF's Fock
+        # builds (the iterations the GPU run took) of n^4 function quartets at
+        # the reference loop's rate, sampled on a few of the molecules
+        import oracle as O
+        bse = O.load_bse_json(os.path.join(ROOT, "rchem_amd", "data", "sto-3g.json"))
+        rates, rates_all, done, secs = [], [], 0, 0.0
+        pick = np.random.default_rng(512).choice(len(mols), 4, replace=False)
+        for i in pick:
+            Zm, xm = mols[i]
+            Bo = O.cgtos_from_bse(bse, Zm, np.asarray(xm, dtype=float).reshape(-1, 3))
+            Dm = np.eye(Bo.n) * 0.5
+            r = jk_rates(Bo, Dm, args.cpu_seconds / 4)
+            rates.append(r["rate_1"])
+            rates_all.append(r["rate_all"])
+            done += r["done_1"]
+            secs += r["t_1"]
+        rate, rate_all = float(np.mean(rates)), float(np.mean(rates_all))
+        quartets = sum(res_i.iterations * float(nb) ** 4 for res_i, nb in zip(rmx, nbf))
+        out["cpu_baseline"] = {
+            "value": len(mols) / (quartets / rate), "unit": "molecules/s", "cores": 1,
+            "kind": r["kind"],
+            "sample": (f"{done} random function quartets of the JK_direct loop on molecules "
+                       f"{sorted(pick.tolist())} in {secs:.1f} s (mean {rate:.0f}/s via "
+                       f"{r['src']}); the batch's Fock builds (each molecule's GPU iteration "
+                       f"count x n^4) = {quartets:.3g} function quartets"),
+            "batch_s_extrapolated": quartets / rate,
+            "all_cores": {"value": len(mols) / (quartets / rate_all), "cores": r["cores"],
+                          "batch_s_extrapolated": quartets / rate_all}}
     print(json.dumps(out), flush=True)
 
 
@@ -224,7 +269,7 @@ def gradient_workload(args, ctx, t_start):
     energy(xyz)  # warm-up (pair layout, programs, kernels)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res, _ = rb.scf(B, xyz, Z, nocc=nocc, thresh_e=1e-11, diis=8, opts=o)
+    res, D_gpu = rb.scf(B, xyz, Z, nocc=nocc, thresh_e=1e-11, diis=8, opts=o)
     t_scf = time.perf_counter() - t0
     say(f"forward SCF {res.iterations} iterations, {t_scf:.3f} s")
     t0 = time.perf_counter()
@@ -270,6 +315,26 @@ def gradient_workload(args, ctx, t_start):
         "max_abs_fd_error": worst,
         "net_force": float(np.max(np.abs(g.sum(axis=0)))),
     }
+    if not args.no_cpu_baseline:
+        # rchem has no gradient: the reference CPU path to one is central
+        # differences of its SCF energy, 2 x 288 SCFs of JK_direct builds
+        import oracle as O
+        bse = O.load_bse_json(os.path.join(ROOT, "rchem_amd", "data", "6-31g.json"))
+        Bo = O.cgtos_from_bse(bse, Z, xyz)
+        r = jk_rates(Bo, D_gpu, args.cpu_seconds)
+        n4 = float(Bo.n) ** 4
+        scf_s = res.iterations * n4 / r["rate_1"]
+        out["cpu_baseline"] = {
+            "value": 2 * ncoord * scf_s, "unit": "s (gradient by central differences)",
+            "cores": 1, "kind": r["kind"],
+            "sample": (f"{r['done_1']} random function quartets of the (H2O)_32 6-31G JK_direct "
+                       f"loop in {r['t_1']:.1f} s ({r['rate_1']:.0f}/s via {r['src']}); one SCF "
+                       f"= {res.iterations} builds = {scf_s:.3g} s, a gradient = 2 x {ncoord} "
+                       f"SCFs"),
+            "forward_scf_s_extrapolated": scf_s,
+            "all_cores": {"value": 2 * ncoord * res.iterations * n4 / r["rate_all"],
+                          "cores": r["cores"],
+                          "forward_scf_s_extrapolated": res.iterations * n4 / r["rate_all"]}}
     print(json.dumps(out), flush=True)
 
 
@@ -478,6 +543,22 @@ def main():
     if args.workload == "eri2000":
         out["metric"] = "ERI shell quartets/s per (la lb|lc ld) class, synthetic 2000-shell microbench"
         out["roofline"]["traffic"] = None
+        if not args.no_cpu_baseline:
+            import oracle as O
+            r = jk_rates(O.cgtos_from_shells(*sh), D, args.cpu_seconds)
+            n4 = float(n) ** 4  # JK_direct visits every (mu, nu, la, si): no screening
+            out["cpu_baseline"] = {
+                "value": st.quartets / (n4 / r["rate_1"]),
+                "unit": "shell quartets/s (same Fock build)", "cores": 1, "kind": r["kind"],
+                "sample": (f"{r['done_1']} random (mu,nu,la,si) function quartets of this "
+                           f"basis's JK_direct loop in {r['t_1']:.1f} s ({r['rate_1']:.0f}/s, "
+                           f"2 contracted ERIs each via {r['src']}); extrapolated Fock build "
+                           f"{n4 / r['rate_1']:.3g} s"),
+                "fock_build_s_extrapolated": n4 / r["rate_1"],
+                "all_cores": {"value": st.quartets / (n4 / r["rate_all"]), "cores": r["cores"],
+                              "fock_build_s_extrapolated": n4 / r["rate_all"],
+                              "sample": (f"{r['done_all']} random function quartets in "
+                                         f"{r['t_all']:.1f} s on {r['cores']} threads")}}
     if args.workload == "h2o64" and not args.no_scf:
         scf = {}
         for p in ("fp64", "mixed"):

@@ -4,6 +4,8 @@
 // and instantiate only the variants each class needs.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "types.hpp"
 
 namespace rchem {
@@ -62,6 +64,32 @@ __global__ void symmetrize_kernel(const double* __restrict__ Jh, const double* _
   K[ij] = Kh[ij] + Kh[ji];
 }
 
+// Block-diagonal versions for a multi-molecule basis (quartets within a
+// molecule only, so J and K are block-diagonal): blockIdx.y = molecule m,
+// elements of its nb[m] x nb[m] block at rows and columns f0[m] ...
+__global__ void symmetrize_blocks_kernel(const double* __restrict__ Jh,
+                                         const double* __restrict__ Kh, double* __restrict__ J,
+                                         double* __restrict__ K, int n, const int* __restrict__ f0,
+                                         const int* __restrict__ nb) {
+  const int m = blockIdx.y, b = nb[m], o = f0[m];
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < b * b; k += gridDim.x * blockDim.x) {
+    const size_t i = o + k / b, j = o + k % b;
+    const size_t ij = i * n + j, ji = j * n + i;
+    J[ij] = Jh[ij] + Jh[ji];
+    K[ij] = Kh[ij] + Kh[ji];
+  }
+}
+
+__global__ void fill_blocks_kernel(double* __restrict__ A, double* __restrict__ B, int n,
+                                   const int* __restrict__ f0, const int* __restrict__ nb) {
+  const int m = blockIdx.y, b = nb[m], o = f0[m];
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < b * b; k += gridDim.x * blockDim.x) {
+    const size_t ij = static_cast<size_t>(o + k / b) * n + o + k % b;
+    A[ij] = 0.0;
+    B[ij] = 0.0;
+  }
+}
+
 hipError_t jp_scatter_launch(const PairRec* pairs, const double* Jp, const JpClass* cls,
                              const int* ncomp, int ncls, int npairs, double* Jh, int nbf,
                              hipStream_t s) {
@@ -85,6 +113,28 @@ hipError_t symmetrize_launch(const double* Jh, const double* Kh, double* J, doub
   if (n <= 0) return hipSuccess;
   dim3 grid((n + 255) / 256, n);
   hipLaunchKernelGGL(symmetrize_kernel, grid, dim3(256), 0, s, Jh, Kh, J, K, n);
+  return hipGetLastError();
+}
+
+// J = Jh + Jh^T, K = Kh + Kh^T on the diagonal blocks only (nmol molecules,
+// at most nmax functions each); the off-diagonal blocks of J and K are not
+// written.
+hipError_t symmetrize_blocks_launch(const double* Jh, const double* Kh, double* J, double* K,
+                                    int n, const int* f0, const int* nb, int nmol, int nmax,
+                                    hipStream_t s) {
+  if (nmol <= 0) return hipSuccess;
+  const unsigned gx = static_cast<unsigned>(std::min(64, (nmax * nmax + 255) / 256));
+  hipLaunchKernelGGL(symmetrize_blocks_kernel, dim3(gx, nmol), dim3(256), 0, s, Jh, Kh, J, K, n,
+                     f0, nb);
+  return hipGetLastError();
+}
+
+// Clears the diagonal blocks of A and B.
+hipError_t fill_blocks_launch(double* A, double* B, int n, const int* f0, const int* nb, int nmol,
+                              int nmax, hipStream_t s) {
+  if (nmol <= 0) return hipSuccess;
+  const unsigned gx = static_cast<unsigned>(std::min(64, (nmax * nmax + 255) / 256));
+  hipLaunchKernelGGL(fill_blocks_kernel, dim3(gx, nmol), dim3(256), 0, s, A, B, n, f0, nb);
   return hipGetLastError();
 }
 

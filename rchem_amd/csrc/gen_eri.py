@@ -27,6 +27,7 @@ Usage: gen_eri.py OUT_DIR [LMAX [KEY ...]]   (KEYs: classes to dispatch, default
 """
 import json
 import math
+import os
 import sys
 
 AX = "xyz"
@@ -227,29 +228,69 @@ def gen_class(la, lb, lc, ld):
                 vrr_flops=vrr_flops, hrr_flops=hflops[0])
 
 
+def _stream_body(targets, acc_index):
+    """Straight-line VRR for the given targets, each handed to acc.put<k>() as
+    soon as it is formed; returns (lines, peak number of live VRR values)."""
+    em = Emitter()
+    v = VRR(em)
+    body = []
+    for (e, f) in targets:
+        n0 = len(em.lines)
+        name = v.node(e, f, 0)
+        body.extend(em.lines[n0:])
+        body.append(f"acc.template put<{acc_index[(e, f)]}>({name});")
+    # peak live set of the emitted order (a target dies when handed over)
+    import re
+    defs = [re.match(r"const R (\w+) = (.*);", ln) for ln in body if ln.startswith("const R")]
+    last = {}
+    pos = 0
+    for ln in body:
+        if ln.startswith("acc."):
+            last[re.search(r"\((\w+)\);", ln).group(1)] = pos
+        else:
+            for d in re.findall(r"\b(v\w+)", ln.split("=", 1)[1]):
+                last[d] = pos
+            pos += 1
+    live, peak, pos = set(), 0, 0
+    for ln in body:
+        if ln.startswith("acc."):
+            continue
+        name = ln.split()[2]
+        live.add(name)
+        peak = max(peak, len(live))
+        live = {x for x in live if last.get(x, -1) > pos}
+        pos += 1
+    return list(em.consts.values()) + body, peak
+
+
 def stream_vrr(la, lb, lc, ld):
     """The VRR of the streaming kernel (eri_stream.hpp): one lane holds one
     primitive quartet and hands each target [e0|f0]^(0) to acc.put<k>() as
     soon as it is formed, k being its accumulator index (the wave-cooperative
     program's layout, ie * len(F) + jf), in increasing k: the lane keeps no
-    accumulators, only the VRR values still needed (~50-300 doubles for
-    L = 5..8)."""
+    accumulators, only the VRR values still needed.  Where that is more than
+    STREAM_LIVE values, the targets are split into parts (boundaries on the
+    16-target groups the kernel sums over the wave), each recomputing the
+    intermediates it needs: more operations, no spills.  Returns the parts'
+    bodies."""
     E = [t for l in range(la, la + lb + 1) for t in cart(l)]
     F = [t for l in range(lc, lc + ld + 1) for t in cart(l)]
-    em = Emitter()
-    v = VRR(em)
-    body = []
-    k = 0
-    for e in E:
-        for f in F:
-            n0 = len(em.lines)
-            name = v.node(e, f, 0)
-            body.extend(em.lines[n0:])
-            body.append(f"acc.template put<{k}>({name});")
-            k += 1
-    return list(em.consts.values()) + body
+    acc_index = {(e, f): ie * len(F) + jf for ie, e in enumerate(E) for jf, f in enumerate(F)}
+    targets = [(e, f) for e in E for f in F]
+    n = len(targets)
+    for parts in range(1, 9):
+        cuts = [0] + [16 * round(n * p / parts / 16) for p in range(1, parts)] + [n]
+        bodies = [_stream_body(targets[cuts[p]:cuts[p + 1]], acc_index) for p in range(parts)]
+        if max(pk for _, pk in bodies) <= STREAM_LIVE or parts == 8:
+            return [b for b, _ in bodies]
 
 
+# VRR values a streaming lane may keep live before the targets are split.  On
+# gfx950, splitting (dd|dp) into 2 parts and (dd|dd) into 3 (live sets 102
+# and 150 instead of 162 and 306 in the emitted order) left the compiler's
+# register allocation and the kernel times unchanged, so every class streams
+# in one part.
+STREAM_LIVE = 1000
 STREAM_L = 5  # classes from this total angular momentum get a streaming kernel
 
 
@@ -330,6 +371,15 @@ def stream_tu_source(la, lb, lc, ld):
         "  return nullptr;", "}", "", "}  // namespace rchem", ""])
 
 
+def write_if_changed(path, text):
+    """Write text to path unless it already holds exactly that (an unchanged
+    output keeps its timestamp, so make rebuilds only what changed)."""
+    old = open(path).read() if os.path.exists(path) else None
+    if old != text:
+        with open(path, "w") as fh:
+            fh.write(text)
+
+
 def write_programs(path, lmax):
     """Operation tables of the wave-cooperative kernel: the integrals of every
     class and the gradient's derivative integrals (grad = 1), split into one
@@ -374,8 +424,7 @@ def write_programs(path, lmax):
                 f"kProgH_{key}, kProgHG_{key}, kProgOut_{key}, kProgNorm_{key}}},")
             out.append("")
     out += ["const HostProgram kHostPrograms[] = {"] + entries + ["};", ""]
-    with open(path, "w") as fh:
-        fh.write("\n".join(out))
+    write_if_changed(path, "\n".join(out))
 
 
 def main():
@@ -407,13 +456,18 @@ def main():
         parts.extend("    " + s for s in g["vrr"])
         parts.append("  }")
         if g["L"] >= STREAM_L:
-            parts.append("  template <typename SA>")
+            bodies = stream_vrr(la, lb, lc, ld)
+            parts.append(f"  static constexpr int NSTREAM = {len(bodies)};  // parts of the streamed VRR")
+            parts.append("  template <int PART, typename SA>")
             parts.append("  __device__ static __forceinline__ void vrr_stream(const double* __restrict__ G,"
                          " double PAx, double PAy, double PAz, double WPx, double WPy, double WPz,")
             parts.append("      double QCx, double QCy, double QCz, double WQx, double WQy, double WQz,"
                          " double oo2z, double oo2e, double oo2ze, double roz, double roe, SA& acc) {")
             parts.append("    using R = double;")
-            parts.extend("    " + x for x in stream_vrr(la, lb, lc, ld))
+            for pi, body in enumerate(bodies):
+                parts.append(f"    {'if' if pi == 0 else '} else if'} constexpr (PART == {pi}) {{")
+                parts.extend("      " + x for x in body)
+            parts.append("    }")
             parts.append("  }")
         parts.append("  template <typename R>")
         parts.append("  __device__ static __forceinline__ void hrr(const R* __restrict__ acc,"
@@ -436,8 +490,7 @@ def main():
             if old != su:
                 with open(path, "w") as fh:
                     fh.write(su)
-    with open(os.path.join(out_dir, "eri_classes.inc"), "w") as fh:
-        fh.write("\n".join(parts))
+    write_if_changed(os.path.join(out_dir, "eri_classes.inc"), "\n".join(parts))
     write_programs(os.path.join(out_dir, "eri_programs.inc"), lmax)
     keys = [f"{a}{b}{c}{d}" for a, b, c, d in classes(lmax)]
     if only:
@@ -466,17 +519,16 @@ def main():
         disp.append(f"  if (la == {k[0]} && lb == {k[1]} && lc == {k[2]} && ld == {k[3]}) "
                     f"return eri_stream_launcher_c{k}(mode);")
     disp += ["  return nullptr;", "}", ""]
-    with open(os.path.join(out_dir, "eri_dispatch.inc"), "w") as fh:
-        fh.write("\n".join(disp))
-    with open(os.path.join(out_dir, "flops_per_class.json"), "w") as fh:
-        json.dump(flops, fh, indent=1, sort_keys=True)
+    write_if_changed(os.path.join(out_dir, "eri_dispatch.inc"), "\n".join(disp))
+    write_if_changed(os.path.join(out_dir, "flops_per_class.json"),
+                     json.dumps(flops, indent=1, sort_keys=True))
     # host-side copy of the operation counts: {la, lb, lc, ld, vrr/prim, hrr/contracted}
     rows = [f"  {{{k[0]}, {k[1]}, {k[2]}, {k[3]}, {v['vrr_per_primitive_quartet']}, "
             f"{v['hrr_per_contracted_quartet']}}}," for k, v in sorted(flops.items())]
-    with open(os.path.join(out_dir, "eri_flops.inc"), "w") as fh:
-        fh.write("// Generated by gen_eri.py -- do not edit.\n"
-                 "static const int kEriFlops[][6] = {\n" + "\n".join(rows) + "\n};\n"
-                 f"constexpr int kEriFlopsRows = {len(rows)};\n")
+    write_if_changed(os.path.join(out_dir, "eri_flops.inc"),
+                     "// Generated by gen_eri.py -- do not edit.\n"
+                     "static const int kEriFlops[][6] = {\n" + "\n".join(rows) + "\n};\n"
+                     f"constexpr int kEriFlopsRows = {len(rows)};\n")
 
 
 if __name__ == "__main__":

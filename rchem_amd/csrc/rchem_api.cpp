@@ -65,6 +65,11 @@ hipError_t jp_scatter_launch(const PairRec* pairs, const double* Jp, const JpCla
                              hipStream_t s);
 hipError_t symmetrize_launch(const double* Jh, const double* Kh, double* J, double* K, int n,
                              hipStream_t s);
+hipError_t symmetrize_blocks_launch(const double* Jh, const double* Kh, double* J, double* K,
+                                    int n, const int* f0, const int* nb, int nmol, int nmax,
+                                    hipStream_t s);
+hipError_t fill_blocks_launch(double* A, double* B, int n, const int* f0, const int* nb, int nmol,
+                              int nmax, hipStream_t s);
 hipError_t fill_launch(double* p, size_t n, double v, hipStream_t s);
 hipError_t fock_launch(const double* H, const double* J, const double* K, double* F, size_t n2,
                        hipStream_t s);
@@ -264,6 +269,10 @@ struct rchem_ctx {
   // Several independent molecules in one basis (batched SCF): molecule of
   // every shell; only pairs and quartets within one molecule exist.
   int nmol = 1;
+  // a multi-molecule basis: each molecule's first function and function
+  // count (device copies), the largest count; J and K are block-diagonal
+  DevBuf mol_f0_d, mol_nb_d;
+  int mol_nmax = 0;
   std::vector<int> shell_mol;                   // empty: one molecule
   std::vector<int> pair_mol;                    // layout order
   std::vector<std::vector<int2>> bucket_segs;   // [pair class][bucket * nmol + molecule]
@@ -348,6 +357,8 @@ struct rchem_ctx {
   bool lds_digestion = true;  // RCHEM_LDS_DIGESTION=0 selects global-atomic digestion
   bool force_generic = false;  // RCHEM_ERI_GENERIC=1: every class on the wave-cooperative kernel
   int eri_stream_mode = 1;     // streaming kernel: 0 never, 1 where faster, 2 wherever it serves
+  bool progress = false;       // RCHEM_PROGRESS=1: report each finished pass of a Fock build
+  bool capturing = false;      // a graph capture is recording the stream (no synchronisation)
   // wave-cooperative kernel: operation tables in device memory (uploaded on first use)
   std::vector<GenericProgram> gprog;
   DevBuf gprog_d;
@@ -1406,8 +1417,14 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
   }
   ctx->last_stats_dev = bf.stats;
   const bool timed_ok = ext == nullptr;
-  HIPC(fill_launch(bf.Jh, n2, 0.0, ctx->stream));
-  HIPC(fill_launch(bf.Kh, n2, 0.0, ctx->stream));
+  const bool blocks = ctx->nmol > 1 && ctx->mol_nmax > 0;  // block-diagonal J and K
+  if (blocks) {
+    HIPC(fill_blocks_launch(bf.Jh, bf.Kh, ctx->nbf, ctx->mol_f0_d.as<int>(), ctx->mol_nb_d.as<int>(),
+                            ctx->nmol, ctx->mol_nmax, ctx->stream));
+  } else {
+    HIPC(fill_launch(bf.Jh, n2, 0.0, ctx->stream));
+    HIPC(fill_launch(bf.Kh, n2, 0.0, ctx->stream));
+  }
   const int nc = static_cast<int>(ctx->pclasses.size());
   HIPC(fill_launch(bf.stats, 4 * static_cast<size_t>(std::max(nc, 1)), 0.0, ctx->stream));
   if (mixed) {
@@ -1511,11 +1528,23 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
         }
       }
     }
+    if (ctx->progress && !ctx->capturing) {
+      // a long build (many passes) reports each finished pass
+      HIPC(hipStreamSynchronize(ctx->stream));
+      std::fprintf(stderr, "[rchem] Fock build: pass %d of %zu done\n",
+                   static_cast<int>(&ps - ctx->passes.data()) + 1, ctx->passes.size());
+      std::fflush(stderr);
+    }
   }
   HIPC(jp_scatter_launch(ctx->pairs_d.as<PairRec>(), bf.jp, ctx->jp_cls_d.as<JpClass>(),
                          ctx->jp_ncomp_d.as<int>(), kNumPairClasses,
                          static_cast<int>(ctx->pairs.size()), bf.Jh, ctx->nbf, ctx->stream));
-  HIPC(symmetrize_launch(bf.Jh, bf.Kh, dJ, dK, ctx->nbf, ctx->stream));
+  if (blocks)
+    HIPC(symmetrize_blocks_launch(bf.Jh, bf.Kh, dJ, dK, ctx->nbf, ctx->mol_f0_d.as<int>(),
+                                  ctx->mol_nb_d.as<int>(), ctx->nmol, ctx->mol_nmax,
+                                  ctx->stream));
+  else
+    HIPC(symmetrize_launch(bf.Jh, bf.Kh, dJ, dK, ctx->nbf, ctx->stream));
   ctx->last_stats = ctx->planned;
   ctx->last_mixed = mixed;
   return RCHEM_OK;
@@ -1613,6 +1642,7 @@ int rchem_ctx_create(int device, rchem_ctx** out) {
   if (const char* e = std::getenv("RCHEM_LDS_DIGESTION")) ctx->lds_digestion = std::atoi(e) != 0;
   if (const char* e = std::getenv("RCHEM_ERI_GENERIC")) ctx->force_generic = std::atoi(e) != 0;
   if (const char* e = std::getenv("RCHEM_ERI_STREAM")) ctx->eri_stream_mode = std::atoi(e);
+  if (const char* e = std::getenv("RCHEM_PROGRESS")) ctx->progress = std::atoi(e) != 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
     g_create_error = "stream creation failed";
     delete ctx;
@@ -1670,6 +1700,7 @@ int rchem_basis_upload(rchem_ctx* ctx, int nshell, const double* xyz, const int*
   if (ctx) {
     ctx->shell_mol.clear();
     ctx->nmol = 1;
+    ctx->mol_nmax = 0;
   }
   if (!ctx) return RCHEM_ERR_INVALID;
   if (nshell <= 0 || !xyz || !l || !nprim || !exps || !coefs)
@@ -1717,6 +1748,7 @@ int rchem_basis_from_atoms(rchem_ctx* ctx, int natom, const int* Z, const double
   ctx->coefs.clear();
   ctx->shell_mol.clear();
   ctx->nmol = 1;
+  ctx->mol_nmax = 0;
   // per atom, per basis-set shell, per l in its angular_momentum list (basis.rs:186-205)
   for (int a = 0; a < natom; ++a) {
     bool found = false;
@@ -2286,9 +2318,11 @@ struct FockGraph {
       return eager();
     }
     HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+    ctx->capturing = true;
     const int rc = eager();
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+    ctx->capturing = false;
     if (rc != RCHEM_OK) {
       if (g) (void)hipGraphDestroy(g);
       return rc;
@@ -2566,9 +2600,11 @@ int scf_run(rchem_ctx* ctx, int natom, const double* Z, const double* xyz,
         // launching kernel by kernel for the rest of the SCF
         ig_steps = std::min(std::max(seen_steps + kStepMargin, 8), kPurifyMax);
         HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+        ctx->capturing = true;
         const int rc = iteration_body(ig_steps);
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+        ctx->capturing = false;
         hipError_t ei = hipErrorUnknown;
         if (rc == RCHEM_OK && ec == hipSuccess && g) ei = hipGraphInstantiateWithFlags(&ix, g, 0);
         if (ei == hipSuccess) {
@@ -2850,6 +2886,12 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
   DevBuf df0, dnb, dnocc, dSp, dX, dW, dT1, dFb, dC, dDb, dw, de, dinfo, dEm;
   HIPC(df0.ensure(sizeof(int) * B));
   HIPC(dnb.ensure(sizeof(int) * B));
+  // the Fock builds of this basis touch only the molecules' diagonal blocks
+  HIPC(ctx->mol_f0_d.ensure(sizeof(int) * B));
+  HIPC(ctx->mol_nb_d.ensure(sizeof(int) * B));
+  HIPC(hipMemcpyAsync(ctx->mol_f0_d.p, f0.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
+  HIPC(hipMemcpyAsync(ctx->mol_nb_d.p, nb.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
+  ctx->mol_nmax = nmax;
   HIPC(dnocc.ensure(sizeof(int) * B));
   for (DevBuf* x : {&dSp, &dX, &dW, &dT1, &dFb, &dC, &dDb}) HIPC(x->ensure(sizeof(double) * m2 * B));
   HIPC(dw.ensure(sizeof(double) * nmax * B));
@@ -3148,6 +3190,7 @@ int rchem_batch_scf(rchem_ctx* ctx, int nmol, const int* natom, const int* Z, co
     if (rc != RCHEM_OK && rc != RCHEM_ERR_NOT_CONVERGED) {
       ctx->shell_mol.clear();
       ctx->nmol = 1;
+      ctx->mol_nmax = 0;
       return rc;
     }
     if (rc == RCHEM_ERR_NOT_CONVERGED) status = rc;
@@ -3157,6 +3200,7 @@ int rchem_batch_scf(rchem_ctx* ctx, int nmol, const int* natom, const int* Z, co
   // the context's basis is the last chunk's; leave it as a plain basis
   ctx->shell_mol.clear();
   ctx->nmol = 1;
+  ctx->mol_nmax = 0;
   ctx->have_basis = false;
   if (status == RCHEM_ERR_NOT_CONVERGED)
     return ctx->fail(RCHEM_ERR_NOT_CONVERGED, "some molecules did not converge in max_iter");
