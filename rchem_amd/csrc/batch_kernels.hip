@@ -10,7 +10,11 @@
 //   antisym:     E_b = A_b - A_b^T;
 //   dots:        <e_b, f_b> for a list of pairs of padded blocks, one workgroup each;
 //   maxabs:      max |e_b| per molecule;
-//   combine:     out_b = sum_i c[b][i] F_i,b (DIIS extrapolation).
+//   combine:     out_b = sum_i c[b][i] F_i,b (DIIS extrapolation);
+//   blend:       block_b of D = (1 - a_b) block_b of D + a_b Dold_b (padded old copy);
+//   gather:      out_s = top-left nout x nout of in_idx[s] (drops retired molecules
+//                and shrinks the padding);
+//   fock:        F = H + 2J - K on the diagonal blocks only.
 #include <hip/hip_runtime.h>
 
 namespace rchem {
@@ -149,18 +153,47 @@ unsigned blocks_for(size_t n) {
   return static_cast<unsigned>(g ? g : 1);
 }
 
-// D <- (1 - a_b) D + a_b D_old on molecule b's diagonal block (a_b = 0: kept).
+// D <- (1 - a_b) D + a_b D_old on molecule b's diagonal block (a_b = 0: kept);
+// D_old is the padded copy (nmax x nmax per molecule, as batch_pack writes it).
 __global__ void blend_kernel(double* __restrict__ D, const double* __restrict__ Dold, size_t n,
                              const int* __restrict__ f0, const int* __restrict__ nb,
-                             const double* __restrict__ alpha) {
+                             const double* __restrict__ alpha, int nmax) {
   const int b = blockIdx.y;
   const double a = alpha[b];
   if (a == 0.0) return;
   const int nn = nb[b];
+  const double* old = Dold + static_cast<size_t>(b) * nmax * nmax;
   for (size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
        k < static_cast<size_t>(nn) * nn; k += static_cast<size_t>(gridDim.x) * blockDim.x) {
-    const size_t i = f0[b] + k / nn, j = f0[b] + k % nn;
-    D[i * n + j] = (1.0 - a) * D[i * n + j] + a * Dold[i * n + j];
+    const size_t i = k / nn, j = k % nn;
+    const size_t ij = (f0[b] + i) * n + f0[b] + j;
+    D[ij] = (1.0 - a) * D[ij] + a * old[j * nmax + i];
+  }
+}
+
+// out + r * out_stride + s * nout^2 <- top-left nout x nout of
+// in + r * in_stride + idx[s] * nin^2 (column-major blocks), r = blockIdx.z.
+__global__ void gather_kernel(const double* __restrict__ in, size_t in_stride, int nin,
+                              const int* __restrict__ idx, double* __restrict__ out,
+                              size_t out_stride, int nout) {
+  const int s = blockIdx.y, r = blockIdx.z;
+  const double* src = in + r * in_stride + static_cast<size_t>(idx[s]) * nin * nin;
+  double* dst = out + r * out_stride + static_cast<size_t>(s) * nout * nout;
+  for (size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+       k < static_cast<size_t>(nout) * nout; k += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const size_t j = k / nout, i = k % nout;
+    dst[k] = src[j * nin + i];
+  }
+}
+
+__global__ void fock_blocks_kernel(const double* __restrict__ H, const double* __restrict__ J,
+                                   const double* __restrict__ K, double* __restrict__ F, size_t n,
+                                   const int* __restrict__ f0, const int* __restrict__ nb) {
+  const int b = blockIdx.y, nn = nb[b];
+  for (size_t k = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+       k < static_cast<size_t>(nn) * nn; k += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const size_t ij = (f0[b] + k / nn) * n + f0[b] + k % nn;
+    F[ij] = H[ij] + 2.0 * J[ij] - K[ij];
   }
 }
 
@@ -168,8 +201,27 @@ __global__ void blend_kernel(double* __restrict__ D, const double* __restrict__ 
 
 hipError_t batch_blend(double* D, const double* Dold, size_t n, const int* f0, const int* nb,
                        const double* alpha, int nmax, int nbatch, hipStream_t s) {
+  if (nbatch <= 0) return hipSuccess;
   hipLaunchKernelGGL(blend_kernel, dim3(blocks_for(static_cast<size_t>(nmax) * nmax), nbatch),
-                     dim3(256), 0, s, D, Dold, n, f0, nb, alpha);
+                     dim3(256), 0, s, D, Dold, n, f0, nb, alpha, nmax);
+  return hipGetLastError();
+}
+
+hipError_t batch_gather(const double* in, size_t in_stride, int nin, const int* idx, int nout_batch,
+                        int nring, double* out, size_t out_stride, int nout, hipStream_t s) {
+  if (nout_batch <= 0 || nring <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_kernel,
+                     dim3(blocks_for(static_cast<size_t>(nout) * nout), nout_batch, nring),
+                     dim3(256), 0, s, in, in_stride, nin, idx, out, out_stride, nout);
+  return hipGetLastError();
+}
+
+hipError_t batch_fock(const double* H, const double* J, const double* K, double* F, size_t n,
+                      const int* f0, const int* nb, int nmol, int nmax, hipStream_t s) {
+  if (nmol <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fock_blocks_kernel,
+                     dim3(blocks_for(static_cast<size_t>(nmax) * nmax), nmol), dim3(256), 0, s,
+                     H, J, K, F, n, f0, nb);
   return hipGetLastError();
 }
 

@@ -33,6 +33,8 @@ using EriLaunchFn = hipError_t (*)(const EriArgs&, int, hipStream_t);
 EriLaunchFn eri_launcher(int la, int lb, int lc, int ld, int mode, int bm, int fp32);
 hipError_t dmax_launch(const double* D, size_t n2, unsigned long long* out, hipStream_t s);
 size_t plan_scan_bytes(int ntasks);
+size_t plan_sort_bytes(int n);
+int plan_class_bits(int nclasses);
 hipError_t plan_launch(const PlanArgs& a, const PlanBuffers& b, hipStream_t s);
 EriLaunchFn eri_jk_lds_launcher(int la, int lb, int lc, int ld, int bm, int fp32, int v);
 EriStreamFn eri_stream_launcher(int la, int lb, int lc, int ld, int mode);
@@ -106,6 +108,10 @@ hipError_t batch_blend(double* D, const double* Dold, size_t n, const int* f0, c
                        const double* alpha, int nmax, int nbatch, hipStream_t s);
 hipError_t batch_combine(const double* Fs, size_t batch_stride, int m, int k, const double* c,
                          int nmax, int nbatch, double* out, hipStream_t s);
+hipError_t batch_gather(const double* in, size_t in_stride, int nin, const int* idx, int nout_batch,
+                        int nring, double* out, size_t out_stride, int nout, hipStream_t s);
+hipError_t batch_fock(const double* H, const double* J, const double* K, double* F, size_t n,
+                      const int* f0, const int* nb, int nmol, int nmax, hipStream_t s);
 hipError_t scale_columns_launch(const double* U, const double* w, double* out, int n,
                                 hipStream_t s);
 
@@ -319,6 +325,7 @@ struct rchem_ctx {
   // J, K and Coulomb accumulators.  pclasses holds the segments of all passes.
   struct Pass {
     int seg_begin, nseg, tasks, units, cap;
+    int ccap;  // chunks per precision, upper bound (tiles / chunk size + units)
   };
   std::vector<PlanClass> pclasses;
   std::vector<int> seg_class;  // segment -> qclasses index
@@ -331,6 +338,11 @@ struct rchem_ctx {
       plan_off32, plan_lo, plan_mid, plan_hi, plan_tiles64, plan_tiles32, plan_ranges,
       plan_stats, plan_dmax, plan_scan_tmp, plan_chc64, plan_chc32, plan_choff64, plan_choff32,
       plan_chunks64, plan_chunks32;
+  // chunk order by ket window (RCHEM_CHUNK_ORDER=1): sort keys and buffers
+  bool chunk_order = false;
+  int plan_key_bits = 0, plan_ccap = 0;
+  size_t plan_sort_bytes = 0;
+  DevBuf plan_chunks_raw, plan_keys_raw, plan_keys_sorted, plan_sort_tmp;
   // pair-indexed Coulomb accumulators
   std::vector<JpClass> jp_cls;
   size_t jp_size = 0;
@@ -1117,14 +1129,16 @@ int prepare_plan(rchem_ctx* ctx, double tau) {
     }
   }
   // segments and passes
-  rchem_ctx::Pass cur{0, 0, 0, 0, 0};
-  double pass_cap = 0.0;
+  rchem_ctx::Pass cur{0, 0, 0, 0, 0, 0};
+  double pass_cap = 0.0, pass_ccap = 0.0;
   auto close_pass = [&]() {
     if (cur.nseg == 0) return;
     cur.cap = static_cast<int>(pass_cap) + 1;
+    cur.ccap = static_cast<int>(std::min(pass_ccap, pass_cap)) + 1;
     ctx->passes.push_back(cur);
-    cur = rchem_ctx::Pass{static_cast<int>(ctx->pclasses.size()), 0, 0, 0, 0};
+    cur = rchem_ctx::Pass{static_cast<int>(ctx->pclasses.size()), 0, 0, 0, 0, 0};
     pass_cap = 0.0;
+    pass_ccap = 0.0;
   };
   auto add_segment = [&](int c, int lo, int hi, double cap) {
     const QClass& qc = ctx->qclasses[c];
@@ -1145,6 +1159,8 @@ int prepare_plan(rchem_ctx* ctx, double tau) {
     cur.units += hi - lo;
     cur.nseg += 1;
     pass_cap += cap;
+    // a unit's chunks: ceil(tiles / chunk size) <= tiles / chunk size + 1
+    pass_ccap += cap / std::max(1, std::min(pc.chunk64, pc.chunk32)) + (hi - lo);
   };
   for (size_t c = 0; c < nq; ++c) {
     const int b0 = ctx->class_begin[ctx->qclasses[c].X];
@@ -1180,6 +1196,27 @@ int prepare_plan(rchem_ctx* ctx, double tau) {
     HIPC(b->ensure(sizeof(int) * nu));
   HIPC(ctx->plan_chunks64.ensure(sizeof(Chunk) * static_cast<size_t>(ctx->plan_cap)));
   HIPC(ctx->plan_chunks32.ensure(sizeof(Chunk) * static_cast<size_t>(ctx->plan_cap)));
+  ctx->plan_key_bits = 0;
+  ctx->plan_ccap = 0;
+  if (ctx->chunk_order) {
+    int nseg = 1;
+    for (const auto& ps : ctx->passes) {
+      nseg = std::max(nseg, ps.nseg);
+      ctx->plan_ccap = std::max(ctx->plan_ccap, ps.ccap);
+    }
+    int ket_bits = 0;
+    while ((static_cast<size_t>(1) << ket_bits) <= (ctx->pairQ.size() >> 6)) ++ket_bits;
+    const int bits = plan_class_bits(nseg) + ket_bits;
+    if (bits <= 32 && ctx->plan_ccap > 0) {
+      ctx->plan_key_bits = std::max(bits, 1);
+      const size_t cc = static_cast<size_t>(ctx->plan_ccap);
+      HIPC(ctx->plan_chunks_raw.ensure(sizeof(Chunk) * 2 * cc));
+      HIPC(ctx->plan_keys_raw.ensure(sizeof(unsigned) * 2 * cc));
+      HIPC(ctx->plan_keys_sorted.ensure(sizeof(unsigned) * cc));
+      ctx->plan_sort_bytes = plan_sort_bytes(ctx->plan_ccap);
+      HIPC(ctx->plan_sort_tmp.ensure(std::max<size_t>(ctx->plan_sort_bytes, 8)));
+    }
+  }
   // pair-indexed Coulomb accumulators, one block per pair class
   ctx->jp_cls.assign(kNumPairClasses, JpClass{});
   std::vector<int> ncomp(2 * kNumPairClasses);
@@ -1476,6 +1513,15 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
     pb.ranges = ctx->plan_ranges.as<int>() + 8 * static_cast<size_t>(ps.seg_begin);
     pb.scan_tmp = ctx->plan_scan_tmp.p;
     pb.scan_bytes = ctx->plan_scan_bytes;
+    if (ctx->plan_key_bits > 0) {
+      pb.key_bits = ctx->plan_key_bits;
+      pb.ccap = std::min(ps.ccap, ctx->plan_ccap);
+      pb.chunks_raw = ctx->plan_chunks_raw.as<Chunk>();
+      pb.keys_raw = ctx->plan_keys_raw.as<unsigned>();
+      pb.keys_sorted = ctx->plan_keys_sorted.as<unsigned>();
+      pb.sort_tmp = ctx->plan_sort_tmp.p;
+      pb.sort_bytes = ctx->plan_sort_bytes;
+    }
     HIPC(plan_launch(pa, pb, ctx->stream));
     for (int sg = ps.seg_begin; sg < ps.seg_begin + ps.nseg; ++sg) {
       const int c = ctx->seg_class[sg];
@@ -1643,6 +1689,7 @@ int rchem_ctx_create(int device, rchem_ctx** out) {
   if (const char* e = std::getenv("RCHEM_ERI_GENERIC")) ctx->force_generic = std::atoi(e) != 0;
   if (const char* e = std::getenv("RCHEM_ERI_STREAM")) ctx->eri_stream_mode = std::atoi(e);
   if (const char* e = std::getenv("RCHEM_PROGRESS")) ctx->progress = std::atoi(e) != 0;
+  if (const char* e = std::getenv("RCHEM_CHUNK_ORDER")) ctx->chunk_order = std::atoi(e) != 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
     g_create_error = "stream creation failed";
     delete ctx;
@@ -2301,7 +2348,11 @@ struct FockGraph {
           const double* H, double* F, size_t n2) {
     auto eager = [&]() -> int {
       TRY(jk_device(ctx, D, J, K, &jk));
-      HIPC(fock_launch(H, J, K, F, n2, ctx->stream));
+      if (ctx->nmol > 1 && ctx->mol_nmax > 0)  // a multi-molecule basis: its diagonal blocks
+        HIPC(batch_fock(H, J, K, F, static_cast<size_t>(ctx->nbf), ctx->mol_f0_d.as<int>(),
+                        ctx->mol_nb_d.as<int>(), ctx->nmol, ctx->mol_nmax, ctx->stream));
+      else
+        HIPC(fock_launch(H, J, K, F, n2, ctx->stream));
       return RCHEM_OK;
     };
     if (!enabled || ctx->profiling) return eager();
@@ -2748,9 +2799,12 @@ constexpr double kBatchPadF = 1.0e4;  // Fock diagonal of a padded block's unuse
 
 // One chunk of the batched SCF: molecules [m0, m1) of the caller's lists as one
 // multi-molecule basis.  zoff/aoff: offsets of molecule m0 in Z and in atoms.
+// Molecules [m0, m1) of the arrays given (their atoms from aoff on) as one
+// multi-molecule basis; mol_id[m] is molecule m's index in the caller's input
+// (for messages).
 int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
                 const double* xyz, int aoff, const std::string& want, const rchem_scf_opts* opts,
-                rchem_scf_result* res) {
+                rchem_scf_result* res, const int* mol_id) {
   const int B = m1 - m0;
   auto lower = [](std::string s) {
     for (auto& c : s) c = static_cast<char>(std::tolower(static_cast<unsigned char>(c)));
@@ -2798,7 +2852,7 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
     }
     mol_atoms[b].y = static_cast<int>(Zc.size());
     if (ne % 2 != 0)
-      return ctx->fail(RCHEM_ERR_INVALID, "molecule " + std::to_string(m0 + b) +
+      return ctx->fail(RCHEM_ERR_INVALID, "molecule " + std::to_string(mol_id[m0 + b]) +
                                               " has an odd electron count");
     nocc[b] = ne / 2;
   }
@@ -2880,10 +2934,18 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
   const rocblas_int nn2 = static_cast<rocblas_int>(n2);
   HIPC(hipMemcpyAsync(dH.p, dT.p, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
   BLASC(rocblas_daxpy(hb, nn2, &one, dV.as<double>(), 1, dH.as<double>(), 1));
-  // padded per-molecule batches
-  const size_t m2 = static_cast<size_t>(nmax) * nmax;
-  const rocblas_stride sm = static_cast<rocblas_stride>(m2);
-  DevBuf df0, dnb, dnocc, dSp, dX, dW, dT1, dFb, dC, dDb, dw, de, dinfo, dEm;
+  // padded per-molecule batches.  The batched linear algebra runs over the
+  // molecules still iterating ("slots", in molecule order): when molecules
+  // retire, the slots' per-molecule state (S, X, the DIIS history) is
+  // gathered into a smaller batch, padded to the largest remaining molecule,
+  // so that the iterations left to the slowest molecules cost in proportion
+  // to them, not to the whole chunk.
+  int Bs = B, nm = nmax;
+  std::vector<int> slot_mol(B);
+  for (int b = 0; b < B; ++b) slot_mol[b] = b;
+  size_t m2 = static_cast<size_t>(nmax) * nmax;
+  rocblas_stride sm = static_cast<rocblas_stride>(m2);
+  DevBuf df0, dnb, dnocc, dSp, dX, dW, dT1, dFb, dC, dDb, dw, de, dinfo, dEm, dslots, dgather;
   HIPC(df0.ensure(sizeof(int) * B));
   HIPC(dnb.ensure(sizeof(int) * B));
   // the Fock builds of this basis touch only the molecules' diagonal blocks
@@ -2898,21 +2960,22 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
   HIPC(de.ensure(sizeof(double) * nmax * B));
   HIPC(dinfo.ensure(sizeof(rocblas_int) * B));
   HIPC(dEm.ensure(sizeof(double) * B));
+  HIPC(dslots.ensure(sizeof(int) * B));
   HIPC(hipMemcpyAsync(df0.p, f0.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(dnb.p, nb.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(dnocc.p, nocc.data(), sizeof(int) * B, hipMemcpyHostToDevice, st));
   auto gemm_b = [&](rocblas_operation ta, rocblas_operation tb, const double* A, const double* Bm,
                     double* C) {
-    return rocblas_dgemm_strided_batched(hb, ta, tb, nmax, nmax, nmax, &one, A, nmax, sm, Bm,
-                                         nmax, sm, &zero, C, nmax, sm, B);
+    return rocblas_dgemm_strided_batched(hb, ta, tb, nm, nm, nm, &one, A, nm, sm, Bm, nm, sm,
+                                         &zero, C, nm, sm, Bs);
   };
   // Loewdin X_b = U_b lambda_b^-1/2 U_b^T (rchem.rs:44-51), padding = identity
-  HIPC(batch_pack(dS.as<double>(), n, df0.as<int>(), dnb.as<int>(), nmax, B, 1.0, dSp.as<double>(), st));
-  HIPC(hipMemcpyAsync(dT1.p, dSp.p, sizeof(double) * m2 * B, hipMemcpyDeviceToDevice, st));
-  BLASC(rocsolver_dsyevd_strided_batched(hb, rocblas_evect_original, rocblas_fill_upper, nmax,
-                                         dT1.as<double>(), nmax, sm, dw.as<double>(), nmax,
-                                         de.as<double>(), nmax, dinfo.as<rocblas_int>(), B));
-  HIPC(batch_inv_sqrt_cols(dT1.as<double>(), dw.as<double>(), nmax, B, dW.as<double>(), st));
+  HIPC(batch_pack(dS.as<double>(), n, df0.as<int>(), dnb.as<int>(), nm, Bs, 1.0, dSp.as<double>(), st));
+  HIPC(hipMemcpyAsync(dT1.p, dSp.p, sizeof(double) * m2 * Bs, hipMemcpyDeviceToDevice, st));
+  BLASC(rocsolver_dsyevd_strided_batched(hb, rocblas_evect_original, rocblas_fill_upper, nm,
+                                         dT1.as<double>(), nm, sm, dw.as<double>(), nm,
+                                         de.as<double>(), nm, dinfo.as<rocblas_int>(), Bs));
+  HIPC(batch_inv_sqrt_cols(dT1.as<double>(), dw.as<double>(), nm, Bs, dW.as<double>(), st));
   BLASC(gemm_b(rocblas_operation_none, rocblas_operation_transpose, dW.as<double>(),
                dT1.as<double>(), dX.as<double>()));
   // density from a padded Fock batch: F' = X^T F X, eigh, C = X C', D = C_occ C_occ^T
@@ -2920,31 +2983,33 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
     BLASC(gemm_b(rocblas_operation_none, rocblas_operation_none, Fb, dX.as<double>(), dT1.as<double>()));
     BLASC(gemm_b(rocblas_operation_transpose, rocblas_operation_none, dX.as<double>(),
                  dT1.as<double>(), dW.as<double>()));
-    BLASC(rocsolver_dsyevd_strided_batched(hb, rocblas_evect_original, rocblas_fill_upper, nmax,
-                                           dW.as<double>(), nmax, sm, dw.as<double>(), nmax,
-                                           de.as<double>(), nmax, dinfo.as<rocblas_int>(), B));
+    BLASC(rocsolver_dsyevd_strided_batched(hb, rocblas_evect_original, rocblas_fill_upper, nm,
+                                           dW.as<double>(), nm, sm, dw.as<double>(), nm,
+                                           de.as<double>(), nm, dinfo.as<rocblas_int>(), Bs));
     BLASC(gemm_b(rocblas_operation_none, rocblas_operation_none, dX.as<double>(), dW.as<double>(),
                  dC.as<double>()));
-    HIPC(batch_mask_cols(dC.as<double>(), nmax, dnocc.as<int>(), B, st));
+    HIPC(batch_mask_cols(dC.as<double>(), nm, dnocc.as<int>(), Bs, st));
     BLASC(gemm_b(rocblas_operation_none, rocblas_operation_transpose, dC.as<double>(),
                  dC.as<double>(), dDb.as<double>()));
-    HIPC(batch_unpack(dDb.as<double>(), nmax, df0.as<int>(), dnb.as<int>(), B, n, dD.as<double>(), st));
+    HIPC(batch_unpack(dDb.as<double>(), nm, df0.as<int>(), dnb.as<int>(), Bs, n, dD.as<double>(), st));
     return RCHEM_OK;
   };
   auto density_dense = [&](const double* Fd) -> int {
-    HIPC(batch_pack(Fd, n, df0.as<int>(), dnb.as<int>(), nmax, B, kBatchPadF, dFb.as<double>(), st));
+    HIPC(batch_pack(Fd, n, df0.as<int>(), dnb.as<int>(), nm, Bs, kBatchPadF, dFb.as<double>(), st));
     return density_b(dFb.as<double>());
   };
+  // per-slot energies into the molecules' entries of e
+  std::vector<double> eslot(B);
   auto energies = [&](const double* Fd, std::vector<double>* e) -> int {
-    HIPC(batch_energy(dD.as<double>(), dH.as<double>(), Fd, n, df0.as<int>(), dnb.as<int>(), B,
+    HIPC(batch_energy(dD.as<double>(), dH.as<double>(), Fd, n, df0.as<int>(), dnb.as<int>(), Bs,
                       dEm.as<double>(), st));
-    e->resize(B);
-    HIPC(hipMemcpyAsync(e->data(), dEm.p, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(eslot.data(), dEm.p, sizeof(double) * Bs, hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
+    for (int s = 0; s < Bs; ++s) (*e)[slot_mol[s]] = eslot[s];
     return RCHEM_OK;
   };
-  TRY(density_dense(dH.as<double>()));  // core-Hamiltonian guess (rchem.rs:56-61)
   std::vector<double> e_cur(B, 0.0), e_prev(B, 0.0), errmax(B, 0.0), de_last(B, 1.0);
+  TRY(density_dense(dH.as<double>()));  // core-Hamiltonian guess (rchem.rs:56-61)
   TRY(energies(dH.as<double>(), &e_cur));
   std::vector<int> conv_at(B, -1);
   std::vector<double> e_final(B, 0.0);
@@ -2978,23 +3043,82 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
     HIPC(dcoef.ensure(sizeof(double) * B * k));
     HIPC(derr.ensure(sizeof(double) * B));
   }
+  // DIIS matrices per molecule (entries <e_i, e_j> of its ring)
   std::vector<double> Bm(static_cast<size_t>(B) * k * k, 0.0);
-  // density damping (opts->damping): previous density and per-molecule factors
+  // density damping (opts->damping): the slots' previous densities (padded)
+  // and per-slot factors
   DevBuf dDold, dalpha;
-  std::vector<double> alpha(B, 0.0);
+  std::vector<double> alpha(B, 0.0), errslot(B, 0.0);
   const bool damping = opts->damping > 0.0;
   if (damping) {
-    HIPC(dDold.ensure(n2 * sizeof(double)));
+    HIPC(dDold.ensure(sizeof(double) * m2 * B));
     HIPC(dalpha.ensure(sizeof(double) * B));
   }
+  auto save_old = [&]() -> int {
+    if (damping)
+      HIPC(batch_pack(dD.as<double>(), n, df0.as<int>(), dnb.as<int>(), nm, Bs, 0.0,
+                      dDold.as<double>(), st));
+    return RCHEM_OK;
+  };
+  // measure: per molecule
   auto damp_b = [&](const std::vector<double>& measure) -> int {
     if (!damping) return RCHEM_OK;
-    for (int b = 0; b < B; ++b)
-      alpha[b] = (conv_at[b] < 0 && measure[b] >= opts->damping_until) ? opts->damping : 0.0;
-    HIPC(hipMemcpyAsync(dalpha.p, alpha.data(), sizeof(double) * B, hipMemcpyHostToDevice, st));
+    for (int s = 0; s < Bs; ++s) {
+      const int b = slot_mol[s];
+      alpha[s] = (conv_at[b] < 0 && measure[b] >= opts->damping_until) ? opts->damping : 0.0;
+    }
+    HIPC(hipMemcpyAsync(dalpha.p, alpha.data(), sizeof(double) * Bs, hipMemcpyHostToDevice, st));
     HIPC(batch_blend(dD.as<double>(), dDold.as<double>(), n, df0.as<int>(), dnb.as<int>(),
-                     dalpha.as<double>(), nmax, B, st));
+                     dalpha.as<double>(), nm, Bs, st));
     HIPC(hipStreamSynchronize(st));  // alpha is reused next iteration
+    return RCHEM_OK;
+  };
+  // drops the retired molecules' slots: gathers S, X and the DIIS history of
+  // the remaining ones, padded to the largest of them (the padding rows and
+  // columns hold the identity, pad * identity or zeros, so the top-left part
+  // of a block is the block at the smaller size)
+  auto compact = [&]() -> int {
+    std::vector<int> keep;
+    int nm_new = 1;
+    for (int s = 0; s < Bs; ++s)
+      if (conv_at[slot_mol[s]] < 0) {
+        keep.push_back(s);
+        nm_new = std::max(nm_new, nb[slot_mol[s]]);
+      }
+    const int Bn = static_cast<int>(keep.size());
+    if (Bn == Bs || Bn == 0) return RCHEM_OK;
+    const size_t m2n = static_cast<size_t>(nm_new) * nm_new;
+    HIPC(hipMemcpyAsync(dslots.p, keep.data(), sizeof(int) * Bn, hipMemcpyHostToDevice, st));
+    auto gather = [&](DevBuf& x, int nring) -> int {
+      HIPC(dgather.ensure(sizeof(double) * m2n * Bn * nring));
+      HIPC(batch_gather(x.as<double>(), m2 * Bs, nm, dslots.as<int>(), Bn, nring,
+                        dgather.as<double>(), m2n * Bn, nm_new, st));
+      HIPC(hipMemcpyAsync(x.p, dgather.p, sizeof(double) * m2n * Bn * nring,
+                          hipMemcpyDeviceToDevice, st));
+      return RCHEM_OK;
+    };
+    TRY(gather(dSp, 1));
+    TRY(gather(dX, 1));
+    if (k > 0) {
+      TRY(gather(dFs, k));
+      TRY(gather(dEs, k));
+    }
+    std::vector<int> mols(Bn), sf0(Bn), snb(Bn), socc(Bn);
+    for (int s = 0; s < Bn; ++s) {
+      mols[s] = slot_mol[keep[s]];
+      sf0[s] = f0[mols[s]];
+      snb[s] = nb[mols[s]];
+      socc[s] = nocc[mols[s]];
+    }
+    HIPC(hipMemcpyAsync(df0.p, sf0.data(), sizeof(int) * Bn, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(dnb.p, snb.data(), sizeof(int) * Bn, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(dnocc.p, socc.data(), sizeof(int) * Bn, hipMemcpyHostToDevice, st));
+    HIPC(hipStreamSynchronize(st));  // keep and the per-slot vectors are host temporaries
+    slot_mol = mols;
+    Bs = Bn;
+    nm = nm_new;
+    m2 = m2n;
+    sm = static_cast<rocblas_stride>(m2);
     return RCHEM_OK;
   };
   int count = 0, it = 0;
@@ -3008,43 +3132,47 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
     if (k > 0) {
       // energy of the current density with its own Fock matrix
       TRY(energies(dF.as<double>(), &e_cur));
-      const int slot = count % k;
-      double* Fslot = dFs.as<double>() + m2 * B * slot;
-      double* Eslot = dEs.as<double>() + m2 * B * slot;
-      HIPC(batch_pack(dF.as<double>(), n, df0.as<int>(), dnb.as<int>(), nmax, B, kBatchPadF, Fslot, st));
-      HIPC(batch_pack(dD.as<double>(), n, df0.as<int>(), dnb.as<int>(), nmax, B, 0.0, dDp.as<double>(), st));
+      const int ring = count % k;
+      double* Fslot = dFs.as<double>() + m2 * Bs * ring;
+      double* Eslot = dEs.as<double>() + m2 * Bs * ring;
+      HIPC(batch_pack(dF.as<double>(), n, df0.as<int>(), dnb.as<int>(), nm, Bs, kBatchPadF, Fslot, st));
+      HIPC(batch_pack(dD.as<double>(), n, df0.as<int>(), dnb.as<int>(), nm, Bs, 0.0, dDp.as<double>(), st));
       BLASC(gemm_b(rocblas_operation_none, rocblas_operation_none, Fslot, dDp.as<double>(), dT1.as<double>()));
       BLASC(gemm_b(rocblas_operation_none, rocblas_operation_none, dT1.as<double>(), dSp.as<double>(), dW.as<double>()));
-      HIPC(batch_antisym(dW.as<double>(), dE1.as<double>(), nmax, B, st));
+      HIPC(batch_antisym(dW.as<double>(), dE1.as<double>(), nm, Bs, st));
       BLASC(gemm_b(rocblas_operation_transpose, rocblas_operation_none, dX.as<double>(), dE1.as<double>(), dT1.as<double>()));
       BLASC(gemm_b(rocblas_operation_none, rocblas_operation_none, dT1.as<double>(), dX.as<double>(), Eslot));
       ++count;
       const int m = std::min(count, k);
-      std::vector<long long> prs(static_cast<size_t>(2) * B * m);
-      for (int b = 0; b < B; ++b)
+      std::vector<long long> prs(static_cast<size_t>(2) * Bs * m);
+      for (int s = 0; s < Bs; ++s)
         for (int i = 0; i < m; ++i) {
-          prs[2 * (static_cast<size_t>(b) * m + i)] = static_cast<long long>(m2 * B * slot + m2 * b);
-          prs[2 * (static_cast<size_t>(b) * m + i) + 1] = static_cast<long long>(m2 * B * i + m2 * b);
+          prs[2 * (static_cast<size_t>(s) * m + i)] = static_cast<long long>(m2 * Bs * ring + m2 * s);
+          prs[2 * (static_cast<size_t>(s) * m + i) + 1] = static_cast<long long>(m2 * Bs * i + m2 * s);
         }
       HIPC(hipMemcpyAsync(dpairs.p, prs.data(), sizeof(long long) * prs.size(), hipMemcpyHostToDevice, st));
-      HIPC(batch_dots(dEs.as<double>(), dpairs.as<long long>(), B * m, m2, ddots.as<double>(), st));
-      HIPC(batch_maxabs(Eslot, nmax, B, derr.as<double>(), st));
-      std::vector<double> dots(static_cast<size_t>(B) * m);
+      HIPC(batch_dots(dEs.as<double>(), dpairs.as<long long>(), Bs * m, m2, ddots.as<double>(), st));
+      HIPC(batch_maxabs(Eslot, nm, Bs, derr.as<double>(), st));
+      std::vector<double> dots(static_cast<size_t>(Bs) * m);
       HIPC(hipMemcpyAsync(dots.data(), ddots.p, sizeof(double) * dots.size(), hipMemcpyDeviceToHost, st));
-      HIPC(hipMemcpyAsync(errmax.data(), derr.p, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+      HIPC(hipMemcpyAsync(errslot.data(), derr.p, sizeof(double) * Bs, hipMemcpyDeviceToHost, st));
       HIPC(hipStreamSynchronize(st));
-      for (int b = 0; b < B; ++b)
-        for (int i = 0; i < m; ++i) {
-          double* Bb = Bm.data() + static_cast<size_t>(b) * k * k;
-          Bb[slot * k + i] = Bb[i * k + slot] = dots[static_cast<size_t>(b) * m + i];
-        }
+      for (int s = 0; s < Bs; ++s) {
+        const int b = slot_mol[s];
+        errmax[b] = errslot[s];
+        double* Bb = Bm.data() + static_cast<size_t>(b) * k * k;
+        for (int i = 0; i < m; ++i)
+          Bb[ring * k + i] = Bb[i * k + ring] = dots[static_cast<size_t>(s) * m + i];
+      }
     } else {
-      if (damping)
-        HIPC(hipMemcpyAsync(dDold.p, dD.p, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+      TRY(save_old());
       TRY(density_dense(dF.as<double>()));
       TRY(damp_b(de_last));  // |dE| of the previous iteration
       TRY(energies(dF.as<double>(), &e_cur));  // rchem's sum D_new o (H + F(D_old))
-      for (int b = 0; b < B; ++b) de_last[b] = std::fabs(e_cur[b] - e_prev[b]);
+      for (int s = 0; s < Bs; ++s) {
+        const int b = slot_mol[s];
+        de_last[b] = std::fabs(e_cur[b] - e_prev[b]);
+      }
     }
     double demax = 0.0;
     bool changed = false;
@@ -3053,8 +3181,8 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
       if (conv_at[b] >= 0) continue;
       const double d = std::fabs(e_cur[b] - e_prev[b]);
       demax = std::max(demax, d);
-      // per-molecule convergence-triggered switch to all-FP64 quartets (every
-      // molecule by iteration 64, so a stalled one cannot stay in FP32)
+      // per-molecule convergence-triggered switch to all-FP64 quartets
+      // (every molecule by iteration 64, so a stalled one cannot stay in FP32)
       if (active[b] == 1 && it > 0 && (d < 1e-7 || it >= 64)) {
         active[b] = 2;
         fp64_at[b] = it + 1;
@@ -3062,7 +3190,7 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
       }
       if (active[b] == 1) ++mixed_left;
     }
-    if (opts->verbose) std::printf("%4d batch max|dE| %12.3e\n", it, demax);
+    if (opts->verbose) std::printf("%4d batch max|dE| %12.3e (%d molecules iterating)\n", it, demax, Bs);
     if (jk.precision == RCHEM_PREC_MIXED && mixed_left == 0)
       jk.precision = RCHEM_PREC_FP64;  // no FP32 quartets left: drop the FP32 kernels
     all_conv = true;
@@ -3089,11 +3217,12 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
     if (retired || changed)
       HIPC(hipMemcpyAsync(ctx->mol_active_d.p, active.data(), sizeof(int) * B,
                           hipMemcpyHostToDevice, st));
+    if (retired) TRY(compact());
     if (k > 0) {
       const int m = std::min(count, k);
-      std::vector<double> coef(static_cast<size_t>(B) * k, 0.0);
-      for (int b = 0; b < B; ++b) {
-        const double* Bb = Bm.data() + static_cast<size_t>(b) * k * k;
+      std::vector<double> coef(static_cast<size_t>(Bs) * k, 0.0);
+      for (int s = 0; s < Bs; ++s) {
+        const double* Bb = Bm.data() + static_cast<size_t>(slot_mol[s]) * k * k;
         const int sz = m + 1;
         std::vector<double> A(static_cast<size_t>(sz) * sz, 0.0), rhs(sz, 0.0);
         for (int i = 0; i < m; ++i) {
@@ -3120,12 +3249,11 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
         }
         const int last = (count - 1) % k;
         for (int i = 0; i < m; ++i)
-          coef[static_cast<size_t>(b) * k + i] = ok ? rhs[i] / A[i * sz + i] : (i == last ? 1.0 : 0.0);
+          coef[static_cast<size_t>(s) * k + i] = ok ? rhs[i] / A[i * sz + i] : (i == last ? 1.0 : 0.0);
       }
       HIPC(hipMemcpyAsync(dcoef.p, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, st));
-      HIPC(batch_combine(dFs.as<double>(), m2 * B, m, k, dcoef.as<double>(), nmax, B, dFb.as<double>(), st));
-      if (damping)
-        HIPC(hipMemcpyAsync(dDold.p, dD.p, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+      HIPC(batch_combine(dFs.as<double>(), m2 * Bs, m, k, dcoef.as<double>(), nm, Bs, dFb.as<double>(), st));
+      TRY(save_old());
       TRY(density_b(dFb.as<double>()));
       TRY(damp_b(errmax));
     }
@@ -3178,15 +3306,40 @@ int rchem_batch_scf(rchem_ctx* ctx, int nmol, const int* natom, const int* Z, co
           for (int q = 0; q < 2; ++q)
             if (kBasisRows[r].am[q] >= 0) nbf[m] += ncart(kBasisRows[r].am[q]);
   }
+  // chunks of molecules of similar size (largest first): the padded batches
+  // of a chunk are as large as its largest molecule, and molecules that take
+  // many iterations (mostly the large ones) keep fewer small ones iterating
+  // with them
+  std::vector<int> order(nmol), astart(nmol);
+  for (int m = 0, s = 0; m < nmol; ++m) {
+    order[m] = m;
+    astart[m] = s;
+    s += natom[m];
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return nbf[x] > nbf[y]; });
+  std::vector<int> natom_s(nmol), Z_s;
+  std::vector<double> xyz_s;
+  Z_s.reserve(a);
+  xyz_s.reserve(3 * static_cast<size_t>(a));
+  for (int m = 0; m < nmol; ++m) {
+    const int src = order[m];
+    natom_s[m] = natom[src];
+    for (int k = 0; k < natom[src]; ++k) {
+      Z_s.push_back(Z[astart[src] + k]);
+      for (int x = 0; x < 3; ++x) xyz_s.push_back(xyz[3 * static_cast<size_t>(astart[src] + k) + x]);
+    }
+  }
+  std::vector<rchem_scf_result> res_s(nmol);
   int status = RCHEM_OK, m0 = 0, aoff = 0;
   while (m0 < nmol) {
     int m1 = m0, tot = 0, aend = aoff;
-    while (m1 < nmol && (m1 == m0 || tot + nbf[m1] <= kBatchMaxNbf)) {
-      tot += nbf[m1];
-      aend += natom[m1];
+    while (m1 < nmol && (m1 == m0 || tot + nbf[order[m1]] <= kBatchMaxNbf)) {
+      tot += nbf[order[m1]];
+      aend += natom_s[m1];
       ++m1;
     }
-    const int rc = batch_chunk(ctx, m0, m1, natom, Z, xyz, aoff, want, opts, res);
+    const int rc = batch_chunk(ctx, m0, m1, natom_s.data(), Z_s.data(), xyz_s.data(), aoff, want,
+                               opts, res_s.data(), order.data());
     if (rc != RCHEM_OK && rc != RCHEM_ERR_NOT_CONVERGED) {
       ctx->shell_mol.clear();
       ctx->nmol = 1;
@@ -3194,6 +3347,7 @@ int rchem_batch_scf(rchem_ctx* ctx, int nmol, const int* natom, const int* Z, co
       return rc;
     }
     if (rc == RCHEM_ERR_NOT_CONVERGED) status = rc;
+    for (int m = m0; m < m1; ++m) res[order[m]] = res_s[m];
     m0 = m1;
     aoff = aend;
   }
