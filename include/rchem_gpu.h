@@ -194,7 +194,7 @@ int rchem_jk_class_timings(rchem_ctx *ctx, rchem_class_timing *out, int max, int
  * table data/tuning_gfx950.txt beside the library (RCHEM_TUNING_FILE names
  * another; empty: none), written by rchem_amd/autotune.py; classes without an
  * entry use shape 0, 2048 workgroups, 64 tiles.  Lines of the table:
- * "la lb lc ld fp32 variant blocks chunk_tiles", '#' comments.  A shape a class
+ * "la lb lc ld fp32 variant blocks chunk_tiles [ket_order]", '#' comments.  A shape a class
  * is not compiled in falls back to shape 0.  Setting an entry re-plans the
  * next build (which allocates). */
 int rchem_lds_variant_info(int variant, int *waves, int *min_waves_per_simd, int *prefetch,
@@ -203,6 +203,16 @@ int rchem_set_class_tuning(rchem_ctx *ctx, int la, int lb, int lc, int ld, int f
                            int blocks, int chunk_tiles);
 int rchem_get_class_tuning(const rchem_ctx *ctx, int la, int lb, int lc, int ld, int fp32,
                            int *variant, int *blocks, int *chunk_tiles);
+/* LDS chunks of a class and precision in ket order (ket_order 1; 0: in bra
+ * order; -1: the default, FP64 lists in ket order, FP32 lists in bra order):
+ * the planner sorts the chunks by their first ket pair, so that the
+ * workgroups running at one time read the same ket records.  The table's
+ * optional ninth column sets it; RCHEM_CHUNK_ORDER=0 / 1 overrides every
+ * class.  get reports the order in effect. */
+int rchem_set_class_ket_order(rchem_ctx *ctx, int la, int lb, int lc, int ld, int fp32,
+                              int ket_order);
+int rchem_get_class_ket_order(const rchem_ctx *ctx, int la, int lb, int lc, int ld, int fp32,
+                              int *ket_order);
 /* Replaces the context's table by the file at path (NULL or "": no table,
  * every class at its defaults); *entries (may be NULL) receives the count. */
 int rchem_load_tuning(rchem_ctx *ctx, const char *path, int *entries);

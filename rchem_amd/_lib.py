@@ -93,6 +93,9 @@ SIGNATURES = {
     "rchem_scf_opts_default": (ctypes.c_int, [ctypes.POINTER(SCFOpts)]),
     "rchem_lds_variant_info": (ctypes.c_int, [ctypes.c_int, _ip, _ip, _ip, _ip, _ip]),
     "rchem_set_class_tuning": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 8),
+    "rchem_set_class_ket_order": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 6),
+    "rchem_get_class_ket_order": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 5 +
+                                  [ctypes.POINTER(ctypes.c_int)]),
     "rchem_get_class_tuning": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 5 +
                                [_ip, _ip, _ip]),
     "rchem_load_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _ip]),

@@ -8,8 +8,10 @@ per-class HIP-event timings:
      primitive pairs per inner iteration, branch-free Boys function;
      rchem_lds_variant_info) at chunks of 64 and of 256 tiles, all classes
      switched together, one timed set of builds per setting;
-  2. persistent workgroup counts (512 ... 8192) at each class's best shape;
-  3. chunk sizes (16 ... 256 tiles) at the best shape and count.
+  2. the chunk order: chunks sorted by their first ket pair (workgroups
+     running together share ket records in L2) or left in bra order;
+  3. persistent workgroup counts (512 ... 8192) at each class's best shape;
+  4. chunk sizes (16 ... 256 tiles) at the best shape and count.
 A choice is kept for a class only where it beats the class's best so far by
 more than the measured noise (1 %).  The FP64 kernels are timed in the
 all-FP64 build and in the FP64 part of the mixed build and tuned on the sum;
@@ -101,10 +103,16 @@ class Tuner:
                 out[k] = out.get(k, 0.0) + v
         return out
 
-    def set(self, key, variant, blocks, chunk):
+    def set(self, key, variant, blocks, chunk, order):
         la, lb, lc, ld, fp32 = key
         self.ctx.check(self.L.rchem_set_class_tuning(self.ctx.h, la, lb, lc, ld, fp32, variant,
                                                      blocks, chunk))
+        self.ctx.check(self.L.rchem_set_class_ket_order(self.ctx.h, la, lb, lc, ld, fp32, order))
+
+    def order(self, key):
+        out = ctypes.c_int()
+        self.ctx.check(self.L.rchem_get_class_ket_order(self.ctx.h, *key, ctypes.byref(out)))
+        return out.value
 
 
 def tune(tuner):
@@ -115,29 +123,30 @@ def tune(tuner):
     keys = sorted(base)
     _log(f"defaults: {len(keys)} class kernels, {sum(base.values()):.2f} ms "
          f"(FP64 + mixed build)")
-    best = {k: {"variant": 0, "blocks": 2048, "chunk": 64, "ms": base[k]} for k in keys}
+    best = {k: {"variant": 0, "blocks": 2048, "chunk": 64, "order": tuner.order(k), "ms": base[k]}
+            for k in keys}
 
     def sweep(name, values, apply):
         for val in values:
             for k in keys:
-                v, b, c = apply(best[k], val)
-                tuner.set(k, v, b, c)
+                tuner.set(k, *apply(best[k], val))
             t = tuner.time_all()
             for k in keys:
                 if k in t and t[k] < best[k]["ms"] * (1.0 - NOISE):
-                    v, b, c = apply(best[k], val)
-                    best[k] = {"variant": v, "blocks": b, "chunk": c, "ms": t[k]}
+                    v, b, c, o = apply(best[k], val)
+                    best[k] = {"variant": v, "blocks": b, "chunk": c, "order": o, "ms": t[k]}
             _log(f"{name} {val}: {sum(t.get(k, 0.0) for k in keys):.2f} ms")
         for k in keys:
-            tuner.set(k, best[k]["variant"], best[k]["blocks"], best[k]["chunk"])
+            tuner.set(k, best[k]["variant"], best[k]["blocks"], best[k]["chunk"], best[k]["order"])
 
     # shape and chunk size interact (the chunk is the workgroup's share of one
     # bra pair): every shape at a short and at a long chunk first
     for chunk in (64, 256):
         sweep(f"shape at chunk {chunk}:", range(tuner.nvariants),
-              lambda cur, v, c=chunk: (v, cur["blocks"], c))
-    sweep("workgroups", BLOCKS, lambda cur, b: (cur["variant"], b, cur["chunk"]))
-    sweep("chunk", CHUNKS, lambda cur, c: (cur["variant"], cur["blocks"], c))
+              lambda cur, v, c=chunk: (v, cur["blocks"], c, cur["order"]))
+    sweep("ket order", (0, 1), lambda cur, o: (cur["variant"], cur["blocks"], cur["chunk"], o))
+    sweep("workgroups", BLOCKS, lambda cur, b: (cur["variant"], b, cur["chunk"], cur["order"]))
+    sweep("chunk", CHUNKS, lambda cur, c: (cur["variant"], cur["blocks"], c, cur["order"]))
     final = tuner.time_all()
     return base, best, final
 
@@ -145,11 +154,12 @@ def tune(tuner):
 def write_table(path, best, device, workload):
     lines = ["# Fock-build launch tuning, written by rchem_amd/autotune.py",
              f"# device: {device}; workload: {workload}",
-             "# la lb lc ld fp32 variant blocks chunk_tiles   (ms per FP64 + mixed build)"]
+             "# la lb lc ld fp32 variant blocks chunk_tiles ket_order   "
+             "(ms per FP64 + mixed build)"]
     for k in sorted(best):
         b = best[k]
         lines.append(f"{k[0]} {k[1]} {k[2]} {k[3]} {k[4]} {b['variant']} {b['blocks']} "
-                     f"{b['chunk']}   # {b['ms']:.3f}")
+                     f"{b['chunk']} {b['order']}   # {b['ms']:.3f}")
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     with open(path, "w") as fh:
         fh.write("\n".join(lines) + "\n")
@@ -190,7 +200,8 @@ def main(argv=None):
               "classes": [{"class": f"({k[0]}{k[1]}|{k[2]}{k[3]})" + (" fp32" if k[4] else ""),
                            "default_ms": base[k], "tuned_ms": final.get(k),
                            "variant": best[k]["variant"], "blocks": best[k]["blocks"],
-                           "chunk_tiles": best[k]["chunk"]} for k in sorted(best)]}
+                           "chunk_tiles": best[k]["chunk"], "ket_order": best[k]["order"]}
+                          for k in sorted(best)]}
     print(json.dumps(report), flush=True)
     ctx.close()
 

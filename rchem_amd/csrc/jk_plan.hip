@@ -14,7 +14,8 @@
 //   5. chunks: the tiles of one (class, bra pair) unit are cut into chunks of
 //              at most the class's chunk size per precision (from the tuning
 //              table; a workgroup's unit of work when K is
-//              digested in LDS), counted, scanned and written; per class the
+//              digested in LDS), counted, scanned and written (optionally
+//              radix-sorted by first ket pair within each class); per class the
 //              [begin, end) tile and chunk ranges of each precision go to device
 //              memory, where the persistent ERI kernels read them.
 #include <hip/hip_runtime.h>
@@ -177,33 +178,59 @@ __global__ void chunk_count_kernel(const PlanArgs a, const int* __restrict__ off
   chc32[u] = (off32[t1] - off32[t0] + ch32 - 1) / ch32;
 }
 
+// cap bounds the tile offsets (tile buffers), ccap the chunk records written
+// (chunk buffers); every record below min(chunk count, ccap) is written.
+// keys64 / keys32 (when set): per chunk, (c << key_shift) | (first ket pair of
+// the chunk / 64), c the chunk's class (segment) in this pass.
 __global__ void chunk_expand_kernel(const PlanArgs a, const int* __restrict__ off64,
                                     const int* __restrict__ off32, const int* __restrict__ choff64,
                                     const int* __restrict__ choff32, Chunk* __restrict__ chunks64,
-                                    Chunk* __restrict__ chunks32, int cap) {
+                                    Chunk* __restrict__ chunks32, int cap, int ccap,
+                                    const Tile* __restrict__ tiles64,
+                                    const Tile* __restrict__ tiles32, unsigned* __restrict__ keys64,
+                                    unsigned* __restrict__ keys32, int key_shift) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= a.nunits) return;
   int c, bra, t0, t1;
   unit_tasks(a, u, &c, &bra, &t0, &t1);
+  const unsigned khigh = static_cast<unsigned>(c) << key_shift;
+  // a class planned in bra order keeps it: its keys differ only by class, and
+  // the sort is stable
+  const int order = a.classes[c].order;
   {
     const int ch = a.classes[c].chunk64;
     const int tb = min(off64[t0], cap), te = min(off64[t1], cap);
     int o = choff64[u];
-    for (int k = tb; k < te && o < cap; k += ch, ++o) chunks64[o] = Chunk{bra, k, min(k + ch, te), 0};
+    for (int k = tb; k < te && o < ccap; k += ch, ++o) {
+      chunks64[o] = Chunk{bra, k, min(k + ch, te), 0};
+      if (keys64) keys64[o] = khigh | ((order & 1) ? static_cast<unsigned>(tiles64[k].ket0 >> 6) : 0u);
+    }
   }
   {
     const int ch = a.classes[c].chunk32;
     const int tb = min(off32[t0], cap), te = min(off32[t1], cap);
     int o = choff32[u];
-    for (int k = tb; k < te && o < cap; k += ch, ++o) chunks32[o] = Chunk{bra, k, min(k + ch, te), 0};
+    for (int k = tb; k < te && o < ccap; k += ch, ++o) {
+      chunks32[o] = Chunk{bra, k, min(k + ch, te), 0};
+      if (keys32) keys32[o] = khigh | ((order & 2) ? static_cast<unsigned>(tiles32[k].ket0 >> 6) : 0u);
+    }
   }
+}
+
+// keys[i] = all ones for i in [total, n): the entries past the chunk count
+// sort to the end.
+__global__ void pad_keys_kernel(unsigned* __restrict__ keys, const int* __restrict__ total, int n) {
+  const int t = min(*total, n);
+  for (int i = t + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    keys[i] = ~0u;
 }
 
 // ranges[8c .. 8c+7] = tiles {b64, e64, b32, e32}, chunks {b64, e64, b32, e32}
 // of class c, clamped to the buffer capacity.
 __global__ void ranges_kernel(const PlanArgs a, const int* __restrict__ off64,
                               const int* __restrict__ off32, const int* __restrict__ choff64,
-                              const int* __restrict__ choff32, int* __restrict__ ranges, int cap) {
+                              const int* __restrict__ choff32, int* __restrict__ ranges, int cap,
+                              int ccap) {
   const int c = threadIdx.x;
   if (c >= a.nclasses) return;
   const int tb = a.classes[c].task_begin;
@@ -214,10 +241,10 @@ __global__ void ranges_kernel(const PlanArgs a, const int* __restrict__ off64,
   ranges[8 * c + 1] = min(off64[te], cap);
   ranges[8 * c + 2] = min(off32[tb], cap);
   ranges[8 * c + 3] = min(off32[te], cap);
-  ranges[8 * c + 4] = min(choff64[ub], cap);
-  ranges[8 * c + 5] = min(choff64[ue], cap);
-  ranges[8 * c + 6] = min(choff32[ub], cap);
-  ranges[8 * c + 7] = min(choff32[ue], cap);
+  ranges[8 * c + 4] = min(choff64[ub], ccap);
+  ranges[8 * c + 5] = min(choff64[ue], ccap);
+  ranges[8 * c + 6] = min(choff32[ub], ccap);
+  ranges[8 * c + 7] = min(choff32[ue], ccap);
 }
 
 hipError_t dmax_launch(const double* D, size_t n2, unsigned long long* out, hipStream_t s) {
@@ -227,6 +254,24 @@ hipError_t dmax_launch(const double* D, size_t n2, unsigned long long* out, hipS
   hipLaunchKernelGGL(dmax_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, D, n2,
                      out);
   return hipGetLastError();
+}
+
+// Bits of the class field of a chunk key (classes of one pass).
+int plan_class_bits(int nclasses) {
+  int bits = 0;
+  while ((1 << bits) < nclasses) ++bits;
+  return bits;
+}
+
+// Temporary-storage bytes of a radix sort of n chunk keys and chunks.
+size_t plan_sort_bytes(int n) {
+  size_t bytes = 0;
+  if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, static_cast<unsigned*>(nullptr),
+                                         static_cast<unsigned*>(nullptr),
+                                         static_cast<Chunk*>(nullptr),
+                                         static_cast<Chunk*>(nullptr), n, 0, 32) != hipSuccess)
+    return 0;
+  return bytes;
 }
 
 // Temporary-storage bytes of a scan over n + 1 entries.
@@ -265,15 +310,40 @@ hipError_t plan_launch(const PlanArgs& a, const PlanBuffers& b, hipStream_t s) {
   sb = b.scan_bytes;
   e = hipcub::DeviceScan::ExclusiveSum(b.scan_tmp, sb, b.chc32, b.choff32, a.nunits + 1, s);
   if (e != hipSuccess) return e;
+  const bool sorting = b.key_bits > 0 && b.ccap > 0;
+  const bool sort64 = sorting && (b.sort_mask & 1);
+  // FP64-only builds have no FP32 chunks
+  const bool sort32 = sorting && (b.sort_mask & 2) && a.mixed_thresh > 0.0;
+  // chunk records per precision: every record below min(count, ccap) is written
+  const int ccap = sorting ? min(b.cap, b.ccap) : b.cap;
+  Chunk* out64 = sort64 ? b.chunks_raw : b.chunks64;
+  Chunk* out32 = sort32 ? b.chunks_raw + b.ccap : b.chunks32;
+  unsigned* k64 = sort64 ? b.keys_raw : nullptr;
+  unsigned* k32 = sort32 ? b.keys_raw + b.ccap : nullptr;
   if (a.nunits > 0) {
     hipLaunchKernelGGL(chunk_expand_kernel, dim3(static_cast<unsigned>((a.nunits + 255) / 256)),
-                       dim3(256), 0, s, a, b.off64, b.off32, b.choff64, b.choff32, b.chunks64,
-                       b.chunks32, b.cap);
+                       dim3(256), 0, s, a, b.off64, b.off32, b.choff64, b.choff32, out64, out32,
+                       b.cap, ccap, b.tiles64, b.tiles32, k64, k32,
+                       b.key_bits - plan_class_bits(a.nclasses));
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
+  for (int p = 0; p < 2; ++p) {
+    if (!(p ? sort32 : sort64)) continue;
+    unsigned* kin = b.keys_raw + static_cast<size_t>(p) * b.ccap;
+    const unsigned pad_blocks = static_cast<unsigned>(min((b.ccap + 255) / 256, 1024));
+    hipLaunchKernelGGL(pad_keys_kernel, dim3(pad_blocks), dim3(256), 0, s, kin,
+                       (p ? b.choff32 : b.choff64) + a.nunits, b.ccap);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    size_t sb = b.sort_bytes;
+    e = hipcub::DeviceRadixSort::SortPairs(b.sort_tmp, sb, kin, b.keys_sorted,
+                                           b.chunks_raw + static_cast<size_t>(p) * b.ccap,
+                                           p ? b.chunks32 : b.chunks64, b.ccap, 0, b.key_bits, s);
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(ranges_kernel, dim3(1), dim3(64), 0, s, a, b.off64, b.off32, b.choff64,
-                     b.choff32, b.ranges, b.cap);
+                     b.choff32, b.ranges, b.cap, ccap);
   return hipGetLastError();
 }
 

@@ -249,6 +249,7 @@ struct ClassTune {
   int variant = 0;
   int blocks = kDefaultBlocks;
   int chunk_tiles = kDefaultChunkTiles;
+  int ket_order = -1;  // LDS chunks in ket order: 1 yes, 0 no, -1 the default (FP64 yes, FP32 no)
 };
 
 // Host-side record of a shell pair before it is laid out.
@@ -325,7 +326,8 @@ struct rchem_ctx {
   // J, K and Coulomb accumulators.  pclasses holds the segments of all passes.
   struct Pass {
     int seg_begin, nseg, tasks, units, cap;
-    int ccap;  // chunks per precision, upper bound (tiles / chunk size + units)
+    int ccap;   // chunks per precision, upper bound (tiles / chunk size + units)
+    int order;  // bit 0 / 1: some class's FP64 / FP32 chunks in ket order
   };
   std::vector<PlanClass> pclasses;
   std::vector<int> seg_class;  // segment -> qclasses index
@@ -338,8 +340,9 @@ struct rchem_ctx {
       plan_off32, plan_lo, plan_mid, plan_hi, plan_tiles64, plan_tiles32, plan_ranges,
       plan_stats, plan_dmax, plan_scan_tmp, plan_chc64, plan_chc32, plan_choff64, plan_choff32,
       plan_chunks64, plan_chunks32;
-  // chunk order by ket window (RCHEM_CHUNK_ORDER=1): sort keys and buffers
-  bool chunk_order = false;
+  // LDS chunks in ket order (per class and precision from the tuning;
+  // RCHEM_CHUNK_ORDER=0 / 1 turns it off / on for every class): sort keys and buffers
+  int chunk_order_env = -1;
   int plan_key_bits = 0, plan_ccap = 0;
   size_t plan_sort_bytes = 0;
   DevBuf plan_chunks_raw, plan_keys_raw, plan_keys_sorted, plan_sort_tmp;
@@ -455,6 +458,17 @@ ClassTune tune_for(const rchem_ctx* ctx, int la, int lb, int lc, int ld, int fp3
   return d;
 }
 
+// Whether the class's LDS chunks of one precision are planned in ket order:
+// concurrently running workgroups then read the same ket pairs (their records
+// stay in L2) instead of every bra's whole ket range.  Measured on (H2O)_64
+// 6-31G: faster for the FP64 lists, slightly slower for the FP32 ones (whose
+// chunks are longer and whose Coulomb atomics then meet on the same kets).
+bool ket_order_of(const rchem_ctx* ctx, const ClassTune& t) {
+  if (ctx->chunk_order_env >= 0) return ctx->chunk_order_env != 0;
+  if (t.ket_order >= 0) return t.ket_order != 0;
+  return t.fp32 == 0;
+}
+
 bool tune_valid(int variant, int blocks, int chunk_tiles) {
   return variant >= 0 && variant < kNumLdsVariants && blocks >= 1 && blocks <= 65536 &&
          chunk_tiles >= 1 && chunk_tiles <= 4096;
@@ -470,7 +484,7 @@ void set_tune(rchem_ctx* ctx, const ClassTune& t) {
 }
 
 // Reads a tuning table: one line per (class, precision),
-//   la lb lc ld fp32 variant blocks chunk_tiles
+//   la lb lc ld fp32 variant blocks chunk_tiles [ket_order]
 // with '#' starting a comment; lines that do not parse or are out of range are
 // skipped.  Returns the number of entries taken.
 int load_tuning(rchem_ctx* ctx, const std::string& path) {
@@ -481,9 +495,10 @@ int load_tuning(rchem_ctx* ctx, const std::string& path) {
   while (std::fgets(line, sizeof(line), f)) {
     if (char* h = std::strchr(line, '#')) *h = '\0';
     ClassTune t;
-    if (std::sscanf(line, "%d %d %d %d %d %d %d %d", &t.la, &t.lb, &t.lc, &t.ld, &t.fp32,
-                    &t.variant, &t.blocks, &t.chunk_tiles) != 8)
-      continue;
+    const int got = std::sscanf(line, "%d %d %d %d %d %d %d %d %d", &t.la, &t.lb, &t.lc, &t.ld,
+                                &t.fp32, &t.variant, &t.blocks, &t.chunk_tiles, &t.ket_order);
+    if (got != 8 && got != 9) continue;
+    if (got == 9 && t.ket_order != 0 && t.ket_order != 1) continue;
     if (t.la < 0 || t.la > kMaxL || t.lb < 0 || t.lb > kMaxL || t.lc < 0 || t.lc > kMaxL ||
         t.ld < 0 || t.ld > kMaxL || (t.fp32 != 0 && t.fp32 != 1) ||
         !tune_valid(t.variant, t.blocks, t.chunk_tiles))
@@ -1129,14 +1144,14 @@ int prepare_plan(rchem_ctx* ctx, double tau) {
     }
   }
   // segments and passes
-  rchem_ctx::Pass cur{0, 0, 0, 0, 0, 0};
+  rchem_ctx::Pass cur{0, 0, 0, 0, 0, 0, 0};
   double pass_cap = 0.0, pass_ccap = 0.0;
   auto close_pass = [&]() {
     if (cur.nseg == 0) return;
     cur.cap = static_cast<int>(pass_cap) + 1;
     cur.ccap = static_cast<int>(std::min(pass_ccap, pass_cap)) + 1;
     ctx->passes.push_back(cur);
-    cur = rchem_ctx::Pass{static_cast<int>(ctx->pclasses.size()), 0, 0, 0, 0, 0};
+    cur = rchem_ctx::Pass{static_cast<int>(ctx->pclasses.size()), 0, 0, 0, 0, 0, 0};
     pass_cap = 0.0;
     pass_ccap = 0.0;
   };
@@ -1150,8 +1165,12 @@ int prepare_plan(rchem_ctx* ctx, double tau) {
     pc.task_begin = cur.tasks;
     pc.same = qc.X == qc.Y;
     pc.unit_begin = cur.units;
-    pc.chunk64 = tune_for(ctx, qc.la, qc.lb, qc.lc, qc.ld, 0).chunk_tiles;
-    pc.chunk32 = tune_for(ctx, qc.la, qc.lb, qc.lc, qc.ld, 1).chunk_tiles;
+    const ClassTune t64 = tune_for(ctx, qc.la, qc.lb, qc.lc, qc.ld, 0);
+    const ClassTune t32 = tune_for(ctx, qc.la, qc.lb, qc.lc, qc.ld, 1);
+    pc.chunk64 = t64.chunk_tiles;
+    pc.chunk32 = t32.chunk_tiles;
+    pc.order = (ket_order_of(ctx, t64) ? 1 : 0) | (ket_order_of(ctx, t32) ? 2 : 0);
+    cur.order |= pc.order;
     ctx->pclasses.push_back(pc);
     ctx->seg_class.push_back(c);
     ctx->seg_cap.push_back(static_cast<int>(cap));
@@ -1198,7 +1217,9 @@ int prepare_plan(rchem_ctx* ctx, double tau) {
   HIPC(ctx->plan_chunks32.ensure(sizeof(Chunk) * static_cast<size_t>(ctx->plan_cap)));
   ctx->plan_key_bits = 0;
   ctx->plan_ccap = 0;
-  if (ctx->chunk_order) {
+  bool any_order = false;
+  for (const auto& ps : ctx->passes) any_order = any_order || ps.order != 0;
+  if (any_order) {
     int nseg = 1;
     for (const auto& ps : ctx->passes) {
       nseg = std::max(nseg, ps.nseg);
@@ -1513,8 +1534,9 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
     pb.ranges = ctx->plan_ranges.as<int>() + 8 * static_cast<size_t>(ps.seg_begin);
     pb.scan_tmp = ctx->plan_scan_tmp.p;
     pb.scan_bytes = ctx->plan_scan_bytes;
-    if (ctx->plan_key_bits > 0) {
+    if (ctx->plan_key_bits > 0 && ps.order != 0) {
       pb.key_bits = ctx->plan_key_bits;
+      pb.sort_mask = ps.order;
       pb.ccap = std::min(ps.ccap, ctx->plan_ccap);
       pb.chunks_raw = ctx->plan_chunks_raw.as<Chunk>();
       pb.keys_raw = ctx->plan_keys_raw.as<unsigned>();
@@ -1689,7 +1711,7 @@ int rchem_ctx_create(int device, rchem_ctx** out) {
   if (const char* e = std::getenv("RCHEM_ERI_GENERIC")) ctx->force_generic = std::atoi(e) != 0;
   if (const char* e = std::getenv("RCHEM_ERI_STREAM")) ctx->eri_stream_mode = std::atoi(e);
   if (const char* e = std::getenv("RCHEM_PROGRESS")) ctx->progress = std::atoi(e) != 0;
-  if (const char* e = std::getenv("RCHEM_CHUNK_ORDER")) ctx->chunk_order = std::atoi(e) != 0;
+  if (const char* e = std::getenv("RCHEM_CHUNK_ORDER")) ctx->chunk_order_env = std::atoi(e) != 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
     g_create_error = "stream creation failed";
     delete ctx;
@@ -1928,6 +1950,26 @@ int rchem_set_class_tuning(rchem_ctx* ctx, int la, int lb, int lc, int ld, int f
   t.chunk_tiles = chunk_tiles;
   set_tune(ctx, t);
   ctx->plan_tau = -1.0;  // chunk sizes are part of the plan
+  return RCHEM_OK;
+}
+
+int rchem_set_class_ket_order(rchem_ctx* ctx, int la, int lb, int lc, int ld, int fp32,
+                              int ket_order) {
+  if (!ctx) return RCHEM_ERR_INVALID;
+  if (la < 0 || la > kMaxL || lb < 0 || lb > kMaxL || lc < 0 || lc > kMaxL || ld < 0 ||
+      ld > kMaxL || (fp32 != 0 && fp32 != 1) || ket_order < -1 || ket_order > 1)
+    return ctx->fail(RCHEM_ERR_INVALID, "rchem_set_class_ket_order: argument out of range");
+  ClassTune t = tune_for(ctx, la, lb, lc, ld, fp32);
+  t.ket_order = ket_order;
+  set_tune(ctx, t);
+  ctx->plan_tau = -1.0;  // the chunk order is part of the plan
+  return RCHEM_OK;
+}
+
+int rchem_get_class_ket_order(const rchem_ctx* ctx, int la, int lb, int lc, int ld, int fp32,
+                              int* ket_order) {
+  if (!ctx || !ket_order) return RCHEM_ERR_INVALID;
+  *ket_order = ket_order_of(ctx, tune_for(ctx, la, lb, lc, ld, fp32 ? 1 : 0)) ? 1 : 0;
   return RCHEM_OK;
 }
 

@@ -166,7 +166,7 @@ struct PlanClass {
   int same;                // X == Y: keep only ket <= bra
   int unit_begin;          // first bra unit; units = bra - bra_begin
   int chunk64, chunk32;    // tiles per LDS-digestion chunk, FP64 and FP32 lists
-  int pad;
+  int order;               // bit 0 / 1: FP64 / FP32 chunks ordered by first ket pair
 };
 
 struct PlanArgs {
@@ -211,6 +211,18 @@ struct PlanBuffers {
   int* ranges;       // 8 per class
   void* scan_tmp;
   size_t scan_bytes;
+  // chunk order by ket window (key_bits > 0): the chunks are written unsorted
+  // with a key (class, first ket pair / 64) and radix-sorted into chunks64 /
+  // chunks32, so that the workgroups running at one time read the same ket
+  // pairs (their records stay in L2) rather than every bra's whole ket range
+  int key_bits;      // 0: bra order (no sort)
+  int sort_mask;     // bit 0 / 1: sort the FP64 / FP32 chunk list
+  int ccap;          // chunks sorted per precision (upper bound of the chunk count)
+  Chunk* chunks_raw;
+  unsigned* keys_raw;
+  unsigned* keys_sorted;
+  void* sort_tmp;
+  size_t sort_bytes;
 };
 
 // Device-side state of one DIIS SCF (Pulay extrapolation over a ring of at most

@@ -1,5 +1,6 @@
 """Launch tuning of the Fock build: every compiled launch shape of the
-LDS-digestion kernel, workgroup count and chunk size gives the same J and K --
+LDS-digestion kernel, workgroup count, chunk size and chunk order (bra order
+or sorted by first ket pair) gives the same J and K --
 in FP64 to the order of floating-point atomics (1e-12 relative); in a mixed
 build the shapes also order the FP32 arithmetic differently, so there the bar
 is the FP32 quartets' share (threshold 1e-5 times FP32 rounding: 1e-9
@@ -46,11 +47,13 @@ def test_every_launch_shape_gives_the_same_jk(gpu_ctx):
             bar = (1e-9 if prec else 1e-12) * scale
             for v in range(nv):
                 blocks, chunk = [(2048, 64), (300, 7), (4096, 128)][v % 3]
+                order = (v // 3) % 2
                 print(f"case: launch shape {v}, {blocks} workgroups, chunks of {chunk}, "
-                      f"precision {prec}", flush=True)
+                      f"ket order {order}, precision {prec}", flush=True)
                 for c in SP_CLASSES:
                     for fp32 in (0, 1):
                         assert L.rchem_set_class_tuning(h, *c, fp32, v, blocks, chunk) == 0
+                        assert L.rchem_set_class_ket_order(h, *c, fp32, order) == 0
                 J = np.zeros((n, n)); K = np.zeros((n, n))
                 rb.JK_direct(J, K, B, D, o)
                 assert np.max(np.abs(J - J0)) < bar, (v, prec, np.max(np.abs(J - J0)) / scale)
@@ -64,10 +67,19 @@ def test_tuning_table_reader(gpu_ctx, tmp_path):
     h = gpu_ctx.h
     p = tmp_path / "t.txt"
     p.write_text("# comment\n1 0 0 0 0 3 1024 32  # kept\n1 0 0 0 1 99 1024 32\n"
-                 "0 0 0 0 1 2 5000 128\nnot a line\n2 2 2 2 0 1 0 64\n")
+                 "0 0 0 0 1 2 5000 128\nnot a line\n2 2 2 2 0 1 0 64\n"
+                 "1 1 0 0 0 1 512 16 0\n1 1 0 0 1 1 512 16 1\n1 1 1 0 0 1 512 16 2\n")
     cnt = ctypes.c_int()
     assert L.rchem_load_tuning(h, str(p).encode(), ctypes.byref(cnt)) == 0
-    assert cnt.value == 2
+    assert cnt.value == 4                              # the ket order 2 line is skipped
+    o = ctypes.c_int()
+    for cls, fp32, want in [((1, 1, 0, 0), 0, 0), ((1, 1, 0, 0), 1, 1),   # from the table
+                            ((1, 0, 0, 0), 0, 1), ((1, 0, 0, 0), 1, 0)]:  # defaults
+        assert L.rchem_get_class_ket_order(h, *cls, fp32, ctypes.byref(o)) == 0
+        assert o.value == want, (cls, fp32)
+    assert L.rchem_set_class_ket_order(h, 1, 0, 0, 0, 1, 1) == 0
+    assert L.rchem_get_class_ket_order(h, 1, 0, 0, 0, 1, ctypes.byref(o)) == 0 and o.value == 1
+    assert L.rchem_set_class_ket_order(h, 1, 0, 0, 0, 1, 2) == 1
     v, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     assert L.rchem_get_class_tuning(h, 1, 0, 0, 0, 0, ctypes.byref(v), ctypes.byref(b),
                                     ctypes.byref(c)) == 0
