@@ -41,6 +41,26 @@ __global__ void dmax_kernel(const double* __restrict__ D, size_t n2,
   if ((threadIdx.x & 63) == 0) atomicMax(out, m);
 }
 
+// The same over the diagonal blocks [f0_m, f0_m + nb_m)^2 of a multi-molecule
+// basis (the rest of D is never read).
+__global__ void dmax_blocks_kernel(const double* __restrict__ D, size_t n,
+                                   const int* __restrict__ f0, const int* __restrict__ nb,
+                                   unsigned long long* __restrict__ out) {
+  const int m = blockIdx.y, b = nb[m];
+  const size_t o = f0[m];
+  unsigned long long mx = 0;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < b * b; k += gridDim.x * blockDim.x) {
+    const double v = fabs(D[(o + k / b) * n + o + k % b]);
+    const unsigned long long bits = __double_as_longlong(v);
+    mx = bits > mx ? bits : mx;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long x = __shfl_xor(mx, off, 64);
+    mx = x > mx ? x : mx;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(out, mx);
+}
+
 // First index in [lo, hi) whose Q falls below `need` (Q non-increasing).
 __device__ __forceinline__ int first_below(const double* __restrict__ Q, int lo, int hi,
                                            double need) {
@@ -272,6 +292,14 @@ size_t plan_sort_bytes(int n) {
                                          static_cast<Chunk*>(nullptr), n, 0, 32) != hipSuccess)
     return 0;
   return bytes;
+}
+
+hipError_t dmax_blocks_launch(const double* D, size_t n, const int* f0, const int* nb, int nmol,
+                              int nmax, unsigned long long* out, hipStream_t s) {
+  if (nmol <= 0) return hipSuccess;
+  const unsigned gx = static_cast<unsigned>(min(64, (nmax * nmax + 255) / 256));
+  hipLaunchKernelGGL(dmax_blocks_kernel, dim3(gx, nmol), dim3(256), 0, s, D, n, f0, nb, out);
+  return hipGetLastError();
 }
 
 // Temporary-storage bytes of a scan over n + 1 entries.

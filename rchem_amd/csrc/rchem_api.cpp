@@ -32,6 +32,8 @@ namespace rchem {
 using EriLaunchFn = hipError_t (*)(const EriArgs&, int, hipStream_t);
 EriLaunchFn eri_launcher(int la, int lb, int lc, int ld, int mode, int bm, int fp32);
 hipError_t dmax_launch(const double* D, size_t n2, unsigned long long* out, hipStream_t s);
+hipError_t dmax_blocks_launch(const double* D, size_t n, const int* f0, const int* nb, int nmol,
+                              int nmax, unsigned long long* out, hipStream_t s);
 size_t plan_scan_bytes(int ntasks);
 size_t plan_sort_bytes(int n);
 int plan_class_bits(int nclasses);
@@ -1487,7 +1489,12 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
   HIPC(fill_launch(bf.stats, 4 * static_cast<size_t>(std::max(nc, 1)), 0.0, ctx->stream));
   if (mixed) {
     HIPC(fill_launch(bf.dmax, 1, 0.0, ctx->stream));
-    HIPC(dmax_launch(dD, n2, reinterpret_cast<unsigned long long*>(bf.dmax), ctx->stream));
+    if (ctx->nmol > 1 && ctx->mol_nmax > 0)  // a multi-molecule basis: its diagonal blocks
+      HIPC(dmax_blocks_launch(dD, static_cast<size_t>(ctx->nbf), ctx->mol_f0_d.as<int>(),
+                              ctx->mol_nb_d.as<int>(), ctx->nmol, ctx->mol_nmax,
+                              reinterpret_cast<unsigned long long*>(bf.dmax), ctx->stream));
+    else
+      HIPC(dmax_launch(dD, n2, reinterpret_cast<unsigned long long*>(bf.dmax), ctx->stream));
   }
   HIPC(fill_launch(bf.jp, std::max<size_t>(ctx->jp_size, 1), 0.0, ctx->stream));
   // the pairs' density blocks in the Jp layout, for the LDS kernels' Coulomb sums
@@ -2836,7 +2843,14 @@ extern "C" {
 
 namespace {
 
-constexpr int kBatchMaxNbf = 8192;    // basis functions per chunk of molecules
+// Basis functions per chunk of molecules (default; RCHEM_BATCH_NBF sets it, up
+// to kBatchMaxNbfLimit, where n^2 doubles per matrix stay below 2^31).  A chunk
+// is one multi-molecule basis with dense n x n matrices (~7 of them alive while
+// it iterates, 3 more during its set-up): 24576 functions keep that under
+// 48 GB, while larger chunks put more molecules into every launch (the
+// 512-molecule STO-3G batch: 8192 -> 9.5 s, 16384 -> 8.6 s, one chunk -> 8.1 s).
+constexpr int kBatchMaxNbf = 24576;
+constexpr int kBatchMaxNbfLimit = 40000;
 constexpr double kBatchPadF = 1.0e4;  // Fock diagonal of a padded block's unused rows
 
 // One chunk of the batched SCF: molecules [m0, m1) of the caller's lists as one
@@ -2976,6 +2990,9 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
   const rocblas_int nn2 = static_cast<rocblas_int>(n2);
   HIPC(hipMemcpyAsync(dH.p, dT.p, n2 * sizeof(double), hipMemcpyDeviceToDevice, st));
   BLASC(rocblas_daxpy(hb, nn2, &one, dV.as<double>(), 1, dH.as<double>(), 1));
+  HIPC(hipStreamSynchronize(st));
+  dT.release();  // H = T + V is all the iterations need of them
+  dV.release();
   // padded per-molecule batches.  The batched linear algebra runs over the
   // molecules still iterating ("slots", in molecule order): when molecules
   // retire, the slots' per-molecule state (S, X, the DIIS history) is
@@ -3020,6 +3037,8 @@ int batch_chunk(rchem_ctx* ctx, int m0, int m1, const int* natom, const int* Z,
   HIPC(batch_inv_sqrt_cols(dT1.as<double>(), dw.as<double>(), nm, Bs, dW.as<double>(), st));
   BLASC(gemm_b(rocblas_operation_none, rocblas_operation_transpose, dW.as<double>(),
                dT1.as<double>(), dX.as<double>()));
+  HIPC(hipStreamSynchronize(st));
+  dS.release();  // its blocks live on in dSp
   // density from a padded Fock batch: F' = X^T F X, eigh, C = X C', D = C_occ C_occ^T
   auto density_b = [&](const double* Fb) -> int {
     BLASC(gemm_b(rocblas_operation_none, rocblas_operation_none, Fb, dX.as<double>(), dT1.as<double>()));
@@ -3372,10 +3391,19 @@ int rchem_batch_scf(rchem_ctx* ctx, int nmol, const int* natom, const int* Z, co
     }
   }
   std::vector<rchem_scf_result> res_s(nmol);
+  int chunk_nbf = kBatchMaxNbf;
+  if (const char* e = std::getenv("RCHEM_BATCH_NBF"))
+    chunk_nbf = std::min(std::max(std::atoi(e), 1), kBatchMaxNbfLimit);
+  // chunks of about equal size: the fewest chunks of at most chunk_nbf
+  // functions, each filled to the mean
+  long long total_nbf = 0;
+  for (int m = 0; m < nmol; ++m) total_nbf += nbf[m];
+  const long long nchunks = std::max<long long>(1, (total_nbf + chunk_nbf - 1) / chunk_nbf);
+  const long long target = (total_nbf + nchunks - 1) / nchunks;
   int status = RCHEM_OK, m0 = 0, aoff = 0;
   while (m0 < nmol) {
     int m1 = m0, tot = 0, aend = aoff;
-    while (m1 < nmol && (m1 == m0 || tot + nbf[order[m1]] <= kBatchMaxNbf)) {
+    while (m1 < nmol && (m1 == m0 || (tot < target && tot + nbf[order[m1]] <= chunk_nbf))) {
       tot += nbf[order[m1]];
       aend += natom_s[m1];
       ++m1;

@@ -3,6 +3,8 @@ quartets in one multi-molecule basis, batched SCF algebra) must give each
 molecule the energy of its own single-molecule SCF on the same device path
 (bar 1e-8 Eh), in plain rchem iterations and with DIIS, in one chunk and over
 several chunks of molecules."""
+import os
+
 import numpy as np
 import pytest
 
@@ -60,10 +62,20 @@ def test_batch_several_chunks_and_mixed(gpu_ctx):
     mols = random_molecules(count=120, seed=77, nmin=10, nmax=30)
     damp = dict(damping=0.3, damping_until=1e-2)
     _say(f"case: batch of {len(mols)} molecules over several chunks, FP64 and mixed, damped DIIS")
-    r64 = rb.batch_scf(mols, "STO-3G", thresh_e=1e-10, diis=8, max_iter=128, ctx=gpu_ctx,
-                       **damp)
-    rmx = rb.batch_scf(mols, "STO-3G", thresh_e=1e-10, diis=8, max_iter=128, ctx=gpu_ctx,
-                       opts=default_opts(precision=PREC_MIXED), **damp)
+    # chunks of at most 2048 functions (the library reads the limit per call):
+    # several chunks, each with its own retirement and slot compaction
+    old = os.environ.get("RCHEM_BATCH_NBF")
+    os.environ["RCHEM_BATCH_NBF"] = "2048"
+    try:
+        r64 = rb.batch_scf(mols, "STO-3G", thresh_e=1e-10, diis=8, max_iter=128, ctx=gpu_ctx,
+                           **damp)
+        rmx = rb.batch_scf(mols, "STO-3G", thresh_e=1e-10, diis=8, max_iter=128, ctx=gpu_ctx,
+                           opts=default_opts(precision=PREC_MIXED), **damp)
+    finally:
+        if old is None:
+            del os.environ["RCHEM_BATCH_NBF"]
+        else:
+            os.environ["RCHEM_BATCH_NBF"] = old
     ok = [i for i in range(len(mols)) if r64[i].converged and rmx[i].converged]
     _say(f"converged in both: {len(ok)} of {len(mols)}")
     assert len(ok) >= 0.9 * len(mols)
