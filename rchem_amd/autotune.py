@@ -11,7 +11,10 @@ per-class HIP-event timings:
   2. the chunk order: chunks sorted by their first ket pair (workgroups
      running together share ket records in L2) or left in bra order;
   3. persistent workgroup counts (512 ... 8192) at each class's best shape;
-  4. chunk sizes (16 ... 256 tiles) at the best shape and count.
+  4. chunk sizes (16 ... 256 tiles) at the best shape and count, then the
+     chunk order once more; a precision whose ket-ordered classes together
+     save less than sorting its chunk list costs (SORT_MS) stays in bra
+     order.
 A choice is kept for a class only where it beats the class's best so far by
 more than the measured noise (1 %).  The FP64 kernels are timed in the
 all-FP64 build and in the FP64 part of the mixed build and tuned on the sum;
@@ -41,6 +44,10 @@ DEFAULT_OUT = os.path.join(HERE, "data", "tuning_gfx950.txt")
 BLOCKS = [512, 1024, 2048, 4096, 8192]
 CHUNKS = [16, 32, 64, 128, 256]
 NOISE = 0.01
+# Sorting one precision's chunk list costs the planner ~0.13 ms per build
+# ((H2O)_64): the kernels a precision's ket order speeds up together must save
+# more than that, or that precision stays in bra order.
+SORT_MS = 0.15
 
 
 def _log(msg):
@@ -147,6 +154,26 @@ def tune(tuner):
     sweep("ket order", (0, 1), lambda cur, o: (cur["variant"], cur["blocks"], cur["chunk"], o))
     sweep("workgroups", BLOCKS, lambda cur, b: (cur["variant"], b, cur["chunk"], cur["order"]))
     sweep("chunk", CHUNKS, lambda cur, c: (cur["variant"], cur["blocks"], c, cur["order"]))
+    # the order again at the chosen chunk size (a chunk's length sets how many
+    # kets its workgroup walks through)
+    sweep("ket order", (0, 1), lambda cur, o: (cur["variant"], cur["blocks"], cur["chunk"], o))
+    # a precision whose ket-ordered classes save less than its list's sort
+    # costs goes back to bra order
+    t_bra = {}
+    for k in keys:
+        tuner.set(k, best[k]["variant"], best[k]["blocks"], best[k]["chunk"], 0)
+    t_bra = tuner.time_all()
+    for fp32 in (0, 1):
+        on = [k for k in keys if k[4] == fp32 and best[k]["order"] == 1 and k in t_bra]
+        saved = sum(t_bra[k] - best[k]["ms"] for k in on)
+        if on and saved < SORT_MS:
+            _log(f"ket order of the {'FP32' if fp32 else 'FP64'} lists saves {saved:.3f} ms "
+                 f"< {SORT_MS} ms: bra order")
+            for k in on:
+                best[k]["order"] = 0
+                best[k]["ms"] = t_bra[k]
+    for k in keys:
+        tuner.set(k, best[k]["variant"], best[k]["blocks"], best[k]["chunk"], best[k]["order"])
     final = tuner.time_all()
     return base, best, final
 
