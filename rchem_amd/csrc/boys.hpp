@@ -78,7 +78,9 @@ __device__ __forceinline__ void boys_exact(R T, const R* __restrict__ table, R* 
   if (T < R(kBoysTMax)) {
     const int k = static_cast<int>(T * R(kBoysInvH) + R(0.5));
     const R d = R(k) * (R(1) / R(kBoysInvH)) - T;  // T0 - T, |d| <= 1/16
-    const R* __restrict__ c = table + k * kBoysTerms;
+    // 0 <= k < kBoysRows: a 24-bit multiply (full rate; a 32-bit one is
+    // quarter rate)
+    const R* __restrict__ c = table + __umul24(static_cast<unsigned>(k), kBoysTerms);
     if constexpr (M == 1) {
       // F_0(T0 - d) = sum_j c_j d^j (c_j = F_j(T0) / j!); F_1 = d/dd of it
       R p = c[K - 1], q = R(0);
@@ -147,7 +149,7 @@ __device__ __forceinline__ void boys_exact_select(R T, const R* __restrict__ tab
     const R Tn = near ? T : R(0);
     const int k = static_cast<int>(Tn * R(kBoysInvH) + R(0.5));
     const R d = R(k) * (R(1) / R(kBoysInvH)) - Tn;
-    const R* __restrict__ c = table + k * kBoysTerms;
+    const R* __restrict__ c = table + __umul24(static_cast<unsigned>(k), kBoysTerms);
     const R rt = fast_rsqrt(near ? R(1) : T);
     const R f0 = R(0.88622692545275801365) * rt;
     if constexpr (M == 1) {
