@@ -292,6 +292,12 @@ def stream_vrr(la, lb, lc, ld):
 # in one part.
 STREAM_LIVE = 1000
 STREAM_L = 5  # classes from this total angular momentum get a streaming kernel
+# L = 5 classes on the per-lane kernel (the streaming kernel is slower for
+# them) that also get LDS digestion: 300-shell s/p/d microbenchmark, ms per
+# FP64 build, global atomics -> LDS digestion: (dp|ds) 52.5 -> 28.5, (dd|ps)
+# 17.3 -> 12.4.  (dp|pp) and (dd|ds) measured unchanged (127, 54 ms: their
+# spills, not the atomics, bound them) and keep global atomics.
+LDS_PER_LANE_BEYOND_4 = {"2120", "2210"}
 
 
 def pair_class(l1, l2):
@@ -311,7 +317,8 @@ def classes(lmax):
 def tu_source(la, lb, lc, ld):
     """Kernel variants of one class.  s/p classes carry everything (both Boys
     modes, FP32, LDS digestion); d classes the production and parity set, with
-    LDS digestion up to L = 4."""
+    LDS digestion up to L = 4 and for the L = 5, 6 classes the per-lane kernel
+    serves (LDS_PER_LANE_BEYOND_4)."""
     key = f"{la}{lb}{lc}{ld}"
     sp = max(la, lb, lc, ld) <= 1
     A = f"{la}, {lb}, {lc}, {ld}"
@@ -346,8 +353,8 @@ def tu_source(la, lb, lc, ld):
         L += [f"    case {v}: return eri_jk_lds_variant<{A}, {v}>(bm, fp32);"
               for v in range(N_LDS_VARIANTS)]
         L += ["    default: return nullptr;", "  }"]
-    elif la + lb + lc + ld <= 4:
-        # d classes whose state fits a lane: LDS digestion for the production
+    elif la + lb + lc + ld <= 4 or key in LDS_PER_LANE_BEYOND_4:
+        # d classes on the per-lane kernel: LDS digestion for the production
         # Boys, default shape only (compile time)
         L += [f"  return v == 0 ? eri_jk_lds_variant<{A}, 0>(bm, fp32) : nullptr;"]
     else:

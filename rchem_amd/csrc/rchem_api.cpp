@@ -822,6 +822,20 @@ int find_program(rchem_ctx* ctx, int la, int lb, int lc, int ld, int grad,
 
 // Kernel for class (la lb|lc ld) in `mode`; `out` stays invalid when neither
 // engine serves the class (angular momentum beyond the Boys grid's orders).
+// Whether the class runs on the streaming kernel (no per-lane accumulators;
+// it takes the HRR and output tables from the class's wave-cooperative
+// program) where it is the faster engine, measured on the 300-shell s/p/d
+// microbenchmark (ms per FP64 build, streaming vs per-lane or
+// wave-cooperative): (dp|dp) 138 vs 246, (dd|pp) 78 vs 150, (dd|dp) 84 vs 470,
+// (dd|dd) 37 vs 113; the per-lane kernels keep (dp|pp) 125 vs 174, (dp|ds) 52
+// vs 142, (dd|ps) 17 vs 72, (dd|ds) 53 vs 62.  RCHEM_ERI_STREAM=0: never, 2:
+// every class it serves (also in place of LDS digestion).
+bool stream_preferred(const rchem_ctx* ctx, int la, int lb, int lc, int ld) {
+  const int key = ((la * 10 + lb) * 10 + lc) * 10 + ld;
+  return ctx->eri_stream_mode == 2 ||
+         (ctx->eri_stream_mode == 1 && (key == 2121 || key == 2211 || key == 2221 || key == 2222));
+}
+
 int class_kernel(rchem_ctx* ctx, int la, int lb, int lc, int ld, int mode, int bm, int fp32,
                  ClassKernel* out) {
   *out = ClassKernel{};
@@ -830,18 +844,7 @@ int class_kernel(rchem_ctx* ctx, int la, int lb, int lc, int ld, int mode, int b
   out->lds_limit = ctx->lds_block;
   if (la + lb + lc + ld > kBoysMaxM) return RCHEM_OK;
   if (!ctx->force_generic) {
-    // the streaming kernel (no per-lane accumulators; it takes the HRR and
-    // output tables from the class's wave-cooperative program) where it is the
-    // faster engine, measured on the 300-shell s/p/d microbenchmark (ms per
-    // FP64 build, streaming vs per-lane or wave-cooperative): (dp|dp) 138 vs
-    // 246, (dd|pp) 78 vs 150, (dd|dp) 84 vs 470, (dd|dd) 37 vs 113; the
-    // per-lane kernels keep (dp|pp) 125 vs 174, (dp|ds) 52 vs 142, (dd|ps) 17
-    // vs 72, (dd|ds) 53 vs 62.  RCHEM_ERI_STREAM=0: never, 2: every class it
-    // serves.
-    const int key = ((la * 10 + lb) * 10 + lc) * 10 + ld;
-    const bool prefer = ctx->eri_stream_mode == 2 ||
-                        (ctx->eri_stream_mode == 1 &&
-                         (key == 2121 || key == 2211 || key == 2221 || key == 2222));
+    const bool prefer = stream_preferred(ctx, la, lb, lc, ld);
     const EriStreamFn sfn =
         prefer && bm == kBoysExact ? eri_stream_launcher(la, lb, lc, ld, mode) : nullptr;
     if (sfn) {
@@ -1574,7 +1577,9 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
       // kernel's own arrays (Boys block, bra primitive pairs: < 12 KB)
       const size_t lds = sizeof(double) * (ncart(qc.la) + ncart(qc.lb)) * static_cast<size_t>(a.kw);
       const bool lds_ok = lds + kLdsStaticReserve <= ctx->lds_block && ctx->lds_digestion &&
-                          !ctx->force_generic;
+                          !ctx->force_generic &&
+                          !(qc.la + qc.lb + qc.lc + qc.ld >= 5 &&
+                            stream_preferred(ctx, qc.la, qc.lb, qc.lc, qc.ld));
       int* ranges = pb.ranges + 8 * (sg - ps.seg_begin);
       for (int prec = 0; prec < (mixed ? 2 : 1); ++prec) {
         const int bm = prec ? kBoysExact : o->boys_mode;
