@@ -25,8 +25,23 @@
 
 namespace rchem {
 
-__global__ void dmax_kernel(const double* __restrict__ D, size_t n2,
-                            unsigned long long* __restrict__ out) {
+// max over the block (256 threads), the result in thread 0
+__device__ __forceinline__ unsigned long long block_max_u64(unsigned long long m) {
+  __shared__ unsigned long long part[4];
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(m, off, 64);
+    m = o > m ? o : m;
+  }
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int w = 1; w < 4; ++w) m = part[w] > m ? part[w] : m;
+  return m;
+}
+
+// One atomic per workgroup (of at most 256): all of them meet on one address.
+__global__ __launch_bounds__(256) void dmax_kernel(const double* __restrict__ D, size_t n2,
+                                                   unsigned long long* __restrict__ out) {
   unsigned long long m = 0;
   for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n2;
        i += static_cast<size_t>(gridDim.x) * blockDim.x) {
@@ -34,11 +49,8 @@ __global__ void dmax_kernel(const double* __restrict__ D, size_t n2,
     const unsigned long long b = __double_as_longlong(v);  // non-negative: bits order = value order
     m = b > m ? b : m;
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned long long o = __shfl_xor(m, off, 64);
-    m = o > m ? o : m;
-  }
-  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+  m = block_max_u64(m);
+  if (threadIdx.x == 0) atomicMax(out, m);
 }
 
 // The same over the diagonal blocks [f0_m, f0_m + nb_m)^2 of a multi-molecule
@@ -54,11 +66,8 @@ __global__ void dmax_blocks_kernel(const double* __restrict__ D, size_t n,
     const unsigned long long bits = __double_as_longlong(v);
     mx = bits > mx ? bits : mx;
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned long long x = __shfl_xor(mx, off, 64);
-    mx = x > mx ? x : mx;
-  }
-  if ((threadIdx.x & 63) == 0) atomicMax(out, mx);
+  mx = block_max_u64(mx);
+  if (threadIdx.x == 0) atomicMax(out, mx);
 }
 
 // First index in [lo, hi) whose Q falls below `need` (Q non-increasing).
@@ -128,7 +137,10 @@ __global__ void plan_kernel(const PlanArgs a) {
     p64 = q64 * npp;
     p32 = q32 * npp;
   }
-  // statistics: one atomic per wave when the wave's tasks share a class
+  // statistics: one atomic per wave when the wave's tasks share a class, into
+  // the wave's replica
+  double* const stats = a.stats + static_cast<size_t>((t >> 6) & (kPlanStatsReplicas - 1)) *
+                                      a.stats_stride;
   const int c0 = __shfl(c, 0, 64);
   const int lastv = __shfl(valid ? 1 : 0, 63, 64);
   const int c63 = __shfl(c, 63, 64);
@@ -138,12 +150,12 @@ __global__ void plan_kernel(const PlanArgs a) {
     p64 = wave_sum_d(p64);
     p32 = wave_sum_d(p32);
     if ((threadIdx.x & 63) == 0) {
-      double* st = a.stats + 4 * c0;
+      double* st = stats + 4 * c0;
       if (q64 > 0) { unsafeAtomicAdd(st + 0, q64); unsafeAtomicAdd(st + 2, p64); }
       if (q32 > 0) { unsafeAtomicAdd(st + 1, q32); unsafeAtomicAdd(st + 3, p32); }
     }
   } else if (valid) {
-    double* st = a.stats + 4 * c;
+    double* st = stats + 4 * c;
     if (q64 > 0) { unsafeAtomicAdd(st + 0, q64); unsafeAtomicAdd(st + 2, p64); }
     if (q32 > 0) { unsafeAtomicAdd(st + 1, q32); unsafeAtomicAdd(st + 3, p32); }
   }
@@ -269,7 +281,7 @@ __global__ void ranges_kernel(const PlanArgs a, const int* __restrict__ off64,
 
 hipError_t dmax_launch(const double* D, size_t n2, unsigned long long* out, hipStream_t s) {
   size_t blocks = (n2 + 255) / 256;
-  if (blocks > 2048) blocks = 2048;
+  if (blocks > 512) blocks = 512;
   if (blocks == 0) blocks = 1;
   hipLaunchKernelGGL(dmax_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, D, n2,
                      out);

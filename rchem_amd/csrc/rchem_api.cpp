@@ -1273,7 +1273,8 @@ int prepare_plan(rchem_ctx* ctx, double tau) {
   HIPC(ctx->plan_tiles64.ensure(sizeof(Tile) * static_cast<size_t>(ctx->plan_cap)));
   HIPC(ctx->plan_tiles32.ensure(sizeof(Tile) * static_cast<size_t>(ctx->plan_cap)));
   HIPC(ctx->plan_ranges.ensure(sizeof(int) * 8 * std::max<size_t>(ctx->pclasses.size(), 1)));
-  HIPC(ctx->plan_stats.ensure(sizeof(double) * 4 * std::max<size_t>(ctx->pclasses.size(), 1)));
+  HIPC(ctx->plan_stats.ensure(sizeof(double) * 4 * kPlanStatsReplicas *
+                              std::max<size_t>(ctx->pclasses.size(), 1)));
   HIPC(ctx->plan_dmax.ensure(sizeof(double)));
   ctx->plan_scan_bytes = std::max(plan_scan_bytes(tasks), plan_scan_bytes(units));
   HIPC(ctx->plan_scan_tmp.ensure(std::max<size_t>(ctx->plan_scan_bytes, 8)));
@@ -1431,7 +1432,7 @@ size_t jk_buffer_doubles(const rchem_ctx* ctx) {
   const size_t n2 = static_cast<size_t>(ctx->nbf) * ctx->nbf;
   const size_t jp = std::max<size_t>(ctx->jp_size, 1);
   const size_t nc = std::max<size_t>(ctx->pclasses.size(), 1);
-  return 2 * up(n2) + 2 * up(jp) + up(4 * nc) + up(1);
+  return 2 * up(n2) + 2 * up(jp) + up(4 * nc * kPlanStatsReplicas) + up(1);
 }
 
 // The build's accumulators carved out of one device range of
@@ -1452,7 +1453,7 @@ JkBuffers jk_buffers_in(const rchem_ctx* ctx, double* base) {
   b.dp = p;
   p += up(jp);
   b.stats = p;
-  p += up(4 * nc);
+  p += up(4 * nc * kPlanStatsReplicas);
   b.dmax = p;
   return b;
 }
@@ -1489,7 +1490,8 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
     HIPC(fill_launch(bf.Kh, n2, 0.0, ctx->stream));
   }
   const int nc = static_cast<int>(ctx->pclasses.size());
-  HIPC(fill_launch(bf.stats, 4 * static_cast<size_t>(std::max(nc, 1)), 0.0, ctx->stream));
+  HIPC(fill_launch(bf.stats, 4 * kPlanStatsReplicas * static_cast<size_t>(std::max(nc, 1)), 0.0,
+                   ctx->stream));
   if (mixed) {
     HIPC(fill_launch(bf.dmax, 1, 0.0, ctx->stream));
     if (ctx->nmol > 1 && ctx->mol_nmax > 0)  // a multi-molecule basis: its diagonal blocks
@@ -1523,6 +1525,7 @@ int jk_device(rchem_ctx* ctx, const double* dD, double* dJ, double* dK, const rc
     pa.mid = ctx->plan_mid.as<int>();
     pa.hi = ctx->plan_hi.as<int>();
     pa.stats = bf.stats + 4 * static_cast<size_t>(ps.seg_begin);
+    pa.stats_stride = 4 * std::max(nc, 1);
     if (!ctx->shell_mol.empty()) {
       pa.segs = ctx->plan_segs_d.as<int2>();
       pa.nmol = ctx->nmol;
@@ -2039,6 +2042,20 @@ static const double* last_stats_src(const rchem_ctx* ctx) {
   return ctx->last_stats_dev ? ctx->last_stats_dev : ctx->plan_stats.as<double>();
 }
 
+// The last build's per-class statistics {quartets64, quartets32, prim64,
+// prim32} (4 per segment): the sum of the planner's replicas.
+static int read_last_stats(rchem_ctx* ctx, std::vector<double>* out) {
+  const size_t per = 4 * ctx->pclasses.size();
+  std::vector<double> all(per * kPlanStatsReplicas);
+  HIPC(hipMemcpyAsync(all.data(), last_stats_src(ctx), sizeof(double) * all.size(),
+                      hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  out->assign(per, 0.0);
+  for (size_t r = 0; r < static_cast<size_t>(kPlanStatsReplicas); ++r)
+    for (size_t i = 0; i < per; ++i) (*out)[i] += all[r * per + i];
+  return RCHEM_OK;
+}
+
 int rchem_reset_class_timings(rchem_ctx* ctx) {
   if (!ctx) return RCHEM_ERR_INVALID;
   TRY(set_device(ctx));
@@ -2053,10 +2070,8 @@ int rchem_jk_class_timings(rchem_ctx* ctx, rchem_class_timing* out, int max, int
   TRY(collect_timers(ctx));
   if (ctx->last_mixed && !ctx->pclasses.empty()) {
     // work per precision of the last build, counted by the planner on the device
-    std::vector<double> dv(4 * ctx->pclasses.size());
-    HIPC(hipMemcpyAsync(dv.data(), last_stats_src(ctx), sizeof(double) * dv.size(),
-                        hipMemcpyDeviceToHost, ctx->stream));
-    HIPC(hipStreamSynchronize(ctx->stream));
+    std::vector<double> dv;
+    TRY(read_last_stats(ctx, &dv));
     for (auto& t : ctx->timers) {
       if (t.cls < 0 || t.cls >= static_cast<int>(ctx->qclasses.size())) continue;
       const QClass& qc = ctx->qclasses[t.cls];
@@ -2098,10 +2113,8 @@ int rchem_jk_last_stats(rchem_ctx* ctx, rchem_jk_stats* st) {
   if (ctx->last_mixed && !ctx->pclasses.empty()) {
     // the planner counted FP64 and FP32 quartets of the last build on the device
     TRY(set_device(ctx));
-    std::vector<double> dv(4 * ctx->pclasses.size());
-    HIPC(hipMemcpyAsync(dv.data(), last_stats_src(ctx), sizeof(double) * dv.size(),
-                        hipMemcpyDeviceToHost, ctx->stream));
-    HIPC(hipStreamSynchronize(ctx->stream));
+    std::vector<double> dv;
+    TRY(read_last_stats(ctx, &dv));
     double q32 = 0.0;
     for (size_t c = 0; c < ctx->pclasses.size(); ++c) q32 += dv[4 * c + 1];
     st->quartets_fp32 = q32;

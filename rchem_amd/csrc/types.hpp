@@ -158,6 +158,12 @@ struct GenericProgram {
 using EriStreamFn = hipError_t (*)(const EriArgs&, const GenericProgram&, int blocks,
                                    hipStream_t, size_t lds_limit);
 
+// The planner's per-class statistics are summed into this many copies (a
+// wave adds into copy wave % kPlanStatsReplicas), so that the adds of the
+// ~10^4 waves of a build do not all queue on the same few addresses; readers
+// sum the copies.
+constexpr int kPlanStatsReplicas = 64;
+
 // One quartet class as the device planner sees it.
 struct PlanClass {
   int bra_begin, bra_end;  // bra pairs (pair class X)
@@ -184,7 +190,9 @@ struct PlanArgs {
   int* lo;                 // per task ket range [lo, mid) in FP64, [mid, hi) in FP32
   int* mid;
   int* hi;
-  double* stats;           // per class: {quartets64, quartets32, prim64, prim32}
+  double* stats;           // per class: {quartets64, quartets32, prim64, prim32}, in
+                           // kPlanStatsReplicas replicas stats_stride doubles apart
+  int stats_stride;
   int nunits;              // bra units (class, bra pair)
   // several molecules in one basis: kets restricted to the bra's molecule,
   // segs[(bucket slot) * nmol + molecule] = its pairs [begin, end) in the bucket
