@@ -11,10 +11,9 @@ per-class HIP-event timings:
   2. the chunk order: chunks sorted by their first ket pair (workgroups
      running together share ket records in L2) or left in bra order;
   3. persistent workgroup counts (512 ... 8192) at each class's best shape;
-  4. chunk sizes (16 ... 256 tiles) at the best shape and count, then the
-     chunk order once more; a precision whose ket-ordered classes together
-     save less than sorting its chunk list costs (SORT_MS) stays in bra
-     order.
+  4. chunk sizes (16 ... 256 tiles) and chunk order together, at the best
+     shape and count; a precision whose ket-ordered classes together save
+     less than sorting its chunk list costs (SORT_MS) stays in bra order.
 A choice is kept for a class only where it beats the class's best so far by
 more than the measured noise (1 %).  The FP64 kernels are timed in the
 all-FP64 build and in the FP64 part of the mixed build and tuned on the sum;
@@ -153,10 +152,10 @@ def tune(tuner):
               lambda cur, v, c=chunk: (v, cur["blocks"], c, cur["order"]))
     sweep("ket order", (0, 1), lambda cur, o: (cur["variant"], cur["blocks"], cur["chunk"], o))
     sweep("workgroups", BLOCKS, lambda cur, b: (cur["variant"], b, cur["chunk"], cur["order"]))
-    sweep("chunk", CHUNKS, lambda cur, c: (cur["variant"], cur["blocks"], c, cur["order"]))
-    # the order again at the chosen chunk size (a chunk's length sets how many
-    # kets its workgroup walks through)
-    sweep("ket order", (0, 1), lambda cur, o: (cur["variant"], cur["blocks"], cur["chunk"], o))
+    # chunk length and order together (a chunk's length sets how many kets its
+    # workgroup walks through, and so how much ket order can share)
+    sweep("chunk, order", [(c, o) for o in (0, 1) for c in CHUNKS],
+          lambda cur, co: (cur["variant"], cur["blocks"], co[0], co[1]))
     # a precision whose ket-ordered classes save less than its list's sort
     # costs goes back to bra order
     t_bra = {}
