@@ -213,6 +213,14 @@ int rchem_set_class_ket_order(rchem_ctx *ctx, int la, int lb, int lc, int ld, in
                               int ket_order);
 int rchem_get_class_ket_order(const rchem_ctx *ctx, int la, int lb, int lc, int ld, int fp32,
                               int *ket_order);
+/* The batched SCF's table: rchem_batch_scf uses it in place of the context's
+ * while it runs (multi-molecule bases of small molecules favour other launch
+ * shapes).  A context loads data/tuning_gfx950_batch.txt beside the library
+ * (RCHEM_TUNING_BATCH_FILE names another; empty: none), written by
+ * `python -m rchem_amd.autotune --workload batch512`; without one the batch
+ * uses the context's table.  NULL or "": none; *entries (may be NULL)
+ * receives the count. */
+int rchem_load_batch_tuning(rchem_ctx *ctx, const char *path, int *entries);
 /* Replaces the context's table by the file at path (NULL or "": no table,
  * every class at its defaults); *entries (may be NULL) receives the count. */
 int rchem_load_tuning(rchem_ctx *ctx, const char *path, int *entries);

@@ -96,6 +96,8 @@ SIGNATURES = {
     "rchem_set_class_ket_order": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 6),
     "rchem_get_class_ket_order": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 5 +
                                   [ctypes.POINTER(ctypes.c_int)]),
+    "rchem_load_batch_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p,
+                                                ctypes.POINTER(ctypes.c_int)]),
     "rchem_get_class_tuning": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 5 +
                                [_ip, _ip, _ip]),
     "rchem_load_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, _ip]),

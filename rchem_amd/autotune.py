@@ -22,10 +22,12 @@ rchem_load_tuning reads (data/tuning_gfx950.txt beside the library is the
 default every context loads), and a JSON report of default vs tuned ms per
 class goes to stdout.
 
-    python -m rchem_amd.autotune [--workload h2o64|h2o32|eri300] [--out PATH]
+    python -m rchem_amd.autotune [--workload h2o64|h2o32|eri300|batch512] [--out PATH]
 
 The workload is a Fock build on the density the bench uses (core-Hamiltonian
-guess) so that the mixed-precision split is the bench's.
+guess) so that the mixed-precision split is the bench's; batch512 times the
+first iterations of the 512-molecule batched SCF instead and writes the
+batched SCF's own table.
 """
 from __future__ import annotations
 
@@ -40,6 +42,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_OUT = os.path.join(HERE, "data", "tuning_gfx950.txt")
+DEFAULT_OUT_BATCH = os.path.join(HERE, "data", "tuning_gfx950_batch.txt")
 BLOCKS = [512, 1024, 2048, 4096, 8192]
 CHUNKS = [16, 32, 64, 128, 256]
 NOISE = 0.01
@@ -121,10 +124,43 @@ class Tuner:
         return out.value
 
 
+class BatchTuner(Tuner):
+    """The batched SCF's Fock builds: the first iterations of rchem_batch_scf
+    over the molecules (multi-molecule bases), class kernels timed over all
+    their builds; the batch runs on the context's table while it tunes."""
+
+    def __init__(self, ctx, molecules, iters):
+        from ._lib import lib
+        self.L = lib()
+        self.ctx = ctx
+        self.mols = molecules
+        self.iters = iters
+        nv = 0
+        while self.L.rchem_lds_variant_info(nv, None, None, None, None, None) == 0:
+            nv += 1
+        self.nvariants = nv
+
+    def time_mode(self, mode):
+        from . import basis as rb
+        from ._lib import PREC_FP64, PREC_MIXED, ClassTiming, default_opts
+        L, h = self.L, self.ctx.h
+        o = default_opts(precision=PREC_MIXED if mode == "mixed" else PREC_FP64)
+        self.ctx.check(L.rchem_reset_class_timings(h))
+        self.ctx.check(L.rchem_set_profiling(h, 1))
+        rb.batch_scf(self.mols, "STO-3G", thresh_e=1e-10, diis=8, max_iter=self.iters, opts=o,
+                     ctx=self.ctx, damping=0.3, damping_until=1e-2)
+        self.ctx.check(L.rchem_set_profiling(h, 0))
+        t = (ClassTiming * 128)()
+        cnt = ctypes.c_int()
+        self.ctx.check(L.rchem_jk_class_timings(h, t, 128, ctypes.byref(cnt)))
+        return {(x.la, x.lb, x.lc, x.ld, x.fp32): x.ms for x in t[:cnt.value]}
+
+
 def tune(tuner):
     from ._lib import lib
     L = lib()
     L.rchem_load_tuning(tuner.ctx.h, None, None)  # start from the defaults
+    L.rchem_load_batch_tuning(tuner.ctx.h, None, None)  # a batch runs on those too
     base = tuner.time_all()
     keys = sorted(base)
     _log(f"defaults: {len(keys)} class kernels, {sum(base.values()):.2f} ms "
@@ -193,16 +229,27 @@ def write_table(path, best, device, workload):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--workload", choices=["h2o64", "h2o32", "eri300"], default="h2o64")
-    ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--workload", choices=["h2o64", "h2o32", "eri300", "batch512"],
+                    default="h2o64")
+    ap.add_argument("--out", default=None,
+                    help="table to write (default: data/tuning_gfx950.txt, for batch512 "
+                         "data/tuning_gfx950_batch.txt, beside the library)")
     ap.add_argument("--steps", type=int, default=3)
     args = ap.parse_args(argv)
     from . import basis as rb
-    from .inputs import nocc_of, synthetic_shells, water_cluster
+    from .inputs import nocc_of, random_molecules, synthetic_shells, water_cluster
 
+    if args.out is None:
+        args.out = DEFAULT_OUT_BATCH if args.workload == "batch512" else DEFAULT_OUT
     t0 = time.perf_counter()
     ctx = rb.Context(0)
-    if args.workload in ("h2o64", "h2o32"):
+    if args.workload == "batch512":
+        mols = random_molecules(count=512, seed=512)
+        tuner = BatchTuner(ctx, mols, args.steps)
+        workload = (f"batched SCF of 512 random H/C/N/O molecules (seed 512), STO-3G, "
+                    f"the first {args.steps} iterations")
+        _log(f"{workload}: {tuner.nvariants} launch shapes")
+    elif args.workload in ("h2o64", "h2o32"):
         dims = (4, 4, 4) if args.workload == "h2o64" else (2, 4, 4)
         Z, xyz = water_cluster(dims=dims, seed=64 if args.workload == "h2o64" else 32)
         B = rb.Basis.new(Z, xyz, "6-31G", ctx=ctx)
@@ -215,8 +262,9 @@ def main(argv=None):
         X = np.random.default_rng(2000).standard_normal((B.nbf, B.nbf)) / B.nbf
         D = X + X.T
         workload = "300-shell s/p/d microbenchmark, Schwarz 1e-12"
-    tuner = Tuner(ctx, B, D, args.steps)
-    _log(f"{workload}: nbf {B.nbf}, {tuner.nvariants} launch shapes")
+    if args.workload != "batch512":
+        tuner = Tuner(ctx, B, D, args.steps)
+        _log(f"{workload}: nbf {B.nbf}, {tuner.nvariants} launch shapes")
     base, best, final = tune(tuner)
     dev = os.environ.get("RCHEM_DEVICE_NAME", "gfx950")
     write_table(args.out, best, dev, workload)

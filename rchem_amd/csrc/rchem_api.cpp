@@ -369,6 +369,7 @@ struct rchem_ctx {
   // per (class, precision) launch choices of the Fock build, from the tuning
   // table (defaults where it has no entry)
   std::vector<ClassTune> tuning;
+  std::vector<ClassTune> tuning_batch;  // used in place of tuning by rchem_batch_scf (if any)
 
   bool profiling = false;
   bool lds_digestion = true;  // RCHEM_LDS_DIGESTION=0 selects global-atomic digestion
@@ -476,20 +477,22 @@ bool tune_valid(int variant, int blocks, int chunk_tiles) {
          chunk_tiles >= 1 && chunk_tiles <= 4096;
 }
 
-void set_tune(rchem_ctx* ctx, const ClassTune& t) {
-  for (ClassTune& x : ctx->tuning)
+void set_tune_in(std::vector<ClassTune>* table, const ClassTune& t) {
+  for (ClassTune& x : *table)
     if (x.la == t.la && x.lb == t.lb && x.lc == t.lc && x.ld == t.ld && x.fp32 == t.fp32) {
       x = t;
       return;
     }
-  ctx->tuning.push_back(t);
+  table->push_back(t);
 }
+
+void set_tune(rchem_ctx* ctx, const ClassTune& t) { set_tune_in(&ctx->tuning, t); }
 
 // Reads a tuning table: one line per (class, precision),
 //   la lb lc ld fp32 variant blocks chunk_tiles [ket_order]
 // with '#' starting a comment; lines that do not parse or are out of range are
 // skipped.  Returns the number of entries taken.
-int load_tuning(rchem_ctx* ctx, const std::string& path) {
+int load_tuning_into(std::vector<ClassTune>* table, const std::string& path) {
   FILE* f = std::fopen(path.c_str(), "r");
   if (!f) return 0;
   int taken = 0;
@@ -505,23 +508,27 @@ int load_tuning(rchem_ctx* ctx, const std::string& path) {
         t.ld < 0 || t.ld > kMaxL || (t.fp32 != 0 && t.fp32 != 1) ||
         !tune_valid(t.variant, t.blocks, t.chunk_tiles))
       continue;
-    set_tune(ctx, t);
+    set_tune_in(table, t);
     ++taken;
   }
   std::fclose(f);
   return taken;
 }
 
-// Where the library's own tuning table lies: data/tuning_gfx950.txt beside
-// the shared library (RCHEM_TUNING_FILE overrides; empty: no table).
-std::string tuning_path() {
-  if (const char* e = std::getenv("RCHEM_TUNING_FILE")) return e;
+int load_tuning(rchem_ctx* ctx, const std::string& path) {
+  return load_tuning_into(&ctx->tuning, path);
+}
+
+// Where one of the library's own tuning tables lies: data/<file> beside the
+// shared library (the environment variable env overrides; empty: no table).
+std::string tuning_path(const char* env, const char* file) {
+  if (const char* e = std::getenv(env)) return e;
   Dl_info info{};
-  if (dladdr(reinterpret_cast<void*>(&tuning_path), &info) == 0 || !info.dli_fname) return "";
+  if (dladdr(reinterpret_cast<void*>(&load_tuning_into), &info) == 0 || !info.dli_fname) return "";
   std::string so = info.dli_fname;
   const size_t slash = so.find_last_of('/');
   const std::string dir = slash == std::string::npos ? "." : so.substr(0, slash);
-  return dir + "/data/tuning_gfx950.txt";
+  return dir + "/data/" + file;
 }
 
 int set_device(rchem_ctx* ctx) {
@@ -1719,8 +1726,12 @@ int rchem_ctx_create(int device, rchem_ctx** out) {
   ctx->device = device;
   ctx->lds_block = prop.sharedMemPerBlock;
   {
-    const std::string tp = tuning_path();
+    // the Fock build's table, and the batched SCF's (multi-molecule bases of
+    // small molecules: a bra pair has few kets, so other shapes win)
+    const std::string tp = tuning_path("RCHEM_TUNING_FILE", "tuning_gfx950.txt");
     if (!tp.empty()) load_tuning(ctx, tp);
+    const std::string bp = tuning_path("RCHEM_TUNING_BATCH_FILE", "tuning_gfx950_batch.txt");
+    if (!bp.empty()) load_tuning_into(&ctx->tuning_batch, bp);
   }
   if (const char* e = std::getenv("RCHEM_LDS_DIGESTION")) ctx->lds_digestion = std::atoi(e) != 0;
   if (const char* e = std::getenv("RCHEM_ERI_GENERIC")) ctx->force_generic = std::atoi(e) != 0;
@@ -2003,6 +2014,14 @@ int rchem_load_tuning(rchem_ctx* ctx, const char* path, int* entries) {
   ctx->tuning.clear();
   ctx->plan_tau = -1.0;
   const int n = (path && *path) ? load_tuning(ctx, path) : 0;
+  if (entries) *entries = n;
+  return RCHEM_OK;
+}
+
+int rchem_load_batch_tuning(rchem_ctx* ctx, const char* path, int* entries) {
+  if (!ctx) return RCHEM_ERR_INVALID;
+  ctx->tuning_batch.clear();
+  const int n = (path && *path) ? load_tuning_into(&ctx->tuning_batch, path) : 0;
   if (entries) *entries = n;
   return RCHEM_OK;
 }
@@ -3369,6 +3388,18 @@ int rchem_batch_scf(rchem_ctx* ctx, int nmol, const int* natom, const int* Z, co
   TRY(check_opts(ctx, &opts->jk));
   TRY(check_scf_opts(ctx, opts));
   TRY(set_device(ctx));
+  // the batch's own launch table while it runs (none: the context's)
+  struct TableSwap {
+    rchem_ctx* c;
+    bool on;
+    TableSwap(rchem_ctx* x) : c(x), on(!x->tuning_batch.empty()) {
+      if (on) std::swap(c->tuning, c->tuning_batch);
+    }
+    ~TableSwap() {
+      if (on) std::swap(c->tuning, c->tuning_batch);
+      c->plan_tau = -1.0;
+    }
+  } table_swap(ctx);
   auto lower = [](std::string s) {
     for (auto& c : s) c = static_cast<char>(std::tolower(static_cast<unsigned char>(c)));
     return s;

@@ -92,6 +92,10 @@ def test_tuning_table_reader(gpu_ctx, tmp_path):
     assert (v.value, b.value, c.value) == (2, 5000, 128)
     assert L.rchem_set_class_tuning(h, 0, 0, 0, 0, 0, 1000, 1, 1) == 1
     assert L.rchem_load_tuning(h, None, ctypes.byref(cnt)) == 0 and cnt.value == 0
+    # the batched SCF's table: same reader
+    assert L.rchem_load_batch_tuning(h, str(p).encode(), ctypes.byref(cnt)) == 0
+    assert cnt.value == 4
+    assert L.rchem_load_batch_tuning(h, None, ctypes.byref(cnt)) == 0 and cnt.value == 0
     shipped = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                            "rchem_amd", "data", "tuning_gfx950.txt")
     if os.path.exists(shipped):
