@@ -64,6 +64,27 @@ __device__ __forceinline__ float fast_rsqrt(float x) {
   return y;
 }
 
+// Nearest grid point of T (0 <= T < kBoysTMax): its index k and d = T0 - T,
+// |d| <= 1/16.  Rounding by adding a power of two (2^52 in FP64, 2^23 in
+// FP32): the sum's low mantissa bits are k (read off the register, no
+// conversion) and subtracting the power again gives T0 exactly -- one add in
+// place of a float-to-int and an int-to-float conversion.  A tie rounds to
+// even, which keeps |d| <= 1/16.
+template <typename R>
+__device__ __forceinline__ void boys_grid(R T, int* k, R* d) {
+  if constexpr (sizeof(R) == 8) {
+    constexpr double kMagic = 4503599627370496.0;  // 2^52
+    const double r = __builtin_fma(T, kBoysInvH, kMagic);
+    *k = static_cast<int>(static_cast<unsigned>(__double_as_longlong(r)));
+    *d = (r - kMagic) * (1.0 / kBoysInvH) - T;
+  } else {
+    constexpr float kMagic = 8388608.0f;  // 2^23
+    const float r = __builtin_fmaf(T, static_cast<float>(kBoysInvH), kMagic);
+    *k = static_cast<int>(__float_as_uint(r) & 0x7fffffu);
+    *d = (r - kMagic) * static_cast<float>(1.0 / kBoysInvH) - T;
+  }
+}
+
 // Order of the table block a kernel of highest order M stages: for M = 1 the
 // order-0 block, whose Taylor polynomial and its derivative give F_0 and F_1
 // in one pass (dF_0/dT = -F_1), with no exponential.
@@ -76,8 +97,9 @@ __device__ __forceinline__ void boys_exact(R T, const R* __restrict__ table, R* 
   static_assert(M <= kBoysMaxM, "Boys order beyond the table");
   constexpr int K = sizeof(R) == 8 ? kBoysTerms : kBoysTermsF;
   if (T < R(kBoysTMax)) {
-    const int k = static_cast<int>(T * R(kBoysInvH) + R(0.5));
-    const R d = R(k) * (R(1) / R(kBoysInvH)) - T;  // T0 - T, |d| <= 1/16
+    int k;
+    R d;  // T0 - T, |d| <= 1/16
+    boys_grid<R>(T, &k, &d);
     // 0 <= k < kBoysRows: a 24-bit multiply (full rate; a 32-bit one is
     // quarter rate)
     const R* __restrict__ c = table + __umul24(static_cast<unsigned>(k), kBoysTerms);
@@ -147,8 +169,9 @@ __device__ __forceinline__ void boys_exact_select(R T, const R* __restrict__ tab
     constexpr int K = sizeof(R) == 8 ? kBoysTerms : kBoysTermsF;
     const bool near = T < R(kBoysTMax);
     const R Tn = near ? T : R(0);
-    const int k = static_cast<int>(Tn * R(kBoysInvH) + R(0.5));
-    const R d = R(k) * (R(1) / R(kBoysInvH)) - Tn;
+    int k;
+    R d;
+    boys_grid<R>(Tn, &k, &d);
     const R* __restrict__ c = table + __umul24(static_cast<unsigned>(k), kBoysTerms);
     const R rt = fast_rsqrt(near ? R(1) : T);
     const R f0 = R(0.88622692545275801365) * rt;
