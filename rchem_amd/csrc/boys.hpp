@@ -65,23 +65,27 @@ __device__ __forceinline__ float fast_rsqrt(float x) {
 }
 
 // Nearest grid point of T (0 <= T < kBoysTMax): its index k and d = T0 - T,
-// |d| <= 1/16.  Rounding by adding a power of two (2^52 in FP64, 2^23 in
-// FP32): the sum's low mantissa bits are k (read off the register, no
-// conversion) and subtracting the power again gives T0 exactly -- one add in
-// place of a float-to-int and an int-to-float conversion.  A tie rounds to
-// even, which keeps |d| <= 1/16.
+// |d| <= 1/16.  Rounding by adding the power of two whose ulp is the grid
+// spacing (2^52 h in FP64, 2^23 h in FP32): the sum r = kMagic + T0 has k in
+// its low mantissa bits (read off the register, no conversion), r - kMagic is
+// T0 exactly, and T0 - T is exact (a multiple of T's ulp below 1/16).  Three
+// adds with the constant in a scalar register, where a float-to-int and an
+// int-to-float conversion (or an FMA against a constant that must be rebuilt
+// in vector registers, gfx9 VOP3 taking no 64-bit literal) cost more.  A tie
+// rounds to even, which keeps |d| <= 1/16.
+static_assert(kBoysInvH == 8.0, "boys_grid's constants assume spacing 1/8");
 template <typename R>
 __device__ __forceinline__ void boys_grid(R T, int* k, R* d) {
   if constexpr (sizeof(R) == 8) {
-    constexpr double kMagic = 4503599627370496.0;  // 2^52
-    const double r = __builtin_fma(T, kBoysInvH, kMagic);
+    constexpr double kMagic = 562949953421312.0;  // 2^49: ulp 1/8 on [2^49, 2^50)
+    const double r = T + kMagic;
     *k = static_cast<int>(static_cast<unsigned>(__double_as_longlong(r)));
-    *d = (r - kMagic) * (1.0 / kBoysInvH) - T;
+    *d = (r - kMagic) - T;
   } else {
-    constexpr float kMagic = 8388608.0f;  // 2^23
-    const float r = __builtin_fmaf(T, static_cast<float>(kBoysInvH), kMagic);
+    constexpr float kMagic = 1048576.0f;  // 2^20: ulp 1/8 on [2^20, 2^21)
+    const float r = T + kMagic;
     *k = static_cast<int>(__float_as_uint(r) & 0x7fffffu);
-    *d = (r - kMagic) * static_cast<float>(1.0 / kBoysInvH) - T;
+    *d = (r - kMagic) - T;
   }
 }
 
