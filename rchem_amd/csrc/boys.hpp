@@ -108,10 +108,13 @@ __device__ __forceinline__ void boys_exact(R T, const R* __restrict__ table, R* 
     // quarter rate)
     const R* __restrict__ c = table + __umul24(static_cast<unsigned>(k), kBoysTerms);
     if constexpr (M == 1) {
-      // F_0(T0 - d) = sum_j c_j d^j (c_j = F_j(T0) / j!); F_1 = d/dd of it
-      R p = c[K - 1], q = R(0);
+      // F_0(T0 - d) = sum_j c_j d^j (c_j = F_j(T0) / j!); F_1 = d/dd of it.
+      // The derivative's chain starts at c_{K-1} (its first step, 0 d + p,
+      // is not left to the compiler, which may not drop a product with 0)
+      R q = c[K - 1];
+      R p = q * d + c[K - 2];
 #pragma unroll
-      for (int j = K - 2; j >= 0; --j) {
+      for (int j = K - 3; j >= 0; --j) {
         q = q * d + p;
         p = p * d + c[j];
       }
@@ -180,9 +183,10 @@ __device__ __forceinline__ void boys_exact_select(R T, const R* __restrict__ tab
     const R rt = fast_rsqrt(near ? R(1) : T);
     const R f0 = R(0.88622692545275801365) * rt;
     if constexpr (M == 1) {
-      R p = c[K - 1], q = R(0);
+      R q = c[K - 1];
+      R p = q * d + c[K - 2];
 #pragma unroll
-      for (int j = K - 2; j >= 0; --j) {
+      for (int j = K - 3; j >= 0; --j) {
         q = q * d + p;
         p = p * d + c[j];
       }

@@ -145,7 +145,7 @@ __device__ __forceinline__ void eri_compute(const EriArgs& a, const Tile t, cons
       for (int m = 0; m <= L; ++m) G[m] = pref * F[m];
       C::template vrr<R>(G, PAx, PAy, PAz, -roz * PQx, -roz * PQy, -roz * PQz, QCx, QCy, QCz,
                          roe * PQx, roe * PQy, roe * PQz, R(pb.oo2z), oo2e, R(0.5) * ooze, roz,
-                         roe, acc);
+                         roe, PQx, PQy, PQz, acc);
     };
     int i = 0;
     if constexpr (UNR == 2) {

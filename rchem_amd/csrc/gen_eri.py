@@ -28,6 +28,7 @@ Usage: gen_eri.py OUT_DIR [LMAX [KEY ...]]   (KEYs: classes to dispatch, default
 import json
 import math
 import os
+import re
 import sys
 
 AX = "xyz"
@@ -98,6 +99,7 @@ class VRR:
     def __init__(self, em):
         self.em = em
         self.memo = {}
+        self.terms = {}  # node name -> its summands (the line is their sum)
 
     def node(self, e, f, m):
         key = (e, f, m)
@@ -113,13 +115,13 @@ class VRR:
             i = min((k for k in range(3) if e[k] > 0), key=lambda k: (e[k] >= 2, k))
             e1 = dec(e, i)
             t1, t2 = self.node(e1, f, m), self.node(e1, f, m + 1)
-            expr = f"PA{AX[i]} * {t1} + WP{AX[i]} * {t2}"
+            terms = [f"PA{AX[i]} * {t1}", f"WP{AX[i]} * {t2}"]
             fl = 3
             if e1[i] > 0:
                 e2 = dec(e1, i)
                 t3, t4 = self.node(e2, f, m), self.node(e2, f, m + 1)
                 c = em.const("oo2z", e1[i])
-                expr += f" + {c} * ({t3} - roz * {t4})"
+                terms.append(f"{c} * ({t3} - roz * {t4})")
                 fl += 4
         else:
             # ket step on C: avoid the (f_i - 1) and electron-transfer terms when possible
@@ -127,25 +129,56 @@ class VRR:
                     key=lambda k: (2 * (f[k] >= 2) + (e[k] > 0), k))
             f1 = dec(f, i)
             t1, t2 = self.node(e, f1, m), self.node(e, f1, m + 1)
-            expr = f"QC{AX[i]} * {t1} + WQ{AX[i]} * {t2}"
+            terms = [f"QC{AX[i]} * {t1}", f"WQ{AX[i]} * {t2}"]
             fl = 3
             if f1[i] > 0:
                 f2 = dec(f1, i)
                 t3, t4 = self.node(e, f2, m), self.node(e, f2, m + 1)
                 c = em.const("oo2e", f1[i])
-                expr += f" + {c} * ({t3} - roe * {t4})"
+                terms.append(f"{c} * ({t3} - roe * {t4})")
                 fl += 4
             if e[i] > 0:
                 e1 = dec(e, i)
                 t5 = self.node(e1, f1, m + 1)
                 c = em.const("oo2ze", e[i])
-                expr += f" + {c} * {t5}"
+                terms.append(f"{c} * {t5}")
                 fl += 2
         name = f"v{tname(e)}_{tname(f)}_{m}"
-        em.lines.append(f"const R {name} = {expr};")
+        self.terms[name] = terms
+        em.lines.append(f"const R {name} = {' + '.join(terms)};")
         em.flops += fl
         self.memo[key] = name
         return name
+
+
+def fuse_accumulation(lines, terms, targets, L):
+    """The VRR lines with each target added into its accumulator.  A target no
+    other line reads is not formed: each of its summands is added into the
+    accumulator on its own (acc += x * y contracts to one FMA, so the sum's
+    final add is saved).  The others are formed and then added.  In the L = 1
+    class the bra step's WP_i G_1 = -PQ_i (roz G_1) takes one product roz G_1
+    for the three components in place of the three WP_i = -roz PQ_i."""
+    used = set()
+    for ln in lines:
+        used.update(re.findall(r"\bv\w+", ln.split("=", 1)[1]))
+    out = []
+    if L == 1:
+        out.append("const R H1 = roz * G[1];")
+    for ln in lines:
+        name = ln.split()[2]
+        if name in targets and name not in used:
+            acc = f"acc[{targets[name]}]"
+            for t in terms[name]:
+                m = re.fullmatch(r"WP([xyz]) \* G\[1\]", t)
+                if L == 1 and m:
+                    out.append(f"{acc} -= PQ{m.group(1)} * H1;")
+                else:
+                    out.append(f"{acc} += {t};")
+        else:
+            out.append(ln)
+    out += [f"acc[{i}] += {name};" for name, i in targets.items()
+            if name not in terms or name in used]
+    return out
 
 
 def gen_class(la, lb, lc, ld):
@@ -161,14 +194,13 @@ def gen_class(la, lb, lc, ld):
     # ---- VRR
     em = Emitter()
     v = VRR(em)
-    acc_lines = []
+    targets = {}
     for e in E:
         for f in F:
-            name = v.node(e, f, 0)
-            acc_lines.append(f"acc[{acc_index[(e, f)]}] += {name};")
+            targets[v.node(e, f, 0)] = acc_index[(e, f)]
             em.flops += 1
     vrr_flops = em.flops
-    vrr_body = list(em.consts.values()) + em.lines + acc_lines
+    vrr_body = list(em.consts.values()) + fuse_accumulation(em.lines, v.terms, targets, L)
 
     # ---- HRR (bra, then ket), memoised
     hl = []
@@ -456,10 +488,12 @@ def main():
         parts.append("  __device__ static __forceinline__ void vrr(const R* __restrict__ G,"
                      " R PAx, R PAy, R PAz, R WPx, R WPy, R WPz,")
         parts.append("      R QCx, R QCy, R QCz, R WQx, R WQy, R WQz,"
-                     " R oo2z, R oo2e, R oo2ze, R roz, R roe, R* __restrict__ acc) {")
+                     " R oo2z, R oo2e, R oo2ze, R roz, R roe, R PQx, R PQy, R PQz,")
+        parts.append("      R* __restrict__ acc) {")
         parts.append("    (void)PAx; (void)PAy; (void)PAz; (void)WPx; (void)WPy; (void)WPz;")
         parts.append("    (void)QCx; (void)QCy; (void)QCz; (void)WQx; (void)WQy; (void)WQz;")
         parts.append("    (void)oo2z; (void)oo2e; (void)oo2ze; (void)roz; (void)roe;")
+        parts.append("    (void)PQx; (void)PQy; (void)PQz;")
         parts.extend("    " + s for s in g["vrr"])
         parts.append("  }")
         if g["L"] >= STREAM_L:
