@@ -47,22 +47,21 @@ __host__ __device__ constexpr int boys_stage_size(int M) {
 
 enum BoysMode { kBoysExact = 0, kBoysCompat = 1 };
 
-// 1/sqrt(x) from the hardware estimate (measured on gfx950: 5.2e-8 relative
-// over [0.05, 2e4]) refined once.  FP64: one third-order (Halley) step,
-// y (1 + e/2 + 3e^2/8) with e = 1 - x y^2, whose error is O(e^3) -- five
-// operations in a chain of five, where two Newton steps take six in a chain
-// of seven; FP32: one Newton step.  ~1 ulp, a fraction of the cost of the
-// IEEE division and square root sequences.
+// 1/sqrt(x) from the hardware estimate.  FP64: the estimate (measured on
+// gfx950: 5.2e-8 relative over [0.05, 2e4]) refined by one third-order
+// (Halley) step, y (1 + e/2 + 3e^2/8) with e = 1 - x y^2, whose error is
+// O(e^3) -- five operations in a chain of five, where two Newton steps take
+// six in a chain of seven; ~1 ulp, a fraction of the cost of the IEEE
+// division and square root sequences.  FP32: the estimate as it is, within
+// 0.86 ulp (measured on gfx950 over [1e-3, 1e5] against FP64); one Newton
+// step in FP32 (four operations) measured 1.53 ulp, its own rounding
+// outweighing what it corrects.
 __device__ __forceinline__ double fast_rsqrt(double x) {
   const double y = __builtin_amdgcn_rsq(x);
   const double e = __builtin_fma(-x * y, y, 1.0);
   return __builtin_fma(y * e, __builtin_fma(0.375, e, 0.5), y);
 }
-__device__ __forceinline__ float fast_rsqrt(float x) {
-  float y = __builtin_amdgcn_rsqf(x);
-  y = y * __builtin_fmaf(-0.5f * x * y, y, 1.5f);
-  return y;
-}
+__device__ __forceinline__ float fast_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // Nearest grid point of T (0 <= T < kBoysTMax): its index k and d = T0 - T,
 // |d| <= 1/16.  Rounding by adding the power of two whose ulp is the grid
