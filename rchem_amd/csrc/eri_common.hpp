@@ -174,13 +174,14 @@ __device__ __forceinline__ const R* boys_src(const EriArgs& a) {
 
 // 8-fold weight: each unique (ab|cd) stands for deg of the ordered quartets;
 // the lower-half accumulators get deg/8 of each of the eight permutations'
-// contributions, and J + J^T, K + K^T restore the rest.
+// contributions, and J + J^T, K + K^T restore the rest.  The weight is
+// 2^(e - 3), e = [a != b] + [c != d] + [ab != cd], written straight into the
+// exponent field (integer operations in place of selects between doubles);
+// an inactive lane gets +0.
 __device__ __forceinline__ double symmetry_weight(const TileInfo& ti) {
-  double s = 0.125;
-  if (ti.bp.flags & 1) s *= 2.0;
-  if (ti.kp.flags & 1) s *= 2.0;
-  if (ti.bidx != ti.kidx) s *= 2.0;
-  return ti.active ? s : 0.0;
+  const int e = (ti.bp.flags & 1) + (ti.kp.flags & 1) + (ti.bidx != ti.kidx ? 1 : 0);
+  const int hi = ti.active ? (1023 - 3 + e) << 20 : 0;
+  return __hiloint2double(hi, 0);
 }
 
 // Element at a byte offset from a base: with a wave-uniform base (scalar
