@@ -268,3 +268,55 @@ def test_microbench_basis_sampled_quartets_every_class(gpu_ctx):
                                                            boys=O.BOYS_COMPAT)
         worst = max(worst, np.max(np.abs(blk - ref)))
     assert worst < 1e-12
+
+
+def test_boys_grid_edges_through_eris_vs_oracle(gpu_ctx):
+    """The exact Boys function at its grid's edge cases, reached through
+    integrals.  One-primitive s and p shells (exponent 1/2) share a centre at
+    x_k = sqrt(2 T_k) on a line, an s shell sits at the origin: (s_k s_k|s_0 s_0),
+    (p_k s_k|s_0 s_0) and (p_k p_k|s_0 s_0) then have zeta = eta = 1, P - Q = x_k
+    and T = rho |P - Q|^2 = T_k to a few ulps, in the orders 0, 1 and 2 (the
+    order-1 Taylor pass and the downward recursion).  T_k: midpoints between
+    grid points (the ties of the rounding to the 1/8 grid, met from either
+    side), grid points, 0, and either side of the switch to the asymptote at
+    T = 36.  A row index that disagrees with its grid point T0, or a wrong
+    switch, shows at once; an index one row off alone would not (the Taylor
+    series still converges there), so this pins the grid's bookkeeping, not
+    its rounding direction.  Bar: 1e-12 relative to the oracle's exact mode."""
+    T = [0.0, 1 / 16, 3 / 16, 15 / 16, 17 / 16, 127 / 16, 129 / 16, 255 / 16, 401 / 16,
+         573 / 16, 575 / 16, 0.125, 1.0, 12.0, 35.875, 36.0 - 1e-9, 36.0, 36.0 + 1e-9, 40.0,
+         90.0]
+    xs = np.sqrt(2.0 * np.array(T))
+    centres = [[0.0, 0.0, 0.0]]
+    ls = [0]
+    for x in xs:
+        centres += [[x, 0.0, 0.0], [x, 0.0, 0.0]]
+        ls += [0, 1]
+    centres = np.array(centres)
+    ls = np.array(ls, dtype=np.int32)
+    npr = np.ones(len(ls), dtype=np.int32)
+    ex = np.full(len(ls), 0.5)
+    cf = np.ones(len(ls))
+    _say(f"case: Boys grid edges, {len(T)} values of T through (ss|ss), (ps|ss), (pp|ss)")
+    B = rb.Basis.from_shells(centres, ls, npr, ex, cf, ctx=gpu_ctx)
+    q = []
+    for k in range(len(T)):
+        s, p = 1 + 2 * k, 2 + 2 * k
+        q += [(s, s, 0, 0), (p, p, 0, 0)]
+        if T[k] > 0:  # at T = 0 every centre coincides and (p s|s s) vanishes by symmetry
+            q.append((p, s, 0, 0))
+    q = np.array(q, dtype=np.int32)
+    blocks = rb.eri_shell_quartets(B, q, default_opts(boys_mode=BOYS_EXACT, schwarz_thresh=0.0))
+    Bo = O.cgtos_from_shells(centres, ls, npr, ex, cf)
+    f0 = np.concatenate([[0], np.cumsum((ls + 1) * (ls + 2) // 2)])
+    worst = 0.0
+    for r, blk in zip(q, blocks):
+        ref = np.zeros(blk.shape)
+        for a in range(blk.shape[0]):
+            for b in range(blk.shape[1]):
+                ref[a, b, 0, 0] = O.contracted_eri(Bo, f0[r[0]] + a, f0[r[1]] + b, f0[r[2]],
+                                                   f0[r[3]], boys=O.BOYS_EXACT)
+        scale = np.max(np.abs(ref))
+        assert scale > 1e-3  # every quartet is far from zero (F_0 >= 0.14 for T <= 40)
+        worst = max(worst, np.max(np.abs(blk - ref)) / scale)
+    assert worst < 1e-12
