@@ -156,7 +156,21 @@ __device__ __forceinline__ void eri_compute(const EriArgs& a, const Tile t, cons
         prim_quartet(pb1);
       }
     }
-    for (; i < bnp; ++i) prim_quartet(bprim[i * bps]);
+    if constexpr (sizeof(R) == 4 && UNR == 1) {
+      // FP32: the next bra record is loaded before this one's primitive
+      // quartet is evaluated, so its scalar-load latency overlaps the work
+      if (bnp > 0) {
+        P cur = bprim[0];
+        for (; i + 1 < bnp; ++i) {
+          const P nxt_b = bprim[(i + 1) * bps];
+          prim_quartet(cur);
+          cur = nxt_b;
+        }
+        prim_quartet(cur);
+      }
+    } else {
+      for (; i < bnp; ++i) prim_quartet(bprim[i * bps]);
+    }
   }
 
   R outr[C::NOUT];
