@@ -158,7 +158,10 @@ __device__ __forceinline__ void eri_compute(const EriArgs& a, const Tile t, cons
     }
     if constexpr (sizeof(R) == 4 && UNR == 1) {
       // FP32: the next bra record is loaded before this one's primitive
-      // quartet is evaluated, so its scalar-load latency overlaps the work
+      // quartet is evaluated, so its scalar-load latency overlaps the work.
+      // FP64 keeps the plain loop: there this form measured no faster, and
+      // in the spilling L = 5, 6 per-lane kernels (quartet mode) it gave wrong
+      // integrals, a cause not found; the FP32 kernels stop at L = 4.
       if (bnp > 0) {
         P cur = bprim[0];
         for (; i + 1 < bnp; ++i) {
